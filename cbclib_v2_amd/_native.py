@@ -1,0 +1,103 @@
+"""ctypes binding of ``libcbx.so`` (the C ABI declared in ``include/cbx.h``).
+
+The library is the only compute path of this package: there is no CPU
+fallback.  :func:`lib` raises ``RuntimeError`` when the shared library is
+missing, so a broken build fails loudly instead of silently running
+something else.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, 'libcbx.so')
+
+CBX_OK, CBX_EVALUE, CBX_EINDEX, CBX_ERUNTIME = 0, 1, 2, 3
+KERNELS = {'biweight': 0, 'gaussian': 1, 'parabolic': 2, 'rectangular': 3, 'triangular': 4}
+OVERLAPS = {'sum': 0, 'max': 1}
+DTYPES = {'bool': 0, 'int32': 1, 'int64': 2}
+
+EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32', 'cbx_draw_lines_f64',
+           'cbx_draw_pixels_f32', 'cbx_draw_pixels_f64', 'cbx_draw_lines_adjoint_f32',
+           'cbx_draw_lines_adjoint_f64', 'cbx_label', 'cbx_label_threshold_f32')
+
+
+class DrawOpts(ctypes.Structure):
+    """``cbx_draw_opts``: tile side, columns per walk checkpoint, deterministic flag."""
+    _fields_ = [('tile', ctypes.c_int32), ('seg', ctypes.c_int32), ('deterministic', ctypes.c_int32),
+                ('reserved', ctypes.c_int32)]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _declare(L: ctypes.CDLL) -> None:
+    P, I64, I32, F, D, C = (ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_float, ctypes.c_double,
+                            ctypes.c_char_p)
+    OP = ctypes.POINTER(DrawOpts)
+    L.cbx_last_error.restype = C
+    L.cbx_last_error.argtypes = []
+    L.cbx_version.restype = C
+    L.cbx_version.argtypes = []
+    L.cbx_kernel_id.restype = I32
+    L.cbx_kernel_id.argtypes = [C]
+    for name, R in (('f32', F), ('f64', D)):
+        fn = getattr(L, 'cbx_draw_lines_' + name)
+        fn.restype = I32
+        fn.argtypes = [P, I64, I64, I64, P, I64, P, R, I32, I32, OP, P]
+        fn = getattr(L, 'cbx_draw_pixels_' + name)
+        fn.restype = I32
+        fn.argtypes = [I64, I64, I64, P, I64, P, R, I32, I32, I32, P, P, P, P, P, I64,
+                       ctypes.POINTER(ctypes.c_int64), P]
+        fn = getattr(L, 'cbx_draw_lines_adjoint_' + name)
+        fn.restype = I32
+        fn.argtypes = [P, I64, I64, I64, P, I64, P, R, I32, P, OP, P]
+    L.cbx_label.restype = I32
+    L.cbx_label.argtypes = [P, I32, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
+    L.cbx_label_threshold_f32.restype = I32
+    L.cbx_label_threshold_f32.argtypes = [P, F, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
+
+
+def lib() -> ctypes.CDLL:
+    """The loaded library; raises RuntimeError if it has not been built."""
+    global _lib
+    if _lib is None:
+        with _lock:
+            if _lib is None:
+                if not os.path.exists(LIB_PATH):
+                    raise RuntimeError(f"cbclib_v2_amd: the HIP library {LIB_PATH} is not built; run "
+                                       "`python -c 'import __graft_entry__ as g; g.build()'` or "
+                                       "`make -C cbclib_v2_amd/csrc`")
+                L = ctypes.CDLL(LIB_PATH)
+                _declare(L)
+                _lib = L
+    return _lib
+
+
+def check(status: int) -> None:
+    """Raise the exception the reference raises for a failed call."""
+    if status == CBX_OK:
+        return
+    msg = lib().cbx_last_error().decode(errors='replace')
+    if status == CBX_EVALUE:
+        raise ValueError(msg)
+    if status == CBX_EINDEX:
+        raise IndexError(msg)
+    raise RuntimeError(msg)
+
+
+def kernel_id(kernel: str) -> int:
+    if kernel not in KERNELS:
+        # kernels::get_type (include.hpp:495-503)
+        raise ValueError(f"kernel is missing for {kernel}")
+    return KERNELS[kernel]
+
+
+def overlap_id(overlap: str) -> int:
+    if overlap not in OVERLAPS:
+        # bresenham.cpp:148
+        raise ValueError(f"Invalid overlap keyword: {overlap}")
+    return OVERLAPS[overlap]

@@ -1,0 +1,336 @@
+// Device core shared by the draw_lines kernels: per-line setup, the exact
+// walk, the pixel error and the five profiles.
+//
+// Exactness contract.  A pixel belongs to a segment's footprint iff the
+// reference's float walk emits it with error <= 1
+// (cbclib_v2/_src/src/bresenham.hpp:352-383, bresenham.cpp:80-86).  That set
+// is decided by float values accumulated one rounded addition at a time, so
+// the kernels here replay the same additions in the same order (no FMA
+// contraction: the library is built with -ffp-contract=off) and evaluate the
+// same predicates.  Divisions by the segment length and by w^2 use a
+// precomputed correctly-rounded reciprocal and one FMA correction step, which
+// returns the correctly-rounded quotient (Markstein's theorem), i.e. exactly
+// what the reference's IEEE division returns.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cbx {
+
+enum Profile { K_BIWEIGHT = 0, K_GAUSSIAN = 1, K_PARABOLIC = 2, K_RECTANGULAR = 3, K_TRIANGULAR = 4 };
+
+// Longest walk accepted (steps along the long axis) and the largest |pixel
+// coordinate| a walk end point may have.
+constexpr int64_t MAX_WALK = int64_t(1) << 26;
+constexpr double MAX_COORD = 1e9;
+
+// error flags raised by the device passes
+constexpr int FLAG_INDEX = 1;     // idxs outside [0, n_frames)
+constexpr int FLAG_LONG = 2;      // walk longer than MAX_WALK or end point beyond MAX_COORD
+constexpr int FLAG_STUCK = 4;     // a walk did not leave its segment within its step budget
+
+// ---------------------------------------------------------------------------
+// type-exact math (float stays float: roundf/ceilf/sqrtf of the reference)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float m_abs(float x) { return __builtin_fabsf(x); }
+__device__ __forceinline__ double m_abs(double x) { return __builtin_fabs(x); }
+__device__ __forceinline__ float m_round(float x) { return __builtin_roundf(x); }
+__device__ __forceinline__ double m_round(double x) { return __builtin_round(x); }
+__device__ __forceinline__ float m_ceil(float x) { return __builtin_ceilf(x); }
+__device__ __forceinline__ double m_ceil(double x) { return __builtin_ceil(x); }
+__device__ __forceinline__ float m_sqrt(float x) { return __fsqrt_rn(x); }
+__device__ __forceinline__ double m_sqrt(double x) { return __dsqrt_rn(x); }
+
+// ---------------------------------------------------------------------------
+// correctly rounded quotient from a correctly rounded reciprocal
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float div_rn(float a, float b, float rb)
+{
+    float q = a * rb;
+    float r = __builtin_fmaf(-q, b, a);
+    float q1 = __builtin_fmaf(r, rb, q);
+    // the correction needs a normal quotient; tiny numerators take the IEEE path
+    return (__builtin_fabsf(q) < 1e-30f) ? a / b : q1;
+}
+
+__device__ __forceinline__ double div_rn(double a, double b, double rb)
+{
+    double q = a * rb;
+    double r = __builtin_fma(-q, b, a);
+    double q1 = __builtin_fma(r, rb, q);
+    return (__builtin_fabs(q) < 1e-290) ? a / b : q1;
+}
+
+// ---------------------------------------------------------------------------
+// per-line constants (everything the walk, the fills and the error need)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct alignas(16) LineSetup {
+    T t[2];        // tangent p1 - p0 (geometry.hpp:316)
+    T nv[2];       // normal derror (y1 - y0, x0 - x1) (bresenham.hpp:310-316)
+    T len, rlen;   // |t| (float sqrt of the float magnitude) and RN(1/len)
+    T LL;          // RN(len * len) (TangentError::error_at)
+    T w, w2;       // width and RN(w * w)
+    double rw2;    // RN64(1 / (double)w2)
+    int32_t m1[2]; // walk end (exclusive, "any coordinate equal" test)
+    int32_t m0[2]; // walk start
+    int32_t frame;
+    int8_t axis;   // long axis (0 = x, 1 = y)
+    int8_t st[2];  // step per axis (+1 / -1)
+    int8_t valid;  // 0: the line emits nothing
+    T e0[2];       // tangent and normal errors at m0
+    T p0[2];       // first end point (for the adjoint's closed-form derivatives)
+};
+
+// Builds the setup exactly as BresenhamPlotter's constructor does
+// (bresenham.hpp:243-253, 303-347).  Returns a FLAG_* on a rejected line.
+template <typename T>
+__device__ __forceinline__ int line_setup(const T *ln, LineSetup<T> &L)
+{
+    T x0 = ln[0], y0 = ln[1], x1 = ln[2], y1 = ln[3], w = ln[4];
+    T p0[2] = {x0, y0}, p1[2] = {x1, y1};
+    L.t[0] = x1 - x0;
+    L.t[1] = y1 - y0;
+    L.nv[0] = y1 - y0;
+    L.nv[1] = x0 - x1;
+    L.w = w;
+    L.p0[0] = x0;
+    L.p0[1] = y0;
+    L.valid = 0;
+    // error() is +inf for w <= 0 and NaN never compares true: nothing emitted
+    if (!(w > T(0))) return 0;
+    T mag = T(0);
+    mag += L.t[0] * L.t[0];
+    mag += L.t[1] * L.t[1];
+    T len = m_sqrt(mag);
+    if (!(len > T(0)) || !__builtin_isfinite(len) || !__builtin_isfinite(w)) return 0;
+    if (!__builtin_isfinite(x0) || !__builtin_isfinite(y0) || !__builtin_isfinite(x1) || !__builtin_isfinite(y1))
+        return 0;
+    L.len = len;
+    L.rlen = T(1) / len;
+    L.LL = len * len;
+    L.w2 = w * w;
+    L.rw2 = 1.0 / (double)L.w2;
+    int axis = (m_abs(L.t[0]) < m_abs(L.t[1])) ? 1 : 0;
+    L.axis = (int8_t)axis;
+    long offset = (long)(m_ceil(w) + T(1));
+    T tau[2] = {L.t[0] / len, L.t[1] / len};
+    T sc = m_abs((T)offset / tau[axis]);
+    for (int i = 0; i < 2; i++) {
+        int st = (p1[i] >= p0[i]) ? 1 : -1;
+        T a0 = m_round(p0[i] - tau[i] * sc);
+        T a1 = m_round(p1[i] + tau[i] * sc);
+        if (!(m_abs(a0) < T(MAX_COORD)) || !(m_abs(a1) < T(MAX_COORD))) return FLAG_LONG;
+        L.st[i] = (int8_t)st;
+        L.m0[i] = (int32_t)a0;
+        L.m1[i] = (int32_t)a1 + st;
+    }
+    int64_t span = (int64_t)L.m1[axis] - (int64_t)L.m0[axis];
+    if (span > MAX_WALK || span < -MAX_WALK) return FLAG_LONG;
+    T d0 = (T)L.m0[0] - x0, d1 = (T)L.m0[1] - y0;
+    T Te = T(0);
+    Te += L.t[0] * d0;
+    Te += L.t[1] * d1;
+    T Ne = T(0);
+    Ne += L.nv[0] * d0;
+    Ne += L.nv[1] * d1;
+    L.e0[0] = Te;
+    L.e0[1] = Ne;
+    L.valid = 1;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// walk state and one step of LineIterator (bresenham.hpp:99-112, 149-154,
+// 198-222)
+// ---------------------------------------------------------------------------
+template <typename T>
+struct WalkState {
+    int32_t c[2];
+    T Te, Ne;
+};
+
+// next[] decision of NormalError::is_next for the pair (x, y).
+template <typename T>
+__device__ __forceinline__ void walk_next(const LineSetup<T> &L, T Ne, bool &nx, bool &ny)
+{
+    T dx = (L.st[0] > 0) ? L.nv[0] : -L.nv[0];
+    T dy = (L.st[1] > 0) ? L.nv[1] : -L.nv[1];
+    T ex = Ne + dx;
+    T ey = Ne + dy;
+    T exy = ex + dy;
+    T aex = m_abs(ex), aey = m_abs(ey), aexy = m_abs(exy);
+    if (aey < aex) {
+        nx = aexy < aey;
+        ny = true;
+    } else {
+        nx = true;
+        ny = aexy < aex;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void walk_step(const LineSetup<T> &L, WalkState<T> &s, bool nx, bool ny)
+{
+    if (nx) {
+        s.c[0] += L.st[0];
+        s.Te += (L.st[0] > 0) ? L.t[0] : -L.t[0];
+        s.Ne += (L.st[0] > 0) ? L.nv[0] : -L.nv[0];
+    }
+    if (ny) {
+        s.c[1] += L.st[1];
+        s.Te += (L.st[1] > 0) ? L.t[1] : -L.t[1];
+        s.Ne += (L.st[1] > 0) ? L.nv[1] : -L.nv[1];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ bool walk_done(const LineSetup<T> &L, const WalkState<T> &s)
+{
+    return s.c[0] == L.m1[0] || s.c[1] == L.m1[1];
+}
+
+// ---------------------------------------------------------------------------
+// pixel error (BresenhamPlotter::error / normal_error, bresenham.hpp:279-291)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float nsq_of(const LineSetup<float> &L, float Ne)
+{
+    float qn = div_rn(Ne, L.len, L.rlen);
+    return qn * qn;   // == (float)((double)qn * qn): the double square is exact
+}
+
+__device__ __forceinline__ double nsq_of(const LineSetup<double> &L, double Ne)
+{
+    double qn = div_rn(Ne, L.len, L.rlen);
+    return qn * qn;
+}
+
+__device__ __forceinline__ float err_of(const LineSetup<float> &L, float Te, float Ne)
+{
+    float a = -Te, b = Te - L.LL;
+    float m = (a < b) ? b : a;
+    float at = (m < 0.0f) ? 0.0f : m;
+    float qt = div_rn(at, L.len, L.rlen);
+    float nsq = nsq_of(L, Ne);
+    // RN64(qt^2 + nsq): the square of a float is exact in double, so one FMA
+    // rounds exactly like the reference's double addition
+    double e = __builtin_fma((double)qt, (double)qt, (double)nsq);
+    return (float)div_rn(e, (double)L.w2, L.rw2);
+}
+
+__device__ __forceinline__ double err_of(const LineSetup<double> &L, double Te, double Ne)
+{
+    double a = -Te, b = Te - L.LL;
+    double m = (a < b) ? b : a;
+    double at = (m < 0.0) ? 0.0 : m;
+    double qt = div_rn(at, L.len, L.rlen);
+    double nsq = nsq_of(L, Ne);
+    double qt2 = qt * qt;
+    double e = qt2 + nsq;
+    return div_rn(e, L.w2, L.rw2);
+}
+
+// ---------------------------------------------------------------------------
+// profiles (include.hpp:446-468): the float versions keep the reference's
+// promotions (std::pow(float, int) is a double square, std::max<T> rounds
+// back to T)
+// ---------------------------------------------------------------------------
+template <int K>
+__device__ __forceinline__ float profile(float x)
+{
+    if constexpr (K == K_BIWEIGHT) {
+        float m = (float)(1.0 - (double)x * (double)x);
+        m = (m < 0.0f) ? 0.0f : m;
+        return (float)(0.9375 * ((double)m * (double)m));
+    } else if constexpr (K == K_GAUSSIAN) {
+        if (__builtin_fabsf(x) <= 1.0f) {
+            float x3 = 3.0f * x;
+            double e = ::exp(-((double)x3 * (double)x3) / 2.0);
+            return (float)(0.3989422804014327 * e);
+        }
+        return 0.0f;
+    } else if constexpr (K == K_PARABOLIC) {
+        float m = (float)(1.0 - (double)x * (double)x);
+        m = (m < 0.0f) ? 0.0f : m;
+        return 0.75f * m;
+    } else if constexpr (K == K_RECTANGULAR) {
+        return (__builtin_fabsf(x) <= 1.0f) ? 1.0f : 0.0f;
+    } else {
+        float m = 1.0f - __builtin_fabsf(x);
+        return (m < 0.0f) ? 0.0f : m;
+    }
+}
+
+template <int K>
+__device__ __forceinline__ double profile(double x)
+{
+    if constexpr (K == K_BIWEIGHT) {
+        double m = 1.0 - x * x;
+        m = (m < 0.0) ? 0.0 : m;
+        return 0.9375 * (m * m);
+    } else if constexpr (K == K_GAUSSIAN) {
+        if (__builtin_fabs(x) <= 1.0) {
+            double x3 = 3.0 * x;
+            return 0.3989422804014327 * ::exp(-(x3 * x3) / 2.0);
+        }
+        return 0.0;
+    } else if constexpr (K == K_PARABOLIC) {
+        double m = 1.0 - x * x;
+        m = (m < 0.0) ? 0.0 : m;
+        return 0.75 * m;
+    } else if constexpr (K == K_RECTANGULAR) {
+        return (__builtin_fabs(x) <= 1.0) ? 1.0 : 0.0;
+    } else {
+        double m = 1.0 - __builtin_fabs(x);
+        return (m < 0.0) ? 0.0 : m;
+    }
+}
+
+// value of an emitted pixel: max_val * K(sqrt(err)) (bresenham.cpp:84)
+template <int K>
+__device__ __forceinline__ float pixel_value(float err, float max_val)
+{
+    return max_val * profile<K>(__fsqrt_rn(err));
+}
+
+template <int K>
+__device__ __forceinline__ double pixel_value(double err, double max_val)
+{
+    return max_val * profile<K>(__dsqrt_rn(err));
+}
+
+// dK/du at u = err = (dist / w)^2 for the adjoint; the triangular profile is
+// handled through dist in the caller.
+template <int K>
+__device__ __forceinline__ float profile_du(float u, float kval)
+{
+    if constexpr (K == K_BIWEIGHT) return -1.875f * (1.0f - u);
+    else if constexpr (K == K_PARABOLIC) return -0.75f;
+    else if constexpr (K == K_GAUSSIAN) return -4.5f * kval;
+    else return 0.0f;
+}
+
+// order-preserving encodings for max accumulation in LDS integer atomics
+__device__ __forceinline__ uint32_t ord_enc(float v)
+{
+    uint32_t b = __float_as_uint(v);
+    return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord_dec(uint32_t e)
+{
+    uint32_t b = (e & 0x80000000u) ? (e & 0x7fffffffu) : ~e;
+    return __uint_as_float(b);
+}
+__device__ __forceinline__ unsigned long long ord_enc(double v)
+{
+    unsigned long long b = (unsigned long long)__double_as_longlong(v);
+    return (b & 0x8000000000000000ull) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double ord_dec(unsigned long long e)
+{
+    unsigned long long b = (e & 0x8000000000000000ull) ? (e & 0x7fffffffffffffffull) : ~e;
+    return __longlong_as_double((long long)b);
+}
+
+} // namespace cbx

@@ -1,0 +1,1076 @@
+// draw_lines on gfx950: segment checkpoints, tile-binned LDS accumulation,
+// pixel lists, the deterministic (reference-order) mode and the adjoint.
+//
+// Data flow of one call (F frames of H x W, N lines):
+//
+//  1. k_segments<pass 1>   one lane per line: builds the line's constants
+//                          (LineSetup) and replays the reference's walk
+//                          once, cutting it into segments of <= SEG
+//                          long-axis columns that lie inside the frame.  A
+//                          segment stores the exact walk state at its first
+//                          column (the checkpoint) and the short-axis pixel
+//                          range its fills can reach.  Counts segments per
+//                          line and (segment, tile) items per tile.
+//  2. exclusive scans      segment offsets per line, item offsets per tile.
+//  3. k_segments<pass 2>   same walk, writes segments and the per-tile item
+//                          lists.
+//  4. k_tiles              one workgroup per TILE x TILE pixel tile: waves
+//                          resume each item's walk from its checkpoint
+//                          (phase 1, one lane per item, column records into
+//                          LDS), then fill the columns (phase 2, one lane per
+//                          column), accumulating into an LDS tile with LDS
+//                          atomics; the tile is flushed to HBM once with
+//                          coalesced read-modify-write.  HBM traffic of the
+//                          image: one read and one write per touched tile.
+//
+// The pixel-list and deterministic modes replace step 4 by an emission pass
+// over segments (k_emit) that writes one entry per emitted pixel in the
+// reference's emission order, a stable radix sort by pixel and an ordered
+// segmented reduction.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <climits>
+#include <cstring>
+#include <string>
+#include <type_traits>
+
+#include "cbx_core.hpp"
+#include "cbx_host.hpp"
+
+namespace cbx {
+
+template <typename T>
+struct Segment {
+    int32_t line;
+    int32_t c[2];   // walk position at the segment's first column
+    int32_t ncols;  // long-axis columns covered from c[axis] in the walk direction
+    T Te, Ne;       // walk errors at that position
+    int32_t lo, hi; // short-axis pixel range the fills can reach, clipped to the frame
+};
+
+struct Geometry {
+    int64_t n_frames;
+    int32_t H, W;
+    int32_t tile, seg;
+    int32_t ntx, nty;
+};
+
+// ---------------------------------------------------------------------------
+// 1/3. segment pre-pass
+// ---------------------------------------------------------------------------
+template <typename T, typename I, bool WRITE, bool BIN>
+__global__ void __launch_bounds__(256)
+k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_lines, Geometry g,
+           LineSetup<T> *__restrict__ setup, int *__restrict__ seg_cnt, int *__restrict__ tile_cnt,
+           const int *__restrict__ seg_off, Segment<T> *__restrict__ segs, const int *__restrict__ tile_off,
+           int *__restrict__ tile_cur, int *__restrict__ items, int *__restrict__ flags)
+{
+    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n_lines) return;
+    LineSetup<T> L;
+    if (!WRITE) {
+        int rc = line_setup(lines + 5 * l, L);
+        int64_t f = idxs ? (int64_t)idxs[l] : ((g.n_frames == 1) ? 0 : l);
+        if (f < 0 || f >= g.n_frames) {
+            atomicOr(flags, FLAG_INDEX);
+            L.valid = 0;
+        }
+        if (rc) {
+            atomicOr(flags, rc);
+            L.valid = 0;
+        }
+        L.frame = (int32_t)f;
+        setup[l] = L;
+        if (!L.valid) {
+            seg_cnt[l] = 0;
+            return;
+        }
+    } else {
+        L = setup[l];
+        if (!L.valid) return;
+    }
+
+    const int a = L.axis, b = 1 - a;
+    const int da = (a == 0) ? g.W : g.H, db = (a == 0) ? g.H : g.W;
+    const int sa = L.st[a];
+    const double ta = (double)m_abs(L.t[a]);
+    const double wl = (double)L.w * (double)L.len;
+
+    WalkState<T> s;
+    s.c[0] = L.m0[0];
+    s.c[1] = L.m0[1];
+    s.Te = L.e0[0];
+    s.Ne = L.e0[1];
+
+    const int base = WRITE ? seg_off[l] : 0;
+    int nseg = 0;
+    bool open = false;
+    int blk = 0, tl = 0, nem = 0, smin = 0, smax = 0;
+    double maxN = 0.0;
+    Segment<T> cur;
+
+    auto close = [&]() {
+        if (nem == 0) return;
+        // fills stop once |N| / len >= w; |N| grows by |t_a| per short step
+        int R = (int)((wl + maxN) / ta) + 2;
+        int lo = max(smin - R, 0), hi = min(smax + R, db - 1);
+        if (lo > hi) return;
+        cur.lo = lo;
+        cur.hi = hi;
+        int sid = base + nseg;
+        if (WRITE) segs[sid] = cur;
+        if (BIN) {
+            for (int tb = lo / g.tile; tb <= hi / g.tile; tb++) {
+                int tx = (a == 0) ? tl : tb, ty = (a == 0) ? tb : tl;
+                int64_t tid = ((int64_t)L.frame * g.nty + ty) * g.ntx + tx;
+                if (!WRITE) atomicAdd(&tile_cnt[tid], 1);
+                else items[tile_off[tid] + atomicAdd(&tile_cur[tid], 1)] = sid;
+            }
+        }
+        nseg++;
+    };
+
+    // every step moves the walk; a walk takes at most one short-axis-only
+    // step per long-axis step, so this budget is never reached by a valid line
+    const int64_t budget = 4 * ((int64_t)abs(L.m1[a] - L.m0[a]) + abs(L.m1[b] - L.m0[b])) + 64;
+    for (int64_t it = 0;; it++) {
+        if (it == budget) {
+            atomicOr(flags, FLAG_STUCK);
+            break;
+        }
+        if (walk_done(L, s)) break;
+        const int c = s.c[a];
+        if ((sa > 0) ? (c >= da) : (c < 0)) break; // no in-frame column is left
+        bool nx, ny;
+        walk_next(L, s.Ne, nx, ny);
+        if (c >= 0 && c < da) {
+            const int bk = c / g.seg;
+            if (!open || bk != blk) {
+                if (open) close();
+                open = true;
+                blk = bk;
+                tl = c / g.tile;
+                nem = 0;
+                maxN = 0.0;
+                smin = INT_MAX;
+                smax = INT_MIN;
+                cur.line = (int32_t)l;
+                cur.c[0] = s.c[0];
+                cur.c[1] = s.c[1];
+                cur.Te = s.Te;
+                cur.Ne = s.Ne;
+                const int b0 = bk * g.seg, b1 = min(b0 + g.seg, da);
+                cur.ncols = (sa > 0) ? (b1 - c) : (c - b0 + 1);
+            }
+            if ((a == 0) ? nx : ny) {
+                nem++;
+                const int sb = s.c[b];
+                smin = min(smin, sb);
+                smax = max(smax, sb);
+                maxN = fmax(maxN, fabs((double)s.Ne));
+            }
+        }
+        walk_step(L, s, nx, ny);
+    }
+    if (open) close();
+    if (!WRITE) seg_cnt[l] = nseg;
+}
+
+// ---------------------------------------------------------------------------
+// shared pieces of the tile kernels
+// ---------------------------------------------------------------------------
+template <typename T>
+struct ColRec {
+    int32_t s; // short-axis centre, INT_MIN when the column is not emitted
+    T Te, Ne;
+};
+
+struct ItemHdr {
+    int32_t seg;
+    int32_t line;
+};
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Phase 1: replay a segment's walk from its checkpoint and record, for each
+// of its columns, the centre and errors at which the reference emits it.
+template <typename T, int SEG>
+__device__ __forceinline__ void walk_columns(const LineSetup<T> &L, const Segment<T> &sg, ColRec<T> *rr, int *flags)
+{
+    for (int k = 0; k < SEG; k++) rr[k].s = INT_MIN;
+    const int a = L.axis, b = 1 - a, sa = L.st[a];
+    WalkState<T> s;
+    s.c[0] = sg.c[0];
+    s.c[1] = sg.c[1];
+    s.Te = sg.Te;
+    s.Ne = sg.Ne;
+    const int c0 = s.c[a];
+    int it = 0;
+    for (; it < 4 * SEG + 16; it++) {
+        if (walk_done(L, s)) break;
+        const int k = (s.c[a] - c0) * sa;
+        if (k >= sg.ncols) break;
+        bool nx, ny;
+        walk_next(L, s.Ne, nx, ny);
+        if ((a == 0) ? nx : ny) {
+            rr[k].s = s.c[b];
+            rr[k].Te = s.Te;
+            rr[k].Ne = s.Ne;
+        }
+        walk_step(L, s, nx, ny);
+    }
+    if (it == 4 * SEG + 16) atomicOr(flags, FLAG_STUCK);
+}
+
+// Phase 2: the orthogonal fills of one emitted column (bresenham.hpp:364-380),
+// restricted to the short-axis window [s_lo, s_hi); emit(s, err) is called
+// for every pixel of the window the reference would hand to draw_pixel.
+template <typename T, typename Emit>
+__device__ __forceinline__ void fill_column(const LineSetup<T> &L, const ColRec<T> &cr, int s_lo, int s_hi,
+                                            Emit &&emit)
+{
+    const int b = 1 - L.axis;
+    if (cr.s >= s_lo && cr.s < s_hi) emit(cr.s, cr.Te, cr.Ne);
+    const T tb = L.t[b], nb = L.nv[b];
+    const int sb = L.st[b];
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        const int dir = half ? -sb : sb;
+        const T dT = (dir > 0) ? tb : -tb, dN = (dir > 0) ? nb : -nb;
+        int s = cr.s;
+        T Te = cr.Te, Ne = cr.Ne;
+        for (;;) {
+            s += dir;
+            Te += dT;
+            Ne += dN;
+            if (!(nsq_of(L, Ne) < L.w2)) break;
+            if (s >= s_lo && s < s_hi) emit(s, Te, Ne);
+            else if ((dir > 0) ? (s >= s_hi) : (s < s_lo)) break;
+        }
+    }
+}
+
+template <typename T, int OV>
+struct AccType {
+    using type = T;
+};
+template <>
+struct AccType<float, 1> {
+    using type = uint32_t;
+};
+template <>
+struct AccType<double, 1> {
+    using type = unsigned long long;
+};
+
+// ---------------------------------------------------------------------------
+// 4. tile kernel (image mode)
+// ---------------------------------------------------------------------------
+template <typename T, int K, int OV, int TILE, int SEG, int NT, int CH>
+__global__ void __launch_bounds__(NT)
+k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const int *__restrict__ items,
+        const Segment<T> *__restrict__ segs, const LineSetup<T> *__restrict__ setup, T max_val,
+        int *__restrict__ flags)
+{
+    using Acc = typename AccType<T, OV>::type;
+    constexpr int NW = NT / 64;
+    __shared__ Acc img[TILE * TILE];
+    __shared__ ColRec<T> rec[NW][CH * SEG];
+    __shared__ ItemHdr hdr[NW][CH];
+
+    const int tile = blockIdx.x;
+    const int beg = tile_off[tile], end = tile_off[tile + 1];
+    if (beg == end) return;
+    const int per_frame = g.ntx * g.nty;
+    const int64_t f = tile / per_frame;
+    const int r = tile - (int)f * per_frame;
+    const int ty = r / g.ntx, tx = r - ty * g.ntx;
+    const int X0 = tx * TILE, Y0 = ty * TILE;
+    const int X1 = min(X0 + TILE, g.W), Y1 = min(Y0 + TILE, g.H);
+
+    for (int i = threadIdx.x; i < TILE * TILE; i += NT) {
+        if constexpr (OV == 0) img[i] = T(-0.0); // additive identity, keeps -0.0 pixels intact
+        else img[i] = Acc(0);                     // below every encoded value
+    }
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    ColRec<T> *R = rec[wave];
+    for (int first = beg + wave * CH; first < end; first += NW * CH) {
+        const int n = min(CH, end - first);
+        if (lane < n) {
+            const int sid = items[first + lane];
+            const Segment<T> sg = segs[sid];
+            const LineSetup<T> L = setup[sg.line];
+            hdr[wave][lane].seg = sid;
+            hdr[wave][lane].line = sg.line;
+            walk_columns<T, SEG>(L, sg, R + lane * SEG, flags);
+        }
+        wave_sync();
+        for (int u = lane; u < n * SEG; u += 64) {
+            const int i = u / SEG, k = u - i * SEG;
+            const ColRec<T> cr = R[u];
+            if (cr.s == INT_MIN) continue;
+            const Segment<T> &sg = segs[hdr[wave][i].seg];
+            const LineSetup<T> &L = setup[hdr[wave][i].line];
+            const int a = L.axis;
+            const int c = sg.c[a] + k * L.st[a];
+            const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
+            fill_column(L, cr, s_lo, s_hi, [&](int s, T Te, T Ne) {
+                const T err = err_of(L, Te, Ne);
+                if (!(err <= T(1))) return;
+                const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
+                const T v = pixel_value<K>(err, max_val);
+                Acc *px = &img[(y - Y0) * TILE + (x - X0)];
+                if constexpr (OV == 0) {
+                    atomicAdd(px, v);
+                } else {
+                    if (v == v) atomicMax(px, ord_enc(v));
+                }
+            });
+        }
+        wave_sync();
+    }
+    __syncthreads();
+
+    // flush: one coalesced read-modify-write of the tile
+    T *fo = out + (f * g.H) * (int64_t)g.W;
+    for (int i = threadIdx.x; i < TILE * TILE; i += NT) {
+        const int y = Y0 + i / TILE, x = X0 + (i % TILE);
+        if (y >= Y1 || x >= X1) continue;
+        T *o = fo + (int64_t)y * g.W + x;
+        if constexpr (OV == 0) {
+            *o = *o + img[i];
+        } else {
+            const Acc e = img[i];
+            if (e != Acc(0)) {
+                const T m = ord_dec(e);
+                const T ov = *o;
+                *o = (ov < m) ? m : ov; // std::max(out, value)
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// emission pass (pixel lists and the deterministic mode)
+// ---------------------------------------------------------------------------
+template <typename T, int K, bool WRITE>
+__global__ void __launch_bounds__(256)
+k_emit(const Segment<T> *__restrict__ segs, int64_t n_segs, const LineSetup<T> *__restrict__ setup, int H, int W,
+       T max_val, int *__restrict__ cnt, const int64_t *__restrict__ off, uint64_t *__restrict__ keys,
+       T *__restrict__ vals, int32_t *__restrict__ lines_out, int *__restrict__ flags)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_segs) return;
+    const Segment<T> sg = segs[i];
+    const LineSetup<T> L = setup[sg.line];
+    const int a = L.axis, b = 1 - a, sa = L.st[a];
+    const int db = (b == 0) ? W : H;
+    const int64_t base = WRITE ? off[i] : 0;
+    int n = 0;
+
+    WalkState<T> s;
+    s.c[0] = sg.c[0];
+    s.c[1] = sg.c[1];
+    s.Te = sg.Te;
+    s.Ne = sg.Ne;
+    const int c0 = s.c[a];
+    int it = 0;
+    for (; it < 4 * 64 + 16; it++) {
+        if (walk_done(L, s)) break;
+        const int k = (s.c[a] - c0) * sa;
+        if (k >= sg.ncols) break;
+        bool nx, ny;
+        walk_next(L, s.Ne, nx, ny);
+        if ((a == 0) ? nx : ny) {
+            ColRec<T> cr;
+            cr.s = s.c[b];
+            cr.Te = s.Te;
+            cr.Ne = s.Ne;
+            const int c = s.c[a];
+            fill_column(L, cr, 0, db, [&](int sp, T Te, T Ne) {
+                const T err = err_of(L, Te, Ne);
+                if (!(err <= T(1))) return;
+                if (WRITE) {
+                    const int x = (a == 0) ? c : sp, y = (a == 0) ? sp : c;
+                    keys[base + n] = ((uint64_t)L.frame * (uint64_t)H + (uint64_t)y) * (uint64_t)W + (uint64_t)x;
+                    vals[base + n] = pixel_value<K>(err, max_val);
+                    if (lines_out) lines_out[base + n] = sg.line;
+                }
+                n++;
+            });
+        }
+        walk_step(L, s, nx, ny);
+    }
+    if (it == 4 * 64 + 16) atomicOr(flags, FLAG_STUCK);
+    if (!WRITE) cnt[i] = n;
+}
+
+__global__ void k_i32_to_i64(const int *__restrict__ in, int64_t *__restrict__ out, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+__global__ void k_iota_u32(uint32_t *__restrict__ v, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = (uint32_t)i;
+}
+
+// run heads of a sorted key array
+__global__ void k_heads(const uint64_t *__restrict__ keys, int64_t n, int64_t *__restrict__ head)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
+}
+
+// ordered reduction of each run of equal keys, in emission (= line) order
+template <typename T, int OV>
+__global__ void k_reduce_runs(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ perm, int64_t n,
+                              const T *__restrict__ vals, const int32_t *__restrict__ line_of,
+                              const int64_t *__restrict__ pos, int64_t H, int64_t W, int32_t *__restrict__ of,
+                              int32_t *__restrict__ oy, int32_t *__restrict__ ox, T *__restrict__ ov,
+                              int32_t *__restrict__ ol, T *__restrict__ image)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t key = keys[i];
+    if (i > 0 && keys[i - 1] == key) return;
+    // image mode starts from the value already in the frame, list mode from 0
+    T acc = image ? image[key] : T(0);
+    int64_t j = i;
+    if (!image) {
+        acc = vals[perm[j]];
+        j++;
+    }
+    for (; j < n && keys[j] == key; j++) {
+        const T v = vals[perm[j]];
+        if (OV == 0) acc = acc + v;
+        else acc = (acc < v) ? v : acc;
+    }
+    if (image) {
+        image[key] = acc;
+        return;
+    }
+    const int64_t p = pos[i];
+    const uint64_t hw = (uint64_t)H * (uint64_t)W;
+    of[p] = (int32_t)(key / hw);
+    const uint64_t r = key % hw;
+    oy[p] = (int32_t)(r / (uint64_t)W);
+    ox[p] = (int32_t)(r % (uint64_t)W);
+    ov[p] = acc;
+    if (ol) ol[p] = line_of[perm[i]];
+}
+
+template <typename T>
+__global__ void k_raw_out(const uint64_t *__restrict__ keys, const T *__restrict__ vals,
+                          const int32_t *__restrict__ line_of, int64_t n, int64_t H, int64_t W,
+                          int32_t *__restrict__ of, int32_t *__restrict__ oy, int32_t *__restrict__ ox,
+                          T *__restrict__ ov, int32_t *__restrict__ ol)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t key = keys[i];
+    const uint64_t hw = (uint64_t)H * (uint64_t)W;
+    of[i] = (int32_t)(key / hw);
+    const uint64_t r = key % hw;
+    oy[i] = (int32_t)(r / (uint64_t)W);
+    ox[i] = (int32_t)(r % (uint64_t)W);
+    ov[i] = vals[i];
+    if (ol) ol[i] = line_of[i];
+}
+
+// ---------------------------------------------------------------------------
+// adjoint (sum overlap): per (segment, tile) partial gradients, then an
+// ordered per-line reduction
+// ---------------------------------------------------------------------------
+template <typename T, int K>
+__device__ __forceinline__ void pixel_grad(const LineSetup<T> &L, int x, int y, T gv, T max_val, double *acc)
+{
+    const T tx = L.t[0], ty = L.t[1];
+    const T rx = (T)x - L.p0[0], ry = (T)y - L.p0[1];
+    const T mag = tx * tx + ty * ty;
+    T ts = (rx * tx + ry * ty) / mag;
+    ts = (ts < T(0)) ? T(0) : ((ts > T(1)) ? T(1) : ts);
+    const T dx = rx - ts * tx, dy = ry - ts * ty;
+    const T d2 = dx * dx + dy * dy;
+    const T w = L.w, w2 = L.w * L.w;
+    const T u = d2 / w2;
+    T dk_dd2, dk_dw;
+    if constexpr (K == K_TRIANGULAR) {
+        const T dist = m_sqrt(d2);
+        dk_dd2 = (dist > T(0)) ? T(-1) / (T(2) * w * dist) : T(0);
+        dk_dw = dist / w2;
+    } else if constexpr (K == K_GAUSSIAN) {
+        const T k1 = T(-4.5) * T(0.3989422804014327) * (T)::exp(-4.5 * (double)u);
+        dk_dd2 = k1 / w2;
+        dk_dw = k1 * (T(-2) * u / w);
+    } else if constexpr (K == K_BIWEIGHT) {
+        const T k1 = T(-1.875) * (T(1) - u);
+        dk_dd2 = k1 / w2;
+        dk_dw = k1 * (T(-2) * u / w);
+    } else if constexpr (K == K_PARABOLIC) {
+        const T k1 = T(-0.75);
+        dk_dd2 = k1 / w2;
+        dk_dw = k1 * (T(-2) * u / w);
+    } else {
+        dk_dd2 = T(0);
+        dk_dw = T(0);
+    }
+    const T sg = gv * max_val;
+    const T c = sg * dk_dd2 * T(-2);
+    acc[0] += (double)(c * (T(1) - ts) * dx);
+    acc[1] += (double)(c * (T(1) - ts) * dy);
+    acc[2] += (double)(c * ts * dx);
+    acc[3] += (double)(c * ts * dy);
+    acc[4] += (double)(sg * dk_dw);
+}
+
+template <typename T, int K, int TILE, int SEG, int NT, int CH>
+__global__ void __launch_bounds__(NT)
+k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ tile_off,
+                const int *__restrict__ items, const Segment<T> *__restrict__ segs,
+                const LineSetup<T> *__restrict__ setup, const int64_t *__restrict__ slot_base, T max_val,
+                double *__restrict__ partial, int *__restrict__ flags)
+{
+    constexpr int NW = NT / 64;
+    static_assert(64 % SEG == 0, "a segment's columns must share one round of a wave");
+    __shared__ T gt[TILE * TILE];
+    __shared__ ColRec<T> rec[NW][CH * SEG];
+    __shared__ ItemHdr hdr[NW][CH];
+
+    const int tile = blockIdx.x;
+    const int beg = tile_off[tile], end = tile_off[tile + 1];
+    if (beg == end) return;
+    const int per_frame = g.ntx * g.nty;
+    const int64_t f = tile / per_frame;
+    const int r = tile - (int)f * per_frame;
+    const int ty = r / g.ntx, tx = r - ty * g.ntx;
+    const int X0 = tx * TILE, Y0 = ty * TILE;
+    const int X1 = min(X0 + TILE, g.W), Y1 = min(Y0 + TILE, g.H);
+
+    const T *fg = grad + (f * g.H) * (int64_t)g.W;
+    for (int i = threadIdx.x; i < TILE * TILE; i += NT) {
+        const int y = Y0 + i / TILE, x = X0 + (i % TILE);
+        gt[i] = (y < Y1 && x < X1) ? fg[(int64_t)y * g.W + x] : T(0);
+    }
+    __syncthreads();
+
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    ColRec<T> *R = rec[wave];
+    for (int first = beg + wave * CH; first < end; first += NW * CH) {
+        const int n = min(CH, end - first);
+        if (lane < n) {
+            const int sid = items[first + lane];
+            const Segment<T> sg = segs[sid];
+            const LineSetup<T> L = setup[sg.line];
+            hdr[wave][lane].seg = sid;
+            hdr[wave][lane].line = sg.line;
+            walk_columns<T, SEG>(L, sg, R + lane * SEG, flags);
+        }
+        wave_sync();
+        const int rounds = (n * SEG + 63) / 64;
+        for (int rd = 0; rd < rounds; rd++) {
+            const int u = lane + 64 * rd;
+            const int i = u / SEG, k = u - i * SEG;
+            double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+            if (i < n) {
+                const ColRec<T> cr = R[u];
+                if (cr.s != INT_MIN) {
+                    const Segment<T> &sg = segs[hdr[wave][i].seg];
+                    const LineSetup<T> &L = setup[hdr[wave][i].line];
+                    const int a = L.axis;
+                    const int c = sg.c[a] + k * L.st[a];
+                    const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
+                    fill_column(L, cr, s_lo, s_hi, [&](int s, T Te, T Ne) {
+                        const T err = err_of(L, Te, Ne);
+                        if (!(err <= T(1))) return;
+                        const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
+                        const T gv = gt[(y - Y0) * TILE + (x - X0)];
+                        if (gv != T(0)) pixel_grad<T, K>(L, x, y, gv, max_val, acc);
+                    });
+                }
+            }
+            // the SEG lanes of one item hold its columns: reduce them in a fixed tree
+#pragma unroll
+            for (int m = SEG / 2; m > 0; m >>= 1)
+#pragma unroll
+                for (int j = 0; j < 5; j++) acc[j] += __shfl_xor(acc[j], m, SEG);
+            if (k == 0 && i < n) {
+                const int sid = hdr[wave][i].seg;
+                const Segment<T> &sg = segs[sid];
+                const int a = setup[hdr[wave][i].line].axis;
+                const int tb = (a == 0) ? ty : tx;
+                double *p = partial + 5 * (slot_base[sid] + (tb - sg.lo / g.tile));
+#pragma unroll
+                for (int j = 0; j < 5; j++) p[j] = acc[j];
+            }
+        }
+        wave_sync();
+    }
+}
+
+template <typename T>
+__global__ void k_slot_count(const Segment<T> *__restrict__ segs, int64_t n_segs, int tile, int64_t *__restrict__ cnt)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n_segs) cnt[i] = segs[i].hi / tile - segs[i].lo / tile + 1;
+}
+
+template <typename T>
+__global__ void k_line_grad(const int *__restrict__ seg_off, const int64_t *__restrict__ slot_base,
+                            const double *__restrict__ partial, int64_t n_lines, T *__restrict__ grad_lines)
+{
+    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n_lines) return;
+    const int64_t s0 = slot_base[seg_off[l]], s1 = slot_base[seg_off[l + 1]];
+    double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int64_t s = s0; s < s1; s++)
+#pragma unroll
+        for (int j = 0; j < 5; j++) acc[j] += partial[5 * s + j];
+#pragma unroll
+    for (int j = 0; j < 5; j++) grad_lines[5 * l + j] = (T)acc[j];
+}
+
+// ---------------------------------------------------------------------------
+// host pipeline
+// ---------------------------------------------------------------------------
+template <typename T>
+struct SegmentPlan {
+    Geometry g;
+    int64_t n_lines = 0;
+    int64_t n_segs = 0;
+    int64_t n_items = 0;
+    int64_t n_tiles = 0;
+    DevBuf<LineSetup<T>> setup;
+    DevBuf<int> seg_cnt, seg_off, tile_cnt, tile_off, tile_cur, items, flags;
+    DevBuf<Segment<T>> segs;
+};
+
+static int check_common(int64_t n_frames, int64_t H, int64_t W, int64_t n_lines, bool have_idxs, int kernel,
+                        int overlap)
+{
+    if (kernel < 0 || kernel > 4) return fail(CBX_EVALUE, "kernel is missing for id " + std::to_string(kernel));
+    if (overlap < 0 || overlap > 1) return fail(CBX_EVALUE, "Invalid overlap keyword");
+    if (n_frames < 0 || H < 0 || W < 0 || n_lines < 0) return fail(CBX_EVALUE, "negative size");
+    if (H > (1 << 30) || W > (1 << 30) || n_frames >= (int64_t(1) << 31))
+        return fail(CBX_EVALUE, "frame shape exceeds the supported range");
+    if (n_lines >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "too many lines");
+    if (!have_idxs && n_lines && n_frames != 1 && n_frames != n_lines)
+        return fail(CBX_EVALUE, "idxs has an incompatible size (" + std::to_string(n_lines) +
+                                    " != " + std::to_string(n_frames) + ")");
+    return CBX_OK;
+}
+
+static int flags_to_status(int flags)
+{
+    if (flags & FLAG_INDEX) return fail(CBX_EINDEX, "idxs is outside of (0, n_frames)");
+    if (flags & FLAG_LONG)
+        return fail(CBX_EVALUE, "a line's walk exceeds the supported length (2^26 px) or coordinate range (1e9)");
+    if (flags & FLAG_STUCK) return fail(CBX_ERUNTIME, "a line walk did not terminate within its step budget");
+    return CBX_OK;
+}
+
+// Builds segments (and, with bin, the per-tile item lists).  One host sync.
+template <typename T, typename I>
+static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t n_lines, int64_t n_frames,
+                      int64_t H, int64_t W, int tile, int seg, bool bin, hipStream_t st)
+{
+    Geometry &g = P.g;
+    g.n_frames = n_frames;
+    g.H = (int32_t)H;
+    g.W = (int32_t)W;
+    g.tile = tile;
+    g.seg = seg;
+    g.ntx = (int32_t)((W + tile - 1) / tile);
+    g.nty = (int32_t)((H + tile - 1) / tile);
+    P.n_lines = n_lines;
+    P.n_tiles = bin ? n_frames * g.ntx * g.nty : 0;
+    if (P.n_tiles >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "too many tiles");
+
+    CBX_HIP(P.setup.alloc(n_lines, st));
+    CBX_HIP(P.seg_cnt.alloc(n_lines + 1, st));
+    CBX_HIP(P.seg_off.alloc(n_lines + 1, st));
+    CBX_HIP(P.flags.alloc(1, st));
+    CBX_HIP(hipMemsetAsync(P.flags.ptr, 0, sizeof(int), st));
+    CBX_HIP(hipMemsetAsync(P.seg_cnt.ptr + n_lines, 0, sizeof(int), st));
+    if (bin) {
+        CBX_HIP(P.tile_cnt.alloc(P.n_tiles + 1, st));
+        CBX_HIP(P.tile_off.alloc(P.n_tiles + 1, st));
+        CBX_HIP(hipMemsetAsync(P.tile_cnt.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
+    }
+    const int blk = 256;
+    if (bin)
+        k_segments<T, I, false, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
+            lines, idxs, n_lines, g, P.setup.ptr, P.seg_cnt.ptr, P.tile_cnt.ptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, P.flags.ptr);
+    else
+        k_segments<T, I, false, false><<<grid_for(n_lines, blk), blk, 0, st>>>(
+            lines, idxs, n_lines, g, P.setup.ptr, P.seg_cnt.ptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+            nullptr, P.flags.ptr);
+    CBX_HIP(hipGetLastError());
+    CBX_HIP(exclusive_scan_i32(P.seg_cnt.ptr, P.seg_off.ptr, n_lines + 1, st));
+    if (bin) CBX_HIP(exclusive_scan_i32(P.tile_cnt.ptr, P.tile_off.ptr, P.n_tiles + 1, st));
+
+    int host[3] = {0, 0, 0};
+    CBX_HIP(hipMemcpyAsync(&host[0], P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipMemcpyAsync(&host[1], P.seg_off.ptr + n_lines, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (bin) CBX_HIP(hipMemcpyAsync(&host[2], P.tile_off.ptr + P.n_tiles, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    int rc = flags_to_status(host[0]);
+    if (rc) return rc;
+    if (host[1] < 0 || host[2] < 0) return fail(CBX_ERUNTIME, "segment count overflow");
+    P.n_segs = host[1];
+    P.n_items = host[2];
+    if (P.n_segs == 0) return CBX_OK;
+
+    CBX_HIP(P.segs.alloc(P.n_segs, st));
+    if (bin) {
+        CBX_HIP(P.items.alloc(P.n_items, st));
+        CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
+        CBX_HIP(hipMemsetAsync(P.tile_cur.ptr, 0, sizeof(int) * P.n_tiles, st));
+        k_segments<T, I, true, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
+            lines, idxs, n_lines, g, P.setup.ptr, nullptr, nullptr, P.seg_off.ptr, P.segs.ptr, P.tile_off.ptr,
+            P.tile_cur.ptr, P.items.ptr, P.flags.ptr);
+    } else {
+        k_segments<T, I, true, false><<<grid_for(n_lines, blk), blk, 0, st>>>(
+            lines, idxs, n_lines, g, P.setup.ptr, nullptr, nullptr, P.seg_off.ptr, P.segs.ptr, nullptr, nullptr,
+            nullptr, P.flags.ptr);
+    }
+    CBX_HIP(hipGetLastError());
+    return CBX_OK;
+}
+
+static void resolve_opts(const cbx_draw_opts *o, int &tile, int &seg)
+{
+    tile = (o && o->tile) ? o->tile : 64;
+    seg = (o && o->seg) ? o->seg : 16;
+}
+
+static int check_opts(int tile, int seg)
+{
+    if (!(tile == 32 || tile == 64)) return fail(CBX_EVALUE, "tile must be 32 or 64");
+    if (!(seg == 8 || seg == 16)) return fail(CBX_EVALUE, "seg must be 8 or 16");
+    return CBX_OK;
+}
+
+template <typename T, int K, int OV, int TILE, int SEG>
+static void launch_tiles(T *out, const SegmentPlan<T> &P, T max_val, hipStream_t st)
+{
+    constexpr int NT = 256, CH = (SEG == 16) ? 16 : 32;
+    k_tiles<T, K, OV, TILE, SEG, NT, CH><<<(unsigned)P.n_tiles, NT, 0, st>>>(
+        out, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr);
+}
+
+template <typename T, int K, int OV>
+static void launch_tiles_k(T *out, const SegmentPlan<T> &P, T max_val, hipStream_t st)
+{
+    if (P.g.tile == 64 && P.g.seg == 16) launch_tiles<T, K, OV, 64, 16>(out, P, max_val, st);
+    else if (P.g.tile == 64) launch_tiles<T, K, OV, 64, 8>(out, P, max_val, st);
+    else if (P.g.seg == 16) launch_tiles<T, K, OV, 32, 16>(out, P, max_val, st);
+    else launch_tiles<T, K, OV, 32, 8>(out, P, max_val, st);
+}
+
+template <typename T, int OV>
+static void launch_tiles_ov(int kernel, T *out, const SegmentPlan<T> &P, T max_val, hipStream_t st)
+{
+    switch (kernel) {
+    case K_BIWEIGHT: launch_tiles_k<T, K_BIWEIGHT, OV>(out, P, max_val, st); break;
+    case K_GAUSSIAN: launch_tiles_k<T, K_GAUSSIAN, OV>(out, P, max_val, st); break;
+    case K_PARABOLIC: launch_tiles_k<T, K_PARABOLIC, OV>(out, P, max_val, st); break;
+    case K_RECTANGULAR: launch_tiles_k<T, K_RECTANGULAR, OV>(out, P, max_val, st); break;
+    default: launch_tiles_k<T, K_TRIANGULAR, OV>(out, P, max_val, st); break;
+    }
+}
+
+template <typename T, bool WRITE>
+static void launch_emit(int kernel, const SegmentPlan<T> &P, T max_val, int *cnt, const int64_t *off,
+                        uint64_t *keys, T *vals, int32_t *lines_out, hipStream_t st)
+{
+    const int blk = 256;
+    const int grid = grid_for(P.n_segs, blk);
+#define CBX_EMIT(KK)                                                                                           \
+    k_emit<T, KK, WRITE><<<grid, blk, 0, st>>>(P.segs.ptr, P.n_segs, P.setup.ptr, P.g.H, P.g.W, max_val, cnt, off, \
+                                              keys, vals, lines_out, P.flags.ptr)
+    switch (kernel) {
+    case K_BIWEIGHT: CBX_EMIT(K_BIWEIGHT); break;
+    case K_GAUSSIAN: CBX_EMIT(K_GAUSSIAN); break;
+    case K_PARABOLIC: CBX_EMIT(K_PARABOLIC); break;
+    case K_RECTANGULAR: CBX_EMIT(K_RECTANGULAR); break;
+    default: CBX_EMIT(K_TRIANGULAR); break;
+    }
+#undef CBX_EMIT
+}
+
+// Emission + optional stable sort/reduction.  image != nullptr: deterministic
+// image mode (accumulates into image in reference order).  Otherwise a pixel
+// list with capacity semantics.
+template <typename T>
+static int emit_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_val, int merge, T *image, int32_t *of,
+                         int32_t *oy, int32_t *ox, T *ov, int32_t *ol, int64_t capacity, int64_t *count,
+                         hipStream_t st)
+{
+    if (count) *count = 0;
+    if (P.n_segs == 0) return CBX_OK;
+    const int blk = 256;
+    DevBuf<int> cnt;
+    DevBuf<int64_t> cnt64, off;
+    CBX_HIP(cnt.alloc(P.n_segs + 1, st));
+    CBX_HIP(cnt64.alloc(P.n_segs + 1, st));
+    CBX_HIP(off.alloc(P.n_segs + 1, st));
+    CBX_HIP(hipMemsetAsync(cnt.ptr + P.n_segs, 0, sizeof(int), st));
+    launch_emit<T, false>(kernel, P, max_val, cnt.ptr, nullptr, nullptr, nullptr, nullptr, st);
+    CBX_HIP(hipGetLastError());
+    k_i32_to_i64<<<grid_for(P.n_segs + 1, blk), blk, 0, st>>>(cnt.ptr, cnt64.ptr, P.n_segs + 1);
+    CBX_HIP(exclusive_scan_i64(cnt64.ptr, off.ptr, P.n_segs + 1, st));
+    int64_t E = 0;
+    int fl = 0;
+    CBX_HIP(hipMemcpyAsync(&E, off.ptr + P.n_segs, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    int rc = flags_to_status(fl);
+    if (rc) return rc;
+    if (E == 0) return CBX_OK;
+    if (!image && !merge) {
+        *count = E;
+        if (E > capacity) return CBX_OK;
+    }
+    if (E >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "more than 2^31 - 1 pixel entries");
+
+    DevBuf<uint64_t> keys;
+    DevBuf<T> vals;
+    DevBuf<int32_t> lof;
+    CBX_HIP(keys.alloc(E, st));
+    CBX_HIP(vals.alloc(E, st));
+    const bool want_lines = (ol != nullptr);
+    if (want_lines) CBX_HIP(lof.alloc(E, st));
+    launch_emit<T, true>(kernel, P, max_val, nullptr, off.ptr, keys.ptr, vals.ptr, want_lines ? lof.ptr : nullptr,
+                         st);
+    CBX_HIP(hipGetLastError());
+
+    if (!image && !merge) {
+        k_raw_out<T><<<grid_for(E, blk), blk, 0, st>>>(keys.ptr, vals.ptr, lof.ptr, E, P.g.H, P.g.W, of, oy, ox, ov,
+                                                      want_lines ? ol : nullptr);
+        CBX_HIP(hipGetLastError());
+        return CBX_OK;
+    }
+
+    // stable radix sort of (pixel key, emission index): equal keys keep line order
+    const uint64_t max_key = (uint64_t)P.g.n_frames * (uint64_t)P.g.H * (uint64_t)P.g.W;
+    int end_bit = 1;
+    while (end_bit < 64 && (uint64_t(1) << end_bit) < max_key) end_bit++;
+    DevBuf<uint64_t> skeys;
+    DevBuf<uint32_t> perm, sperm;
+    CBX_HIP(skeys.alloc(E, st));
+    CBX_HIP(perm.alloc(E, st));
+    CBX_HIP(sperm.alloc(E, st));
+    k_iota_u32<<<grid_for(E, blk), blk, 0, st>>>(perm.ptr, E);
+    size_t tmp_bytes = 0;
+    CBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.ptr, skeys.ptr, perm.ptr, sperm.ptr, (int)E, 0,
+                                               end_bit, st));
+    DevBuf<char> tmp;
+    CBX_HIP(tmp.alloc(tmp_bytes, st));
+    CBX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, keys.ptr, skeys.ptr, perm.ptr, sperm.ptr, (int)E,
+                                               0, end_bit, st));
+    if (image) {
+        if (overlap == 0)
+            k_reduce_runs<T, 0><<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, sperm.ptr, E, vals.ptr, nullptr, nullptr,
+                                                                  P.g.H, P.g.W, nullptr, nullptr, nullptr, nullptr,
+                                                                  nullptr, image);
+        else
+            k_reduce_runs<T, 1><<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, sperm.ptr, E, vals.ptr, nullptr, nullptr,
+                                                                  P.g.H, P.g.W, nullptr, nullptr, nullptr, nullptr,
+                                                                  nullptr, image);
+        CBX_HIP(hipGetLastError());
+        return CBX_OK;
+    }
+    DevBuf<int64_t> head, pos;
+    CBX_HIP(head.alloc(E + 1, st));
+    CBX_HIP(pos.alloc(E + 1, st));
+    CBX_HIP(hipMemsetAsync(head.ptr + E, 0, sizeof(int64_t), st));
+    k_heads<<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, E, head.ptr);
+    CBX_HIP(exclusive_scan_i64(head.ptr, pos.ptr, E + 1, st));
+    int64_t U = 0;
+    CBX_HIP(hipMemcpyAsync(&U, pos.ptr + E, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    *count = U;
+    if (U > capacity) return CBX_OK;
+    if (overlap == 0)
+        k_reduce_runs<T, 0><<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, sperm.ptr, E, vals.ptr, lof.ptr, pos.ptr,
+                                                              P.g.H, P.g.W, of, oy, ox, ov, want_lines ? ol : nullptr,
+                                                              nullptr);
+    else
+        k_reduce_runs<T, 1><<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, sperm.ptr, E, vals.ptr, lof.ptr, pos.ptr,
+                                                              P.g.H, P.g.W, of, oy, ox, ov, want_lines ? ol : nullptr,
+                                                              nullptr);
+    CBX_HIP(hipGetLastError());
+    return CBX_OK;
+}
+
+template <typename T, typename I>
+static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const T *lines, int64_t n_lines,
+                           const I *idxs, T max_val, int kernel, int overlap, const cbx_draw_opts *opts, void *stream)
+{
+    int rc = check_common(n_frames, H, W, n_lines, idxs != nullptr, kernel, overlap);
+    if (rc) return rc;
+    int tile, seg;
+    resolve_opts(opts, tile, seg);
+    if ((rc = check_opts(tile, seg))) return rc;
+    if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    const bool det = opts && opts->deterministic;
+    SegmentPlan<T> P;
+    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, !det, st))) return rc;
+    if (P.n_segs == 0) return CBX_OK;
+    if (det) return emit_pipeline<T>(P, kernel, overlap, max_val, 1, out, nullptr, nullptr, nullptr, nullptr,
+                                     nullptr, 0, nullptr, st);
+    if (overlap == 0) launch_tiles_ov<T, 0>(kernel, out, P, max_val, st);
+    else launch_tiles_ov<T, 1>(kernel, out, P, max_val, st);
+    CBX_HIP(hipGetLastError());
+    return CBX_OK;
+}
+
+template <typename T, typename I>
+static int draw_pixels_impl(int64_t n_frames, int64_t H, int64_t W, const T *lines, int64_t n_lines, const I *idxs,
+                            T max_val, int kernel, int overlap, int merge, int32_t *of, int32_t *oy, int32_t *ox,
+                            T *ov, int32_t *ol, int64_t capacity, int64_t *count, void *stream)
+{
+    if (!count) return fail(CBX_EVALUE, "count must not be NULL");
+    *count = 0;
+    int rc = check_common(n_frames, H, W, n_lines, idxs != nullptr, kernel, overlap);
+    if (rc) return rc;
+    if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
+    hipStream_t st = (hipStream_t)stream;
+    SegmentPlan<T> P;
+    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, 64, 16, false, st))) return rc;
+    return emit_pipeline<T>(P, kernel, overlap, max_val, merge, nullptr, of, oy, ox, ov, ol, capacity, count, st);
+}
+
+template <typename T, int K>
+static void launch_adjoint(const T *grad, const SegmentPlan<T> &P, const int64_t *slot_base, T max_val,
+                           double *partial, hipStream_t st)
+{
+    constexpr int NT = 256;
+    if (P.g.tile == 64 && P.g.seg == 16)
+        k_adjoint_tiles<T, K, 64, 16, NT, 16><<<(unsigned)P.n_tiles, NT, 0, st>>>(
+            grad, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+    else if (P.g.tile == 64)
+        k_adjoint_tiles<T, K, 64, 8, NT, 32><<<(unsigned)P.n_tiles, NT, 0, st>>>(
+            grad, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+    else if (P.g.seg == 16)
+        k_adjoint_tiles<T, K, 32, 16, NT, 16><<<(unsigned)P.n_tiles, NT, 0, st>>>(
+            grad, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+    else
+        k_adjoint_tiles<T, K, 32, 8, NT, 32><<<(unsigned)P.n_tiles, NT, 0, st>>>(
+            grad, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+}
+
+template <typename T, typename I>
+static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, const T *lines, int64_t n_lines,
+                        const I *idxs, T max_val, int kernel, T *grad_lines, const cbx_draw_opts *opts, void *stream)
+{
+    int rc = check_common(n_frames, H, W, n_lines, idxs != nullptr, kernel, 0);
+    if (rc) return rc;
+    int tile, seg;
+    resolve_opts(opts, tile, seg);
+    if ((rc = check_opts(tile, seg))) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    if (n_lines == 0) return CBX_OK;
+    CBX_HIP(hipMemsetAsync(grad_lines, 0, sizeof(T) * 5 * n_lines, st));
+    if (n_frames * H * W == 0 || kernel == K_RECTANGULAR) return CBX_OK;
+    SegmentPlan<T> P;
+    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
+    if (P.n_segs == 0) return CBX_OK;
+    const int blk = 256;
+    DevBuf<int64_t> scnt, sbase;
+    CBX_HIP(scnt.alloc(P.n_segs + 1, st));
+    CBX_HIP(sbase.alloc(P.n_segs + 1, st));
+    CBX_HIP(hipMemsetAsync(scnt.ptr + P.n_segs, 0, sizeof(int64_t), st));
+    k_slot_count<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.segs.ptr, P.n_segs, tile, scnt.ptr);
+    CBX_HIP(exclusive_scan_i64(scnt.ptr, sbase.ptr, P.n_segs + 1, st));
+    // every slot is written by exactly one tile (P.n_items of them)
+    DevBuf<double> partial;
+    CBX_HIP(partial.alloc(5 * P.n_items, st));
+    switch (kernel) {
+    case K_BIWEIGHT: launch_adjoint<T, K_BIWEIGHT>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
+    case K_GAUSSIAN: launch_adjoint<T, K_GAUSSIAN>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
+    case K_PARABOLIC: launch_adjoint<T, K_PARABOLIC>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
+    default: launch_adjoint<T, K_TRIANGULAR>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
+    }
+    CBX_HIP(hipGetLastError());
+    k_line_grad<T><<<grid_for(n_lines, blk), blk, 0, st>>>(P.seg_off.ptr, sbase.ptr, partial.ptr, n_lines,
+                                                          grad_lines);
+    CBX_HIP(hipGetLastError());
+    int fl = 0;
+    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    return flags_to_status(fl);
+}
+
+} // namespace cbx
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int cbx_draw_lines_f32(float *out, int64_t n_frames, int64_t height, int64_t width, const float *lines,
+                       int64_t n_lines, const int32_t *idxs, float max_val, int kernel, int overlap,
+                       const cbx_draw_opts *opts, void *stream)
+{
+    return cbx::draw_lines_impl<float, int32_t>(out, n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
+                                                overlap, opts, stream);
+}
+
+int cbx_draw_lines_f64(double *out, int64_t n_frames, int64_t height, int64_t width, const double *lines,
+                       int64_t n_lines, const int64_t *idxs, double max_val, int kernel, int overlap,
+                       const cbx_draw_opts *opts, void *stream)
+{
+    return cbx::draw_lines_impl<double, int64_t>(out, n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
+                                                 overlap, opts, stream);
+}
+
+int cbx_draw_pixels_f32(int64_t n_frames, int64_t height, int64_t width, const float *lines, int64_t n_lines,
+                        const int32_t *idxs, float max_val, int kernel, int overlap, int merge, int32_t *frame,
+                        int32_t *y, int32_t *x, float *value, int32_t *line, int64_t capacity, int64_t *count,
+                        void *stream)
+{
+    return cbx::draw_pixels_impl<float, int32_t>(n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
+                                                 overlap, merge, frame, y, x, value, line, capacity, count, stream);
+}
+
+int cbx_draw_pixels_f64(int64_t n_frames, int64_t height, int64_t width, const double *lines, int64_t n_lines,
+                        const int64_t *idxs, double max_val, int kernel, int overlap, int merge, int32_t *frame,
+                        int32_t *y, int32_t *x, double *value, int32_t *line, int64_t capacity, int64_t *count,
+                        void *stream)
+{
+    return cbx::draw_pixels_impl<double, int64_t>(n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
+                                                  overlap, merge, frame, y, x, value, line, capacity, count, stream);
+}
+
+int cbx_draw_lines_adjoint_f32(const float *grad_out, int64_t n_frames, int64_t height, int64_t width,
+                               const float *lines, int64_t n_lines, const int32_t *idxs, float max_val, int kernel,
+                               float *grad_lines, const cbx_draw_opts *opts, void *stream)
+{
+    return cbx::adjoint_impl<float, int32_t>(grad_out, n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
+                                             grad_lines, opts, stream);
+}
+
+int cbx_draw_lines_adjoint_f64(const double *grad_out, int64_t n_frames, int64_t height, int64_t width,
+                               const double *lines, int64_t n_lines, const int64_t *idxs, double max_val,
+                               int kernel, double *grad_lines, const cbx_draw_opts *opts, void *stream)
+{
+    return cbx::adjoint_impl<double, int64_t>(grad_out, n_frames, height, width, lines, n_lines, idxs, max_val,
+                                              kernel, grad_lines, opts, stream);
+}
+
+} // extern "C"

@@ -1,0 +1,57 @@
+// Host-side helpers shared by the C-ABI entry points: status codes, the
+// thread-local error message, stream-ordered scratch buffers and the hipcub
+// scans.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+
+#include "../../include/cbx.h"
+
+namespace cbx {
+
+void set_error(const std::string &msg);
+int fail(int code, const std::string &msg);
+
+#define CBX_HIP(expr)                                                                                 \
+    do {                                                                                              \
+        hipError_t e_ = (expr);                                                                       \
+        if (e_ != hipSuccess)                                                                         \
+            return ::cbx::fail(CBX_ERUNTIME, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+// Stream-ordered device allocation released on scope exit.
+template <typename V>
+struct DevBuf {
+    V *ptr = nullptr;
+    size_t n = 0;
+    hipStream_t stream = nullptr;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    hipError_t alloc(size_t count, hipStream_t s)
+    {
+        release();
+        stream = s;
+        n = count;
+        if (count == 0) return hipSuccess;
+        return hipMallocAsync((void **)&ptr, sizeof(V) * count, s);
+    }
+    void release()
+    {
+        if (ptr) (void)hipFreeAsync(ptr, stream);
+        ptr = nullptr;
+        n = 0;
+    }
+    ~DevBuf() { release(); }
+};
+
+hipError_t exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s);
+hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
+
+inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
+
+} // namespace cbx

@@ -1,0 +1,151 @@
+"""``ndimage.draw_lines`` on MI355X, a drop-in for cbclib_v2.ndimage.draw_lines
+(cbclib_v2/ndimage.py:1 -> cbclib_v2/_src/functions.py:275-302), plus the
+pixel-list mode :func:`draw_pixels`.
+
+Same arguments, defaults, in-place semantics and exceptions as the reference:
+
+* ``out`` (..., H, W) float32 or float64 is accumulated in place and returned;
+* ``lines`` (..., 5) rows ``[x0, y0, x1, y1, width]`` (x = column);
+* ``idxs`` frame of each line in the flattened leading dims of ``out``, or
+  None (then there must be one frame or one line per frame,
+  bresenham.cpp:5-20); wrong size -> ValueError, out of range -> IndexError
+  (bresenham.cpp:22-36);
+* ``kernel`` one of biweight, gaussian, parabolic, rectangular, triangular
+  (unknown -> ValueError, include.hpp:495-503); ``overlap`` 'sum' or 'max'
+  (else ValueError, bresenham.cpp:146-148).
+
+Pixel sets are bit-identical to the reference CPU walk (bresenham.hpp:352-383);
+values are the reference's per-pixel values, accumulated per LDS tile (sum
+order differs from the reference's, within float rounding) or, with
+``deterministic=True``, in the single-threaded reference's exact order (then
+the image is bit-identical to ``bresenham.draw_lines(..., num_threads=1)``).
+"""
+from __future__ import annotations
+
+import ctypes
+from math import prod
+from typing import Any, NamedTuple, Optional
+
+import numpy as np
+import torch
+
+from . import _native, device
+from ._arrays import NUMPY, dtype_of, inout, namespace, ptr, result, shape_of, stream_ptr, to_device
+
+__all__ = ['draw_lines', 'draw_pixels', 'PixelList']
+
+
+def _line_count(lines: Any) -> int:
+    shape = shape_of(lines)
+    L = shape[-1] if len(shape) else 0
+    if L == 7:
+        raise NotImplementedError("3D lines (rows of 7) are not supported by cbclib_v2_amd")
+    if L != 5:
+        # bresenham.cpp:137 / cuda_draw_lines.cu:936
+        raise RuntimeError(f"Invalid lines size ({L}) at axis {max(len(shape) - 1, 0)}")
+    return prod(shape[:-1])
+
+
+def _check_idxs(idxs: Any, n_lines: int, n_frames: int) -> None:
+    if idxs is None:
+        if n_lines and n_frames not in (1, n_lines):
+            raise ValueError(f"idxs has an icnompatible size ({n_lines} != {n_frames})")
+        return
+    n = prod(shape_of(idxs))
+    if n != n_lines:
+        raise ValueError(f"idxs has an invalid size ({n} != {n_lines})")
+
+
+def _opts(tile: Optional[int], seg: Optional[int], deterministic: bool):
+    return _native.DrawOpts(int(tile or 0), int(seg or 0), 1 if deterministic else 0, 0)
+
+
+def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, kernel: str = 'rectangular',
+               overlap: str = 'sum', *, tile: Optional[int] = None, seg: Optional[int] = None,
+               deterministic: bool = False):
+    """Draw thick lines with a profile into ``out`` (in place) and return ``out``.
+
+    Extra keyword-only arguments (no reference counterpart): ``tile`` (32 or
+    64, pixel tile per workgroup), ``seg`` (8 or 16, walk checkpoint spacing)
+    and ``deterministic`` (reference summation order, bit-exact).
+    """
+    device.require_rocm('draw_lines')
+    kid = _native.kernel_id(kernel)
+    oid = _native.overlap_id(overlap)
+    oshape = shape_of(out)
+    odt = dtype_of(out)
+    if odt not in (np.float32, np.float64):
+        raise TypeError(f"out must be float32 or float64, got {odt}")
+    n_lines = _line_count(lines)
+    if len(oshape) < 2:
+        raise ValueError(f"Output array has insufficient number of dimensions ({len(oshape)} < 2)")
+    n_frames, H, W = prod(oshape[:-2]), oshape[-2], oshape[-1]
+    _check_idxs(idxs, n_lines, n_frames)
+    f64 = odt == np.float64
+    t_out, finish = inout(out, torch.float64 if f64 else torch.float32)
+    if n_lines == 0 or t_out.numel() == 0:
+        return finish()
+    t_lines = to_device(lines, torch.float64 if f64 else torch.float32)
+    t_idxs = None if idxs is None else to_device(idxs, torch.int64 if f64 else torch.int32)
+    L = _native.lib()
+    fn = L.cbx_draw_lines_f64 if f64 else L.cbx_draw_lines_f32
+    opts = _opts(tile, seg, deterministic)
+    _native.check(fn(t_out.data_ptr(), n_frames, H, W, t_lines.data_ptr(), n_lines, ptr(t_idxs), float(max_val),
+                     kid, oid, ctypes.byref(opts), stream_ptr()))
+    return finish()
+
+
+class PixelList(NamedTuple):
+    """Sparse render: entry i is pixel (frame[i], y[i], x[i]) with value[i].
+
+    ``line[i]`` is the line that emitted the entry (raw lists) or the first
+    line that touched the pixel (merged lists).
+    """
+    frame: Any
+    y: Any
+    x: Any
+    value: Any
+    line: Any
+
+
+def draw_pixels(shape, lines: Any, idxs: Any = None, max_val: float = 1.0, kernel: str = 'rectangular',
+                overlap: str = 'sum', merge: bool = True, dtype=np.float32) -> PixelList:
+    """Pixel-list mode of :func:`draw_lines`.
+
+    ``shape`` is the shape ``out`` would have.  ``merge=False`` returns every
+    (line, pixel) entry in the order the reference emits them
+    (bresenham.cpp:80-86); ``merge=True`` one entry per touched pixel sorted by
+    (frame, y, x), reduced in ascending line order.  Indices are bit-identical
+    to the reference walk.  Arrays are torch tensors on the GPU when ``lines``
+    is a GPU array, numpy arrays otherwise.
+    """
+    device.require_rocm('draw_pixels')
+    kid = _native.kernel_id(kernel)
+    oid = _native.overlap_id(overlap)
+    shape = tuple(int(s) for s in shape)
+    if len(shape) < 2:
+        raise ValueError("shape must have at least two dimensions")
+    n_lines = _line_count(lines)
+    n_frames, H, W = prod(shape[:-2]), shape[-2], shape[-1]
+    _check_idxs(idxs, n_lines, n_frames)
+    f64 = np.dtype(dtype) == np.float64
+    tdt = torch.float64 if f64 else torch.float32
+    like = namespace(lines)
+    t_lines = to_device(lines, tdt)
+    t_idxs = None if idxs is None else to_device(idxs, torch.int64 if f64 else torch.int32)
+    L = _native.lib()
+    fn = L.cbx_draw_pixels_f64 if f64 else L.cbx_draw_pixels_f32
+    dev = t_lines.device
+    count = ctypes.c_int64(0)
+    cap = 0
+    bufs = None
+    for _ in range(3):
+        bufs = [torch.empty(cap, dtype=torch.int32, device=dev) for _ in range(3)] + \
+               [torch.empty(cap, dtype=tdt, device=dev), torch.empty(cap, dtype=torch.int32, device=dev)]
+        _native.check(fn(n_frames, H, W, t_lines.data_ptr(), n_lines, ptr(t_idxs), float(max_val), kid, oid,
+                         1 if merge else 0, *[b.data_ptr() for b in bufs], cap, ctypes.byref(count), stream_ptr()))
+        if count.value <= cap:
+            break
+        cap = count.value
+    n = count.value
+    return PixelList(*[result(b[:n], like) for b in bufs])

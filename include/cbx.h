@@ -1,0 +1,154 @@
+/*
+ * cbx — MI355X (gfx950) streak rasterization and connected-component
+ * labelling, C ABI.
+ *
+ * Drop-in boundary for the hot path of cbclib_v2: ndimage.draw_lines and
+ * label.label.  Every pointer argument below is a DEVICE pointer (HBM) unless
+ * stated otherwise; every call is stream-ordered on `stream` (a hipStream_t,
+ * NULL = the default stream).  Arrays are C-contiguous.  Calls return a
+ * status code; on failure cbx_last_error() holds a message for the calling
+ * thread.  Status codes map onto the exceptions the reference raises:
+ *   CBX_EVALUE   -> ValueError   (std::invalid_argument in the reference)
+ *   CBX_EINDEX   -> IndexError   (std::out_of_range)
+ *   CBX_ERUNTIME -> RuntimeError (std::runtime_error / CUDA errors)
+ * A call that returns an error has not modified its outputs.
+ */
+#ifndef CBX_H_
+#define CBX_H_
+
+#include <stdint.h>
+
+#if defined(__GNUC__) || defined(__clang__)
+#define CBX_API __attribute__((visibility("default")))
+#else
+#define CBX_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CBX_OK 0
+#define CBX_EVALUE 1
+#define CBX_EINDEX 2
+#define CBX_ERUNTIME 3
+
+/* Profiles, numbered as kernels::type (cbclib_v2/_src/src/include.hpp:473-480). */
+#define CBX_BIWEIGHT 0
+#define CBX_GAUSSIAN 1
+#define CBX_PARABOLIC 2
+#define CBX_RECTANGULAR 3
+#define CBX_TRIANGULAR 4
+
+/* Overlap modes (bresenham.cpp:142-149). */
+#define CBX_SUM 0
+#define CBX_MAX 1
+
+/* label() input element types. */
+#define CBX_BOOL 0
+#define CBX_INT32 1
+#define CBX_INT64 2
+
+/* Message of the last failed call on this thread ("" if none). */
+CBX_API const char *cbx_last_error(void);
+
+/* Library version string. */
+CBX_API const char *cbx_version(void);
+
+/* Profile name -> CBX_* id, -1 if unknown (kernels::get_type, include.hpp:495-503). */
+CBX_API int cbx_kernel_id(const char *name);
+
+/*
+ * Launch options of draw_lines (NULL = automatic).  `tile` is the side of the
+ * square pixel tile one workgroup accumulates in LDS (32 or 64), `seg` the
+ * number of long-axis columns per walk checkpoint (8 or 16).  `deterministic`
+ * = 1 accumulates every pixel in ascending line order starting from the value
+ * already in `out`, which is the single-threaded reference's order: the
+ * result is then bit-identical to bresenham.draw_lines(..., num_threads=1).
+ * The replacement of the reference's `grid` argument
+ * (cuda_draw_lines.pyi:8-10).
+ */
+typedef struct cbx_draw_opts {
+    int32_t tile;
+    int32_t seg;
+    int32_t deterministic;
+    int32_t reserved;
+} cbx_draw_opts;
+
+/*
+ * draw_lines: render thick profiled segments into `out` (accumulated in
+ * place).  out: n_frames x height x width; lines: n_lines x 5 rows
+ * [x0, y0, x1, y1, width] (x = column); idxs: frame of each line or NULL
+ * (then n_frames must be 1 or n_lines, bresenham.cpp:5-20).  Out-of-range
+ * idxs -> CBX_EINDEX (bresenham.cpp:22-36).  overlap = CBX_SUM | CBX_MAX.
+ * Replaces cuda_draw_lines.draw_lines (cuda_draw_lines.cu:940-946, bound at
+ * 1098-1113) and bresenham.draw_lines (bresenham.cpp:142-149, bound at
+ * 329-330); the f32 entry takes int32 idxs and the f64 entry int64, the two
+ * overloads the reference binds.
+ */
+CBX_API int cbx_draw_lines_f32(float *out, int64_t n_frames, int64_t height, int64_t width, const float *lines,
+                       int64_t n_lines, const int32_t *idxs, float max_val, int kernel, int overlap,
+                       const cbx_draw_opts *opts, void *stream);
+CBX_API int cbx_draw_lines_f64(double *out, int64_t n_frames, int64_t height, int64_t width, const double *lines,
+                       int64_t n_lines, const int64_t *idxs, double max_val, int kernel, int overlap,
+                       const cbx_draw_opts *opts, void *stream);
+
+/*
+ * Pixel-list mode: the (frame, y, x, value) entries of the render instead of
+ * a dense image.  merge = 0 returns the raw per-line entries in the order the
+ * reference emits them into its per-thread ImageBuffer (bresenham.cpp:80-86,
+ * bresenham.hpp:10-34), with `line` = the segment index; merge = 1 returns one
+ * entry per touched pixel, sorted by (frame, y, x), whose value is the
+ * overlap-reduction of its entries in ascending line order (the image value
+ * on a zero frame for CBX_SUM; `line` is then the first contributing line).
+ * `*count` receives the number of entries; outputs are written only if
+ * *count <= capacity (call again with a larger buffer otherwise).
+ */
+CBX_API int cbx_draw_pixels_f32(int64_t n_frames, int64_t height, int64_t width, const float *lines, int64_t n_lines,
+                        const int32_t *idxs, float max_val, int kernel, int overlap, int merge, int32_t *frame,
+                        int32_t *y, int32_t *x, float *value, int32_t *line, int64_t capacity, int64_t *count,
+                        void *stream);
+CBX_API int cbx_draw_pixels_f64(int64_t n_frames, int64_t height, int64_t width, const double *lines, int64_t n_lines,
+                        const int64_t *idxs, double max_val, int kernel, int overlap, int merge, int32_t *frame,
+                        int32_t *y, int32_t *x, double *value, int32_t *line, int64_t capacity, int64_t *count,
+                        void *stream);
+
+/*
+ * Adjoint of draw_lines (overlap = sum): grad_lines[i] = d<grad_out, image>/d(x0, y0, x1, y1, width)
+ * of line i, over the pixels line i draws.  No reference counterpart (the
+ * reference has no gradient); the forward it differentiates is
+ * cbx_draw_lines_*.
+ */
+CBX_API int cbx_draw_lines_adjoint_f32(const float *grad_out, int64_t n_frames, int64_t height, int64_t width,
+                               const float *lines, int64_t n_lines, const int32_t *idxs, float max_val, int kernel,
+                               float *grad_lines, const cbx_draw_opts *opts, void *stream);
+CBX_API int cbx_draw_lines_adjoint_f64(const double *grad_out, int64_t n_frames, int64_t height, int64_t width,
+                               const double *lines, int64_t n_lines, const int64_t *idxs, double max_val,
+                               int kernel, double *grad_lines, const cbx_draw_opts *opts, void *stream);
+
+/*
+ * label: connected components of equal-valued non-zero elements of `inp`
+ * (dtype CBX_BOOL / CBX_INT32 / CBX_INT64, ndim dims of `shape`) under the
+ * neighbour `shifts` (n_shifts x ndim, HOST array, the structure's non-zero
+ * shifts).  out (int32, same shape) receives 0 for background and ids
+ * 1..n in raster order of each component's smallest flat index, after
+ * dropping components with fewer than npts elements.  *n_labels (HOST)
+ * receives n.  Replaces cuda_label.label (cuda_label.cu:294-378, bound at
+ * 847-849) and label.label (label.cpp:109-175, bound at 530-532).
+ */
+CBX_API int cbx_label(const void *inp, int dtype, int ndim, const int64_t *shape, const int64_t *shifts, int64_t n_shifts,
+              int64_t npts, int32_t *out, int64_t *n_labels, void *stream);
+
+/*
+ * label of (image > vmin) without materialising the mask: the threshold of
+ * the detection pipeline (RegionDetector.detect_regions,
+ * data_processing.py:630-631) fused into the first labelling pass.
+ */
+CBX_API int cbx_label_threshold_f32(const float *image, float vmin, int ndim, const int64_t *shape, const int64_t *shifts,
+                            int64_t n_shifts, int64_t npts, int32_t *out, int64_t *n_labels, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CBX_H_ */

@@ -1,0 +1,588 @@
+/*
+ * ORACLE — test infrastructure only.
+ *
+ * CPU restatement of cbclib_v2's streak rasterizer (draw_lines) and of its
+ * connected-component labelling (label), written from the reference's
+ * behaviour, used as the checker for the HIP product path.  Only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this
+ * library; the product (cbclib_v2_amd) never does.
+ *
+ * Arithmetic contract (what makes pixel indices bit-exact):
+ *   - float walk state is accumulated incrementally in T (float or double),
+ *     one rounding per addition, exactly as BaseError::increment
+ *     (cbclib_v2/_src/src/bresenham.hpp:59-63) does;
+ *   - std::pow(float, 2) in the reference promotes to double (C++11 overload
+ *     rules), so squares of float quotients are taken in double here too
+ *     (bresenham.hpp:282, 289);
+ *   - the file is compiled with -ffp-contract=off so that no product is
+ *     fused into an FMA (the reference is built with plain -O3 on x86-64,
+ *     which has no FMA in its baseline ISA).
+ *
+ * Every function cites the reference lines it follows.  Pinning (what checks
+ * this file against something independent) is listed per function.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define CBXO_OK 0
+#define CBXO_EVALUE 1   /* ValueError  (std::invalid_argument)   */
+#define CBXO_EINDEX 2   /* IndexError  (std::out_of_range)       */
+#define CBXO_ERUNTIME 3 /* RuntimeError                          */
+
+enum { K_BIWEIGHT = 0, K_GAUSSIAN = 1, K_PARABOLIC = 2, K_RECTANGULAR = 3, K_TRIANGULAR = 4 };
+
+/* Longest walk (in steps) accepted; the reference has no limit and simply
+ * runs for as long as the segment is. */
+#define CBXO_MAX_WALK (1L << 26)
+
+/* ------------------------------------------------------------------------ */
+/* Profiles: include.hpp:446-468 (kernel_traits).  Float versions reproduce  */
+/* the promotions of the C++ expressions (std::pow(float, int) -> double,    */
+/* std::max<T> -> conversion back to T).                                     */
+/* Pinned by: closed forms (tests/test_oracle_profiles.py) and by the        */
+/* compiled reference through draw_lines images.                             */
+/* ------------------------------------------------------------------------ */
+
+static float prof_f32(int k, float x)
+{
+    switch (k) {
+    case K_BIWEIGHT: {
+        /* T(0.9375) * pow(max<T>(1 - pow(x, 2), 0), 2) */
+        float m = (float)(1.0 - (double)x * (double)x);
+        m = (m < 0.0f) ? 0.0f : m;
+        return (float)((double)0.9375f * ((double)m * (double)m));
+    }
+    case K_GAUSSIAN: {
+        /* if |x| <= 1: M_1_SQRT2PI * exp(-pow(3 * x, 2) / 2) */
+        if (fabsf(x) <= 1.0f) {
+            float x3 = 3.0f * x;
+            double e = exp(-((double)x3 * (double)x3) / 2.0);
+            return (float)(0.3989422804014327 * e);
+        }
+        return 0.0f;
+    }
+    case K_PARABOLIC: {
+        /* T(0.75) * max<T>(1 - pow(x, 2), 0) */
+        float m = (float)(1.0 - (double)x * (double)x);
+        m = (m < 0.0f) ? 0.0f : m;
+        return 0.75f * m;
+    }
+    case K_RECTANGULAR:
+        return (fabsf(x) <= 1.0f) ? 1.0f : 0.0f;
+    case K_TRIANGULAR: {
+        float m = 1.0f - fabsf(x);
+        return (m < 0.0f) ? 0.0f : m;
+    }
+    }
+    return 0.0f;
+}
+
+static double prof_f64(int k, double x)
+{
+    switch (k) {
+    case K_BIWEIGHT: {
+        double m = 1.0 - x * x;
+        m = (m < 0.0) ? 0.0 : m;
+        return 0.9375 * (m * m);
+    }
+    case K_GAUSSIAN:
+        if (fabs(x) <= 1.0) {
+            double x3 = 3.0 * x;
+            return 0.3989422804014327 * exp(-(x3 * x3) / 2.0);
+        }
+        return 0.0;
+    case K_PARABOLIC: {
+        double m = 1.0 - x * x;
+        m = (m < 0.0) ? 0.0 : m;
+        return 0.75 * m;
+    }
+    case K_RECTANGULAR:
+        return (fabs(x) <= 1.0) ? 1.0 : 0.0;
+    case K_TRIANGULAR: {
+        double m = 1.0 - fabs(x);
+        return (m < 0.0) ? 0.0 : m;
+    }
+    }
+    return 0.0;
+}
+
+double cbxo_profile_f64(int k, double x) { return prof_f64(k, x); }
+float cbxo_profile_f32(int k, float x) { return prof_f32(k, x); }
+
+/* ------------------------------------------------------------------------ */
+/* The walk.  Follows BresenhamPlotter (bresenham.hpp:233-348), LineIterator */
+/* (118-231), the error classes (38-113) and draw_line_2d (352-383).         */
+/* Emits every (x, y, err) the reference hands to its draw_pixel callback,   */
+/* in the same order (bresenham.cpp:80-88).                                  */
+/* Pinned by: the compiled reference module (oracle/_ref) on random inputs   */
+/* (tests/test_oracle_vs_ref.py) and the committed golden fixtures; and by   */
+/* a brute-force {dist <= w} footprint away from the boundary.               */
+/* ------------------------------------------------------------------------ */
+
+typedef void (*emit_f32_t)(void *ctx, long x, long y, float err);
+typedef void (*emit_f64_t)(void *ctx, long x, long y, double err);
+
+/* error() of BresenhamPlotter for T = float (bresenham.hpp:279-291). */
+static float err_f32(float T, float N, float len, float w)
+{
+    if (w <= 0.0f) return INFINITY;
+    float a = -T, b = T - len * len;
+    float m = (a < b) ? b : a;          /* std::max(a, b) */
+    float at = (m < 0.0f) ? 0.0f : m;   /* std::max(., T()) */
+    float qt = at / len;
+    float qn = N / len;
+    float nsq = (float)((double)qn * (double)qn);
+    double e = (double)qt * (double)qt + (double)nsq;
+    return (float)(e / (double)(w * w));
+}
+
+static float nerr_f32(float N, float len)
+{
+    float qn = N / len;
+    return (float)((double)qn * (double)qn);
+}
+
+static double err_f64(double T, double N, double len, double w)
+{
+    if (w <= 0.0) return INFINITY;
+    double a = -T, b = T - len * len;
+    double m = (a < b) ? b : a;
+    double at = (m < 0.0) ? 0.0 : m;
+    double qt = at / len;
+    double qn = N / len;
+    double e = qt * qt + qn * qn;
+    return e / (w * w);
+}
+
+static double nerr_f64(double N, double len)
+{
+    double qn = N / len;
+    return qn * qn;
+}
+
+#define WALK_BODY(REAL, ROUND, CEIL, FABS, SQRT, ERRF, NERRF, EMIT_T)                           \
+    REAL x0 = ln[0], y0 = ln[1], x1 = ln[2], y1 = ln[3], w = ln[4];                             \
+    REAL p0[2] = {x0, y0}, p1[2] = {x1, y1};                                                    \
+    /* tangent (geometry.hpp:316) and normal (bresenham.hpp:310-316) */                          \
+    REAL t[2] = {x1 - x0, y1 - y0};                                                             \
+    REAL nv[2] = {y1 - y0, x0 - x1};                                                            \
+    if (!(w > (REAL)0)) return CBXO_OK; /* error() == inf for w <= 0; NaN never emits */        \
+    REAL mag = (REAL)0; mag += t[0] * t[0]; mag += t[1] * t[1];                                 \
+    REAL len = SQRT(mag);                                                                       \
+    if (!(len > (REAL)0) || !isfinite(len) || !isfinite(w)) return CBXO_OK;                     \
+    if (!isfinite(x0) || !isfinite(y0) || !isfinite(x1) || !isfinite(y1)) return CBXO_OK;       \
+    int axis = (FABS(t[0]) < FABS(t[1])) ? 1 : 0; /* first max |tau| */                         \
+    int sax = 1 - axis;                                                                         \
+    long offset = (long)(CEIL(w) + (REAL)1);                                                    \
+    REAL tau[2] = {t[0] / len, t[1] / len};                                                     \
+    REAL sc = FABS((REAL)offset / tau[axis]);                                                   \
+    long step[2], m0[2], m1[2];                                                                 \
+    for (int i = 0; i < 2; i++) {                                                               \
+        step[i] = (p1[i] >= p0[i]) ? 1 : -1;                                                    \
+        REAL a0 = ROUND(p0[i] - tau[i] * sc);                                                   \
+        REAL a1 = ROUND(p1[i] + tau[i] * sc);                                                   \
+        if (!(FABS(a0) < (REAL)1e9) || !(FABS(a1) < (REAL)1e9)) return CBXO_EVALUE;             \
+        m0[i] = (long)a0;                                                                       \
+        m1[i] = (long)a1 + step[i];                                                             \
+    }                                                                                           \
+    if (labs(m1[axis] - m0[axis]) > CBXO_MAX_WALK) return CBXO_EVALUE;                          \
+    /* BaseError(derr, point, origin): error = dot(derr, point - origin) */                     \
+    REAL d0 = (REAL)m0[0] - p0[0], d1 = (REAL)m0[1] - p0[1];                                    \
+    REAL T = (REAL)0; T += t[0] * d0; T += t[1] * d1;                                           \
+    REAL N = (REAL)0; N += nv[0] * d0; N += nv[1] * d1;                                         \
+    REAL w2 = w * w;                                                                            \
+    long cur[2] = {m0[0], m0[1]};                                                               \
+    /* for (iter = begin; iter != end; ++iter): end test is "any coordinate equal" */           \
+    while (!(cur[0] == m1[0] || cur[1] == m1[1])) {                                             \
+        /* update() with NormalError::is_next (bresenham.hpp:99-112, 208-222) */               \
+        REAL ex = N + (REAL)step[0] * nv[0];                                                    \
+        REAL ey = N + (REAL)step[1] * nv[1];                                                    \
+        REAL exy = ex + (REAL)step[1] * nv[1];                                                  \
+        int nx, ny;                                                                             \
+        if (FABS(ey) < FABS(ex)) { nx = (FABS(exy) < FABS(ey)); ny = 1; }                       \
+        else { nx = 1; ny = (FABS(exy) < FABS(ex)); }                                           \
+        int nxt[2] = {nx, ny};                                                                  \
+        if (nxt[axis]) {                                                                        \
+            emit(ctx, cur[0], cur[1], ERRF(T, N, len, w));                                      \
+            for (int half = 0; half < 2; half++) {                                              \
+                long dir = half ? -step[sax] : step[sax];                                       \
+                long c[2] = {cur[0], cur[1]};                                                   \
+                REAL Tf = T, Nf = N;                                                            \
+                for (;;) {                                                                      \
+                    c[sax] += dir;                                                              \
+                    Tf += (REAL)dir * t[sax];                                                   \
+                    Nf += (REAL)dir * nv[sax];                                                  \
+                    if (!(NERRF(Nf, len) < w2)) break;                                          \
+                    emit(ctx, c[0], c[1], ERRF(Tf, Nf, len, w));                                \
+                }                                                                               \
+            }                                                                                   \
+        }                                                                                       \
+        /* operator++: increment axis 0 then axis 1 (bresenham.hpp:149-154, 198-206) */         \
+        for (int i = 0; i < 2; i++) if (nxt[i]) {                                               \
+            cur[i] += step[i];                                                                  \
+            T += (REAL)step[i] * t[i];                                                          \
+            N += (REAL)step[i] * nv[i];                                                         \
+        }                                                                                       \
+    }                                                                                           \
+    return CBXO_OK;
+
+static int walk_f32(const float *ln, emit_f32_t emit, void *ctx)
+{
+    WALK_BODY(float, roundf, ceilf, fabsf, sqrtf, err_f32, nerr_f32, float)
+}
+
+static int walk_f64(const double *ln, emit_f64_t emit, void *ctx)
+{
+    WALK_BODY(double, round, ceil, fabs, sqrt, err_f64, nerr_f64, double)
+}
+
+/* ------------------------------------------------------------------------ */
+/* idxs rules: fill_indices / check_indices (bresenham.cpp:5-36, 54-55).     */
+/* ------------------------------------------------------------------------ */
+
+static int check_idxs_i32(const int32_t *idxs, long n_lines, long n_frames)
+{
+    if (!idxs) return (n_lines == 0 || n_frames == 1 || n_frames == n_lines) ? CBXO_OK : CBXO_EVALUE;
+    for (long i = 0; i < n_lines; i++)
+        if (idxs[i] < 0 || idxs[i] >= n_frames) return CBXO_EINDEX;
+    return CBXO_OK;
+}
+
+static int check_idxs_i64(const int64_t *idxs, long n_lines, long n_frames)
+{
+    if (!idxs) return (n_lines == 0 || n_frames == 1 || n_frames == n_lines) ? CBXO_OK : CBXO_EVALUE;
+    for (long i = 0; i < n_lines; i++)
+        if (idxs[i] < 0 || idxs[i] >= n_frames) return CBXO_EINDEX;
+    return CBXO_OK;
+}
+
+static long frame_of(long i, long n_frames, const void *idxs, int wide)
+{
+    if (idxs) return wide ? (long)((const int64_t *)idxs)[i] : (long)((const int32_t *)idxs)[i];
+    return (n_frames == 1) ? 0 : i;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Image accumulation: draw_lines_nd_impl with one thread                    */
+/* (bresenham.cpp:38-101): keep error <= 1 && inbound, value =               */
+/* max_val * K(sqrt(error)), then out[idx + HW*frame] += value or max.       */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    void *out; long H, W, frame; int kernel, overlap; double max_val;
+    /* pixel-list mode */
+    long count, cap; long line;
+    int32_t *pl_line, *pl_frame, *pl_y, *pl_x; void *pl_val; void *pl_err;
+} draw_ctx;
+
+static void draw_px_f32(void *vctx, long x, long y, float err)
+{
+    draw_ctx *c = (draw_ctx *)vctx;
+    if (!(err <= 1.0f) || y < 0 || y >= c->H || x < 0 || x >= c->W) return;
+    float v = (float)c->max_val * prof_f32(c->kernel, sqrtf(err));
+    float *o = (float *)c->out + (c->frame * c->H + y) * c->W + x;
+    if (c->overlap) *o = (*o < v) ? v : *o;   /* std::max(out, value) */
+    else *o = *o + v;
+}
+
+static void draw_px_f64(void *vctx, long x, long y, double err)
+{
+    draw_ctx *c = (draw_ctx *)vctx;
+    if (!(err <= 1.0) || y < 0 || y >= c->H || x < 0 || x >= c->W) return;
+    double v = c->max_val * prof_f64(c->kernel, sqrt(err));
+    double *o = (double *)c->out + (c->frame * c->H + y) * c->W + x;
+    if (c->overlap) *o = (*o < v) ? v : *o;
+    else *o = *o + v;
+}
+
+int cbxo_draw_lines_f32(float *out, long n_frames, long H, long W, const float *lines, long n_lines,
+                        const int32_t *idxs, float max_val, int kernel, int overlap)
+{
+    if (kernel < 0 || kernel > 4 || overlap < 0 || overlap > 1) return CBXO_EVALUE;
+    int rc = check_idxs_i32(idxs, n_lines, n_frames);
+    if (rc) return rc;
+    draw_ctx c = {out, H, W, 0, kernel, overlap, max_val};
+    for (long i = 0; i < n_lines; i++) {
+        c.frame = frame_of(i, n_frames, idxs, 0);
+        rc = walk_f32(lines + 5 * i, draw_px_f32, &c);
+        if (rc) return rc;
+    }
+    return CBXO_OK;
+}
+
+int cbxo_draw_lines_f64(double *out, long n_frames, long H, long W, const double *lines, long n_lines,
+                        const int64_t *idxs, double max_val, int kernel, int overlap)
+{
+    if (kernel < 0 || kernel > 4 || overlap < 0 || overlap > 1) return CBXO_EVALUE;
+    int rc = check_idxs_i64(idxs, n_lines, n_frames);
+    if (rc) return rc;
+    draw_ctx c = {out, H, W, 0, kernel, overlap, max_val};
+    for (long i = 0; i < n_lines; i++) {
+        c.frame = frame_of(i, n_frames, idxs, 1);
+        rc = walk_f64(lines + 5 * i, draw_px_f64, &c);
+        if (rc) return rc;
+    }
+    return CBXO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Raw pixel list: the entries the reference pushes into its per-thread      */
+/* ImageBuffer (bresenham.cpp:80-86, bresenham.hpp:10-34), in emission order */
+/* for one thread: (line, frame, y, x, value, err).  Returns the number of   */
+/* entries; arrays are written only up to cap.                               */
+/* ------------------------------------------------------------------------ */
+
+static void list_px_f32(void *vctx, long x, long y, float err)
+{
+    draw_ctx *c = (draw_ctx *)vctx;
+    if (!(err <= 1.0f) || y < 0 || y >= c->H || x < 0 || x >= c->W) return;
+    long k = c->count++;
+    if (k >= c->cap) return;
+    c->pl_line[k] = (int32_t)c->line; c->pl_frame[k] = (int32_t)c->frame;
+    c->pl_y[k] = (int32_t)y; c->pl_x[k] = (int32_t)x;
+    ((float *)c->pl_val)[k] = (float)c->max_val * prof_f32(c->kernel, sqrtf(err));
+    if (c->pl_err) ((float *)c->pl_err)[k] = err;
+}
+
+static void list_px_f64(void *vctx, long x, long y, double err)
+{
+    draw_ctx *c = (draw_ctx *)vctx;
+    if (!(err <= 1.0) || y < 0 || y >= c->H || x < 0 || x >= c->W) return;
+    long k = c->count++;
+    if (k >= c->cap) return;
+    c->pl_line[k] = (int32_t)c->line; c->pl_frame[k] = (int32_t)c->frame;
+    c->pl_y[k] = (int32_t)y; c->pl_x[k] = (int32_t)x;
+    ((double *)c->pl_val)[k] = c->max_val * prof_f64(c->kernel, sqrt(err));
+    if (c->pl_err) ((double *)c->pl_err)[k] = err;
+}
+
+long cbxo_pixel_list_f32(long n_frames, long H, long W, const float *lines, long n_lines, const int32_t *idxs,
+                         float max_val, int kernel, int32_t *line, int32_t *frame, int32_t *y, int32_t *x,
+                         float *val, float *err, long cap)
+{
+    if (kernel < 0 || kernel > 4) return -CBXO_EVALUE;
+    int rc = check_idxs_i32(idxs, n_lines, n_frames);
+    if (rc) return -rc;
+    draw_ctx c = {NULL, H, W, 0, kernel, 0, max_val, 0, cap, 0, line, frame, y, x, val, err};
+    for (long i = 0; i < n_lines; i++) {
+        c.frame = frame_of(i, n_frames, idxs, 0);
+        c.line = i;
+        rc = walk_f32(lines + 5 * i, list_px_f32, &c);
+        if (rc) return -rc;
+    }
+    return c.count;
+}
+
+long cbxo_pixel_list_f64(long n_frames, long H, long W, const double *lines, long n_lines, const int64_t *idxs,
+                         double max_val, int kernel, int32_t *line, int32_t *frame, int32_t *y, int32_t *x,
+                         double *val, double *err, long cap)
+{
+    if (kernel < 0 || kernel > 4) return -CBXO_EVALUE;
+    int rc = check_idxs_i64(idxs, n_lines, n_frames);
+    if (rc) return -rc;
+    draw_ctx c = {NULL, H, W, 0, kernel, 0, max_val, 0, cap, 0, line, frame, y, x, val, err};
+    for (long i = 0; i < n_lines; i++) {
+        c.frame = frame_of(i, n_frames, idxs, 1);
+        c.line = i;
+        rc = walk_f64(lines + 5 * i, list_px_f64, &c);
+        if (rc) return -rc;
+    }
+    return c.count;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 closed form (dense brute force): value = max_val * K(dist / w) for   */
+/* dist(p, segment) <= w, dist from the clamped projection of                */
+/* LineND::project_to_streak (geometry.hpp:319-339); the same formula is the */
+/* reference test's image_numpy (tests/draw_line_test.py:154-167).           */
+/* Pinned by: analytic special cases (tests/test_oracle_closed_form.py).     */
+/* ------------------------------------------------------------------------ */
+
+static double seg_dist2(double px, double py, double x0, double y0, double x1, double y1)
+{
+    double tx = x1 - x0, ty = y1 - y0, mag = tx * tx + ty * ty;
+    double cx = 0.5 * (x0 + x1), cy = 0.5 * (y0 + y1);
+    double qx = x0, qy = y0;
+    if (mag > 0.0) {
+        double r = ((px - cx) * tx + (py - cy) * ty) / mag;
+        r = (r < -0.5) ? -0.5 : ((r > 0.5) ? 0.5 : r);
+        qx = r * tx + cx; qy = r * ty + cy;
+    }
+    return (px - qx) * (px - qx) + (py - qy) * (py - qy);
+}
+
+int cbxo_render_closed_f64(double *out, long n_frames, long H, long W, const double *lines, long n_lines,
+                           const int64_t *idxs, double max_val, int kernel)
+{
+    if (kernel < 0 || kernel > 4) return CBXO_EVALUE;
+    int rc = check_idxs_i64(idxs, n_lines, n_frames);
+    if (rc) return rc;
+    for (long i = 0; i < n_lines; i++) {
+        const double *l = lines + 5 * i;
+        double w = l[4];
+        if (!(w > 0.0)) continue;
+        long f = frame_of(i, n_frames, idxs, 1);
+        long xa = (long)floor(fmin(l[0], l[2]) - w) - 1, xb = (long)ceil(fmax(l[0], l[2]) + w) + 1;
+        long ya = (long)floor(fmin(l[1], l[3]) - w) - 1, yb = (long)ceil(fmax(l[1], l[3]) + w) + 1;
+        if (xa < 0) xa = 0;
+        if (ya < 0) ya = 0;
+        if (xb > W - 1) xb = W - 1;
+        if (yb > H - 1) yb = H - 1;
+        for (long y = ya; y <= yb; y++)
+            for (long x = xa; x <= xb; x++) {
+                double d2 = seg_dist2((double)x, (double)y, l[0], l[1], l[2], l[3]);
+                double u = d2 / (w * w);
+                if (u <= 1.0) out[(f * H + y) * W + x] += max_val * prof_f64(kernel, sqrt(u));
+            }
+    }
+    return CBXO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* FP64 adjoint (no reference counterpart: the reference has no gradient).   */
+/* dL/dtheta_l = sum over the pixels line l emits of                         */
+/*   g[p] * max_val * dK/du * du/dtheta,   u = dist^2 / w^2,                 */
+/* with d(dist^2)/dp0 = -2 (1 - t*) (p - q), d(dist^2)/dp1 = -2 t* (p - q),  */
+/* q the clamped projection and t* in [0, 1] its parameter (envelope         */
+/* theorem), du/dw = -2u/w.  The pixel set is the float walk's (f32 lines)   */
+/* or the double walk's (f64 lines), so it is the set the forward uses.      */
+/* Pinned by: FP64 central finite differences of cbxo_render_closed_f64 and  */
+/* analytic special cases (tests/test_oracle_adjoint.py).                    */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    const double *g; long H, W, frame; int kernel; double max_val;
+    double x0, y0, x1, y1, w; double acc[5];
+} adj_ctx;
+
+static void adj_px(adj_ctx *c, long x, long y)
+{
+    if (y < 0 || y >= c->H || x < 0 || x >= c->W) return;
+    double gv = c->g[(c->frame * c->H + y) * c->W + x];
+    if (gv == 0.0) return;
+    double tx = c->x1 - c->x0, ty = c->y1 - c->y0, mag = tx * tx + ty * ty;
+    double rx = (double)x - c->x0, ry = (double)y - c->y0;
+    double ts = (rx * tx + ry * ty) / mag;
+    ts = (ts < 0.0) ? 0.0 : ((ts > 1.0) ? 1.0 : ts);
+    double dx = rx - ts * tx, dy = ry - ts * ty;
+    double d2 = dx * dx + dy * dy, w = c->w, u = d2 / (w * w);
+    double dist = sqrt(d2);
+    /* dK/d(dist^2) and dK/dw */
+    double dk_dd2, dk_dw;
+    switch (c->kernel) {
+    case K_BIWEIGHT: { double k1 = -1.875 * (1.0 - u); dk_dd2 = k1 / (w * w); dk_dw = k1 * (-2.0 * u / w); break; }
+    case K_PARABOLIC: { double k1 = -0.75; dk_dd2 = k1 / (w * w); dk_dw = k1 * (-2.0 * u / w); break; }
+    case K_GAUSSIAN: { double k1 = -4.5 * prof_f64(K_GAUSSIAN, sqrt(u)); dk_dd2 = k1 / (w * w); dk_dw = k1 * (-2.0 * u / w); break; }
+    case K_TRIANGULAR: dk_dd2 = (dist > 0.0) ? -1.0 / (2.0 * w * dist) : 0.0; dk_dw = dist / (w * w); break;
+    default: dk_dd2 = 0.0; dk_dw = 0.0; break;
+    }
+    double s = gv * c->max_val;
+    c->acc[0] += s * dk_dd2 * (-2.0 * (1.0 - ts) * dx);
+    c->acc[1] += s * dk_dd2 * (-2.0 * (1.0 - ts) * dy);
+    c->acc[2] += s * dk_dd2 * (-2.0 * ts * dx);
+    c->acc[3] += s * dk_dd2 * (-2.0 * ts * dy);
+    c->acc[4] += s * dk_dw;
+}
+
+static void adj_px_f32(void *vctx, long x, long y, float err)
+{
+    if (!(err <= 1.0f)) return;
+    adj_px((adj_ctx *)vctx, x, y);
+}
+
+static void adj_px_f64(void *vctx, long x, long y, double err)
+{
+    if (!(err <= 1.0)) return;
+    adj_px((adj_ctx *)vctx, x, y);
+}
+
+/* lines_f32 may be NULL: then the pixel set comes from the double walk of lines. */
+int cbxo_adjoint_f64(const double *grad_img, long n_frames, long H, long W, const double *lines,
+                     const float *lines_f32, long n_lines, const int64_t *idxs, double max_val, int kernel,
+                     double *grad_lines)
+{
+    if (kernel < 0 || kernel > 4) return CBXO_EVALUE;
+    int rc = check_idxs_i64(idxs, n_lines, n_frames);
+    if (rc) return rc;
+    for (long i = 0; i < n_lines; i++) {
+        const double *l = lines + 5 * i;
+        adj_ctx c = {grad_img, H, W, frame_of(i, n_frames, idxs, 1), kernel, max_val,
+                     l[0], l[1], l[2], l[3], l[4], {0, 0, 0, 0, 0}};
+        if (lines_f32) rc = walk_f32(lines_f32 + 5 * i, adj_px_f32, &c);
+        else rc = walk_f64(l, adj_px_f64, &c);
+        if (rc) return rc;
+        for (int j = 0; j < 5; j++) grad_lines[5 * i + j] = c.acc[j];
+    }
+    return CBXO_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* label: connected components of equal-valued truthy pixels under the      */
+/* structure's shifts, ids in raster order of each component's minimum flat */
+/* index, components with fewer than npts pixels dropped and the rest       */
+/* renumbered consecutively.  Same partition and numbering as label<I>      */
+/* (label.cpp:109-175) + LabelResult.to_array (label.cpp:459-493), which    */
+/* keeps a region at its minimum index and concatenates chunks in order.    */
+/* Union-find with the smaller index as the root.                           */
+/* Pinned by: scipy.ndimage.label (identical numbering), brute-force BFS on */
+/* tiny masks, and the compiled reference label module.                     */
+/* dtype: 0 = bool/uint8, 1 = int32, 2 = int64.                             */
+/* ------------------------------------------------------------------------ */
+
+static long uf_find(long *p, long i)
+{
+    long r = i;
+    while (p[r] != r) r = p[r];
+    while (p[i] != r) { long n = p[i]; p[i] = r; i = n; }
+    return r;
+}
+
+static long val_at(const void *inp, int dtype, long i)
+{
+    if (dtype == 0) return ((const uint8_t *)inp)[i];
+    if (dtype == 1) return ((const int32_t *)inp)[i];
+    return (long)((const int64_t *)inp)[i];
+}
+
+long cbxo_label(const void *inp, int dtype, int ndim, const int64_t *shape, const int64_t *shifts, long n_shifts,
+                long npts, int32_t *out)
+{
+    if (ndim < 1 || ndim > 16) return -CBXO_EVALUE;
+    long size = 1;
+    for (int d = 0; d < ndim; d++) size *= (long)shape[d];
+    if (size == 0) return 0;
+    long *par = (long *)malloc(sizeof(long) * (size_t)size);
+    long *cnt = (long *)calloc((size_t)size, sizeof(long));
+    if (!par || !cnt) { free(par); free(cnt); return -CBXO_ERUNTIME; }
+    for (long i = 0; i < size; i++) par[i] = val_at(inp, dtype, i) ? i : -1;
+    long coord[16];
+    for (long i = 0; i < size; i++) {
+        if (par[i] < 0) continue;
+        long rem = i;
+        for (int d = ndim - 1; d >= 0; d--) { coord[d] = rem % shape[d]; rem /= shape[d]; }
+        long vi = val_at(inp, dtype, i);
+        for (long s = 0; s < n_shifts; s++) {
+            const int64_t *sh = shifts + s * ndim;
+            long k = 0, ok = 1;
+            for (int d = 0; d < ndim; d++) {
+                long c = coord[d] + (long)sh[d];
+                if (c < 0 || c >= shape[d]) { ok = 0; break; }
+                k = k * (long)shape[d] + c;
+            }
+            if (!ok || par[k] < 0 || val_at(inp, dtype, k) != vi) continue;
+            long a = uf_find(par, i), b = uf_find(par, k);
+            if (a == b) continue;
+            if (a < b) par[b] = a; else par[a] = b;
+        }
+    }
+    for (long i = 0; i < size; i++) if (par[i] >= 0) cnt[uf_find(par, i)]++;
+    long n = 0;
+    for (long i = 0; i < size; i++) {
+        if (par[i] == i) cnt[i] = (cnt[i] >= npts) ? ++n : 0;
+    }
+    for (long i = 0; i < size; i++) out[i] = (par[i] >= 0) ? (int32_t)cnt[uf_find(par, i)] : 0;
+    free(par); free(cnt);
+    return n;
+}

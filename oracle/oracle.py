@@ -1,0 +1,211 @@
+"""ORACLE — test infrastructure only.
+
+ctypes front end of ``liboracle.so`` (the C restatement in ``cbx_oracle.c``)
+and a loader for the reference's own compiled CPU modules in ``oracle/_ref``.
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline
+leg may import this module; the product package never does.
+
+The functions mirror the reference call signatures they restate:
+
+* :func:`draw_lines` — ``bresenham.draw_lines`` with ``num_threads=1``
+  (cbclib_v2/_src/src/bresenham.cpp:38-149, 329-330);
+* :func:`pixel_list` — the per-line ImageBuffer entries the reference
+  accumulates (bresenham.cpp:80-86), in emission order;
+* :func:`label` — ``label.label`` + ``LabelResult.to_array``
+  (label.cpp:109-175, 459-493) returned as an int32 map and a count;
+* :func:`render_closed_f64` / :func:`adjoint_f64` — the FP64 closed form and
+  its gradient (no reference counterpart for the gradient).
+"""
+from __future__ import annotations
+
+import ctypes
+import glob
+import importlib.util
+import os
+import subprocess
+import sys
+from typing import Optional, Tuple
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+KERNELS = {'biweight': 0, 'gaussian': 1, 'parabolic': 2, 'rectangular': 3, 'triangular': 4}
+OVERLAPS = {'sum': 0, 'max': 1}
+
+_lib = None
+
+
+def build() -> None:
+    """Compile liboracle.so (and oracle/_ref when the reference sources exist)."""
+    subprocess.run(['make', '-s', '-C', HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, 'liboracle.so')
+        if not os.path.exists(path):
+            build()
+        _lib = ctypes.CDLL(path)
+        L, P, F, D, I = ctypes.c_long, ctypes.c_void_p, ctypes.c_float, ctypes.c_double, ctypes.c_int
+        _lib.cbxo_draw_lines_f32.argtypes = [P, L, L, L, P, L, P, F, I, I]
+        _lib.cbxo_draw_lines_f64.argtypes = [P, L, L, L, P, L, P, D, I, I]
+        _lib.cbxo_pixel_list_f32.argtypes = [L, L, L, P, L, P, F, I, P, P, P, P, P, P, L]
+        _lib.cbxo_pixel_list_f32.restype = L
+        _lib.cbxo_pixel_list_f64.argtypes = [L, L, L, P, L, P, D, I, P, P, P, P, P, P, L]
+        _lib.cbxo_pixel_list_f64.restype = L
+        _lib.cbxo_render_closed_f64.argtypes = [P, L, L, L, P, L, P, D, I]
+        _lib.cbxo_adjoint_f64.argtypes = [P, L, L, L, P, P, L, P, D, I, P]
+        _lib.cbxo_label.argtypes = [P, I, I, P, P, L, L, P]
+        _lib.cbxo_label.restype = L
+        _lib.cbxo_profile_f64.argtypes = [I, D]
+        _lib.cbxo_profile_f64.restype = D
+        _lib.cbxo_profile_f32.argtypes = [I, F]
+        _lib.cbxo_profile_f32.restype = F
+    return _lib
+
+
+def _ptr(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _raise(rc: int, what: str):
+    if rc == 1:
+        raise ValueError(what)
+    if rc == 2:
+        raise IndexError(what)
+    if rc:
+        raise RuntimeError(what)
+
+
+def _frames(out_shape) -> Tuple[int, int, int]:
+    H, W = out_shape[-2], out_shape[-1]
+    return int(np.prod(out_shape[:-2], dtype=np.int64)), int(H), int(W)
+
+
+def draw_lines(out: np.ndarray, lines: np.ndarray, idxs: Optional[np.ndarray] = None,
+               max_val: float = 1.0, kernel: str = 'rectangular', overlap: str = 'sum') -> np.ndarray:
+    """Single-threaded restatement of bresenham.draw_lines; ``out`` is updated in place."""
+    if kernel not in KERNELS:
+        raise ValueError(f"kernel is missing for {kernel}")
+    if overlap not in OVERLAPS:
+        raise ValueError(f"Invalid overlap keyword: {overlap}")
+    assert out.flags.c_contiguous
+    F, H, W = _frames(out.shape)
+    if out.dtype == np.float32:
+        ln = np.ascontiguousarray(lines, dtype=np.float32).reshape(-1, 5)
+        ix = None if idxs is None else np.ascontiguousarray(idxs, dtype=np.int32).ravel()
+        if ix is not None and ix.size != ln.shape[0]:
+            raise ValueError("idxs has an invalid size")
+        rc = lib().cbxo_draw_lines_f32(_ptr(out), F, H, W, _ptr(ln), ln.shape[0], _ptr(ix),
+                                       float(max_val), KERNELS[kernel], OVERLAPS[overlap])
+    elif out.dtype == np.float64:
+        ln = np.ascontiguousarray(lines, dtype=np.float64).reshape(-1, 5)
+        ix = None if idxs is None else np.ascontiguousarray(idxs, dtype=np.int64).ravel()
+        if ix is not None and ix.size != ln.shape[0]:
+            raise ValueError("idxs has an invalid size")
+        rc = lib().cbxo_draw_lines_f64(_ptr(out), F, H, W, _ptr(ln), ln.shape[0], _ptr(ix),
+                                       float(max_val), KERNELS[kernel], OVERLAPS[overlap])
+    else:
+        raise TypeError(out.dtype)
+    _raise(rc, "oracle draw_lines failed")
+    return out
+
+
+def pixel_list(shape, lines: np.ndarray, idxs: Optional[np.ndarray] = None, max_val: float = 1.0,
+               kernel: str = 'rectangular', dtype=np.float32):
+    """Raw emission list (line, frame, y, x, value, err), one entry per emitted pixel."""
+    F, H, W = _frames(shape)
+    f64 = np.dtype(dtype) == np.float64
+    ln = np.ascontiguousarray(lines, dtype=np.float64 if f64 else np.float32).reshape(-1, 5)
+    ix = None if idxs is None else np.ascontiguousarray(idxs, dtype=np.int64 if f64 else np.int32).ravel()
+    fn = lib().cbxo_pixel_list_f64 if f64 else lib().cbxo_pixel_list_f32
+    cap = 0
+    while True:
+        arrs = [np.empty(cap, np.int32) for _ in range(4)]
+        val = np.empty(cap, ln.dtype)
+        err = np.empty(cap, ln.dtype)
+        n = fn(F, H, W, _ptr(ln), ln.shape[0], _ptr(ix), float(max_val), KERNELS[kernel],
+               *[_ptr(a) for a in arrs], _ptr(val), _ptr(err), cap)
+        if n < 0:
+            _raise(-n, "oracle pixel_list failed")
+        if n <= cap:
+            return tuple(a[:n] for a in arrs) + (val[:n], err[:n])
+        cap = n
+
+
+def render_closed_f64(shape, lines: np.ndarray, idxs: Optional[np.ndarray] = None, max_val: float = 1.0,
+                      kernel: str = 'rectangular') -> np.ndarray:
+    F, H, W = _frames(shape)
+    out = np.zeros((F, H, W), np.float64)
+    ln = np.ascontiguousarray(lines, dtype=np.float64).reshape(-1, 5)
+    ix = None if idxs is None else np.ascontiguousarray(idxs, dtype=np.int64).ravel()
+    rc = lib().cbxo_render_closed_f64(_ptr(out), F, H, W, _ptr(ln), ln.shape[0], _ptr(ix),
+                                      float(max_val), KERNELS[kernel])
+    _raise(rc, "oracle render_closed failed")
+    return out.reshape(shape)
+
+
+def adjoint_f64(grad_img: np.ndarray, lines: np.ndarray, idxs: Optional[np.ndarray] = None,
+                max_val: float = 1.0, kernel: str = 'rectangular', walk: str = 'f32') -> np.ndarray:
+    """FP64 gradient of sum(grad_img * image) w.r.t. (x0, y0, x1, y1, w) of every line.
+
+    ``walk`` picks the pixel set: 'f32' uses the float walk (what an f32 forward
+    draws), 'f64' the double walk.
+    """
+    F, H, W = _frames(grad_img.shape)
+    g = np.ascontiguousarray(grad_img, dtype=np.float64)
+    ln = np.ascontiguousarray(lines, dtype=np.float64).reshape(-1, 5)
+    l32 = np.ascontiguousarray(lines, dtype=np.float32).reshape(-1, 5) if walk == 'f32' else None
+    ix = None if idxs is None else np.ascontiguousarray(idxs, dtype=np.int64).ravel()
+    out = np.zeros_like(ln)
+    rc = lib().cbxo_adjoint_f64(_ptr(g), F, H, W, _ptr(ln), _ptr(l32), ln.shape[0], _ptr(ix),
+                                float(max_val), KERNELS[kernel], _ptr(out))
+    _raise(rc, "oracle adjoint failed")
+    return out
+
+
+def label(inp: np.ndarray, shifts: np.ndarray, npts: int = 1) -> Tuple[np.ndarray, int]:
+    """Connected components; ``shifts`` are the structure's non-zero shifts (n, ndim)."""
+    a = np.ascontiguousarray(inp)
+    if a.dtype == np.bool_:
+        a, dt = a.view(np.uint8), 0
+    elif a.dtype == np.int32:
+        dt = 1
+    elif a.dtype == np.int64:
+        dt = 2
+    else:
+        raise TypeError(a.dtype)
+    shape = np.asarray(a.shape, np.int64)
+    sh = np.ascontiguousarray(shifts, dtype=np.int64).reshape(-1, a.ndim)
+    out = np.zeros(a.shape, np.int32)
+    n = lib().cbxo_label(_ptr(a), dt, a.ndim, _ptr(shape), _ptr(sh), sh.shape[0], int(npts), _ptr(out))
+    if n < 0:
+        _raise(-n, "oracle label failed")
+    return out, int(n)
+
+
+def profile(kernel: str, x: float, dtype=np.float64) -> float:
+    if np.dtype(dtype) == np.float32:
+        return lib().cbxo_profile_f32(KERNELS[kernel], float(np.float32(x)))
+    return lib().cbxo_profile_f64(KERNELS[kernel], float(x))
+
+
+# ---------------------------------------------------------------------------
+# the reference's compiled CPU modules (built by oracle/Makefile from the
+# reference sources; absent when the reference tree was not available)
+# ---------------------------------------------------------------------------
+
+def ref_module(name: str):
+    """Import oracle/_ref/<name>*.so (``bresenham`` or ``label``) or return None."""
+    hits = glob.glob(os.path.join(HERE, '_ref', name + '*.so'))
+    if not hits:
+        return None
+    key = 'cbx_oracle_ref_' + name
+    if key in sys.modules:
+        return sys.modules[key]
+    spec = importlib.util.spec_from_file_location(name, hits[0])
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    sys.modules[key] = mod
+    return mod
