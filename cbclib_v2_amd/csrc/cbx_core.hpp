@@ -39,8 +39,11 @@ __device__ __forceinline__ float m_round(float x) { return __builtin_roundf(x); 
 __device__ __forceinline__ double m_round(double x) { return __builtin_round(x); }
 __device__ __forceinline__ float m_ceil(float x) { return __builtin_ceilf(x); }
 __device__ __forceinline__ double m_ceil(double x) { return __builtin_ceil(x); }
-__device__ __forceinline__ float m_sqrt(float x) { return __fsqrt_rn(x); }
-__device__ __forceinline__ double m_sqrt(double x) { return __dsqrt_rn(x); }
+// IEEE square roots.  Note: HIP's __fsqrt_rn is the native v_sqrt_f32 (1 ulp,
+// not correctly rounded) unless OCML_BASIC_ROUNDED_OPERATIONS is defined;
+// __builtin_sqrtf under -fhip-fp32-correctly-rounded-divide-sqrt is exact.
+__device__ __forceinline__ float m_sqrt(float x) { return __builtin_sqrtf(x); }
+__device__ __forceinline__ double m_sqrt(double x) { return __builtin_sqrt(x); }
 
 // ---------------------------------------------------------------------------
 // correctly rounded quotient from a correctly rounded reciprocal
@@ -291,13 +294,13 @@ __device__ __forceinline__ double profile(double x)
 template <int K>
 __device__ __forceinline__ float pixel_value(float err, float max_val)
 {
-    return max_val * profile<K>(__fsqrt_rn(err));
+    return max_val * profile<K>(m_sqrt(err));
 }
 
 template <int K>
 __device__ __forceinline__ double pixel_value(double err, double max_val)
 {
-    return max_val * profile<K>(__dsqrt_rn(err));
+    return max_val * profile<K>(m_sqrt(err));
 }
 
 // dK/du at u = err = (dist / w)^2 for the adjoint; the triangular profile is

@@ -420,57 +420,6 @@ __global__ void k_i32_to_i64(const int *__restrict__ in, int64_t *__restrict__ o
     if (i < n) out[i] = in[i];
 }
 
-__global__ void k_iota_u32(uint32_t *__restrict__ v, int64_t n)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) v[i] = (uint32_t)i;
-}
-
-// run heads of a sorted key array
-__global__ void k_heads(const uint64_t *__restrict__ keys, int64_t n, int64_t *__restrict__ head)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) head[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1 : 0;
-}
-
-// ordered reduction of each run of equal keys, in emission (= line) order
-template <typename T, int OV>
-__global__ void k_reduce_runs(const uint64_t *__restrict__ keys, const uint32_t *__restrict__ perm, int64_t n,
-                              const T *__restrict__ vals, const int32_t *__restrict__ line_of,
-                              const int64_t *__restrict__ pos, int64_t H, int64_t W, int32_t *__restrict__ of,
-                              int32_t *__restrict__ oy, int32_t *__restrict__ ox, T *__restrict__ ov,
-                              int32_t *__restrict__ ol, T *__restrict__ image)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint64_t key = keys[i];
-    if (i > 0 && keys[i - 1] == key) return;
-    // image mode starts from the value already in the frame, list mode from 0
-    T acc = image ? image[key] : T(0);
-    int64_t j = i;
-    if (!image) {
-        acc = vals[perm[j]];
-        j++;
-    }
-    for (; j < n && keys[j] == key; j++) {
-        const T v = vals[perm[j]];
-        if (OV == 0) acc = acc + v;
-        else acc = (acc < v) ? v : acc;
-    }
-    if (image) {
-        image[key] = acc;
-        return;
-    }
-    const int64_t p = pos[i];
-    const uint64_t hw = (uint64_t)H * (uint64_t)W;
-    of[p] = (int32_t)(key / hw);
-    const uint64_t r = key % hw;
-    oy[p] = (int32_t)(r / (uint64_t)W);
-    ox[p] = (int32_t)(r % (uint64_t)W);
-    ov[p] = acc;
-    if (ol) ol[p] = line_of[perm[i]];
-}
-
 template <typename T>
 __global__ void k_raw_out(const uint64_t *__restrict__ keys, const T *__restrict__ vals,
                           const int32_t *__restrict__ line_of, int64_t n, int64_t H, int64_t W,
@@ -487,6 +436,193 @@ __global__ void k_raw_out(const uint64_t *__restrict__ keys, const T *__restrict
     ox[i] = (int32_t)(r % (uint64_t)W);
     ov[i] = vals[i];
     if (ol) ol[i] = line_of[i];
+}
+
+// ---------------------------------------------------------------------------
+// ordered (reference-order) accumulation: deterministic images and merged
+// pixel lists
+// ---------------------------------------------------------------------------
+
+// Sorts each tile's item list by segment id, i.e. by (line, walk position):
+// the order in which the reference's single thread visits the segments.
+// One workgroup per tile, bitonic sort in LDS.
+constexpr int SORT_MAX = 8192;
+constexpr int FLAG_TILE_FULL = 8;
+
+__global__ void __launch_bounds__(256)
+k_sort_tile_items(const int *__restrict__ tile_off, int *__restrict__ items, int *__restrict__ flags)
+{
+    __shared__ int buf[SORT_MAX];
+    const int beg = tile_off[blockIdx.x], n = tile_off[blockIdx.x + 1] - beg;
+    if (n <= 1) return;
+    if (n > SORT_MAX) {
+        if (threadIdx.x == 0) atomicOr(flags, FLAG_TILE_FULL);
+        return;
+    }
+    int p2 = 2;
+    while (p2 < n) p2 <<= 1;
+    for (int i = threadIdx.x; i < p2; i += blockDim.x) buf[i] = (i < n) ? items[beg + i] : INT_MAX;
+    __syncthreads();
+    for (int k = 2; k <= p2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < p2; i += blockDim.x) {
+                const int ixj = i ^ j;
+                if (ixj > i) {
+                    const int a = buf[i], b = buf[ixj];
+                    const bool up = (i & k) == 0;
+                    if ((a > b) == up) {
+                        buf[i] = b;
+                        buf[ixj] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) items[beg + i] = buf[i];
+}
+
+// One wave per tile.  Items are applied one at a time in sorted order; the
+// lanes of an item cover its columns (no pixel repeats inside one line), so
+// every pixel sees its contributions in ascending line order, starting from
+// the value already in the frame (image mode) or from its first contribution
+// (list mode: `first` records the first contributing line).
+template <typename T, int K, int OV, int TILE, int SEG, bool LIST>
+__global__ void __launch_bounds__(64)
+k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__restrict__ first_line, Geometry g,
+                const int *__restrict__ tile_off, const int *__restrict__ items, const Segment<T> *__restrict__ segs,
+                const LineSetup<T> *__restrict__ setup, T max_val, int *__restrict__ flags)
+{
+    static_assert(2 * SEG <= 64, "two lanes per column");
+    __shared__ T acc[TILE * TILE];
+    __shared__ int32_t first[TILE * TILE];
+    __shared__ ColRec<T> rec[64 * SEG];
+    __shared__ ItemHdr hdr[64];
+
+    const int tile = blockIdx.x;
+    const int beg = tile_off[tile], end = tile_off[tile + 1];
+    if (beg == end) return;
+    const int per_frame = g.ntx * g.nty;
+    const int64_t f = tile / per_frame;
+    const int r = tile - (int)f * per_frame;
+    const int ty = r / g.ntx, tx = r - ty * g.ntx;
+    const int X0 = tx * TILE, Y0 = ty * TILE;
+    const int X1 = min(X0 + TILE, g.W), Y1 = min(Y0 + TILE, g.H);
+    T *fo = img + (f * g.H) * (int64_t)g.W;
+    const int lane = threadIdx.x;
+
+    for (int i = lane; i < TILE * TILE; i += 64) {
+        const int y = Y0 + i / TILE, x = X0 + (i % TILE);
+        if (LIST) acc[i] = T(0);
+        else acc[i] = (y < Y1 && x < X1) ? fo[(int64_t)y * g.W + x] : T(0);
+        first[i] = -1;
+    }
+    wave_sync();
+
+    for (int c0 = beg; c0 < end; c0 += 64) {
+        const int n = min(64, end - c0);
+        if (lane < n) {
+            const int sid = items[c0 + lane];
+            const Segment<T> sg = segs[sid];
+            const LineSetup<T> L = setup[sg.line];
+            hdr[lane].seg = sid;
+            hdr[lane].line = sg.line;
+            walk_columns<T, SEG>(L, sg, rec + lane * SEG, flags);
+        }
+        wave_sync();
+        for (int j = 0; j < n; j++) {
+            if (lane < 2 * SEG) {
+                const int k = lane >> 1, half = lane & 1;
+                const ColRec<T> cr = rec[j * SEG + k];
+                if (cr.s != INT_MIN) {
+                    const Segment<T> &sg = segs[hdr[j].seg];
+                    const LineSetup<T> &L = setup[hdr[j].line];
+                    const int line = hdr[j].line;
+                    const int a = L.axis, b = 1 - a;
+                    const int c = sg.c[a] + k * L.st[a];
+                    const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
+                    auto put = [&](int s, T Te, T Ne) {
+                        const T err = err_of(L, Te, Ne);
+                        if (!(err <= T(1))) return;
+                        const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
+                        const int p = (y - Y0) * TILE + (x - X0);
+                        const T v = pixel_value<K>(err, max_val);
+                        if (LIST && first[p] < 0) {
+                            first[p] = line;
+                            acc[p] = v;
+                        } else {
+                            if (OV == 0) acc[p] = acc[p] + v;
+                            else acc[p] = (acc[p] < v) ? v : acc[p];
+                            if (!LIST && first[p] < 0) first[p] = line;
+                        }
+                    };
+                    // lane pair (k, 0): centre and the +step half; (k, 1): the -step half
+                    if (half == 0 && cr.s >= s_lo && cr.s < s_hi) put(cr.s, cr.Te, cr.Ne);
+                    const int dir = half ? -L.st[b] : L.st[b];
+                    const T dT = (dir > 0) ? L.t[b] : -L.t[b], dN = (dir > 0) ? L.nv[b] : -L.nv[b];
+                    int s = cr.s;
+                    T Te = cr.Te, Ne = cr.Ne;
+                    for (;;) {
+                        s += dir;
+                        Te += dT;
+                        Ne += dN;
+                        if (!(nsq_of(L, Ne) < L.w2)) break;
+                        if (s >= s_lo && s < s_hi) put(s, Te, Ne);
+                        else if ((dir > 0) ? (s >= s_hi) : (s < s_lo)) break;
+                    }
+                }
+            }
+            wave_sync();
+        }
+    }
+
+    for (int i = lane; i < TILE * TILE; i += 64) {
+        const int y = Y0 + i / TILE, x = X0 + (i % TILE);
+        if (y >= Y1 || x >= X1) continue;
+        const int64_t o = (int64_t)y * g.W + x;
+        if (LIST) {
+            const int64_t q = (f * g.H) * (int64_t)g.W + o;
+            touched[q] = first[i] >= 0;
+            if (first[i] >= 0) {
+                fo[o] = acc[i];
+                if (first_line) first_line[q] = first[i];
+            }
+        } else if (first[i] >= 0) {
+            fo[o] = acc[i];
+        }
+    }
+}
+
+// stream compaction of a dense (image, touched) pair in raster order
+__global__ void k_row_count(const uint8_t *__restrict__ touched, int64_t rows, int W, int64_t *__restrict__ cnt)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const uint8_t *t = touched + r * W;
+    int64_t c = 0;
+    for (int x = 0; x < W; x++) c += t[x];
+    cnt[r] = c;
+}
+
+template <typename T>
+__global__ void k_row_write(const uint8_t *__restrict__ touched, const T *__restrict__ img,
+                            const int32_t *__restrict__ first_line, int64_t rows, int H, int W,
+                            const int64_t *__restrict__ off, int32_t *__restrict__ of, int32_t *__restrict__ oy,
+                            int32_t *__restrict__ ox, T *__restrict__ ov, int32_t *__restrict__ ol)
+{
+    const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    int64_t p = off[r];
+    const int64_t base = r * W;
+    for (int x = 0; x < W; x++) {
+        if (!touched[base + x]) continue;
+        of[p] = (int32_t)(r / H);
+        oy[p] = (int32_t)(r % H);
+        ox[p] = x;
+        ov[p] = img[base + x];
+        if (ol) ol[p] = first_line[base + x];
+        p++;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -811,15 +947,13 @@ static void launch_emit(int kernel, const SegmentPlan<T> &P, T max_val, int *cnt
 #undef CBX_EMIT
 }
 
-// Emission + optional stable sort/reduction.  image != nullptr: deterministic
-// image mode (accumulates into image in reference order).  Otherwise a pixel
-// list with capacity semantics.
+// Raw pixel list: one entry per emitted pixel, in the reference's emission
+// order (segments are stored line-major, in walk order).
 template <typename T>
-static int emit_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_val, int merge, T *image, int32_t *of,
-                         int32_t *oy, int32_t *ox, T *ov, int32_t *ol, int64_t capacity, int64_t *count,
-                         hipStream_t st)
+static int raw_list_pipeline(SegmentPlan<T> &P, int kernel, T max_val, int32_t *of, int32_t *oy, int32_t *ox, T *ov,
+                             int32_t *ol, int64_t capacity, int64_t *count, hipStream_t st)
 {
-    if (count) *count = 0;
+    *count = 0;
     if (P.n_segs == 0) return CBX_OK;
     const int blk = 256;
     DevBuf<int> cnt;
@@ -839,13 +973,8 @@ static int emit_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_val, 
     CBX_HIP(hipStreamSynchronize(st));
     int rc = flags_to_status(fl);
     if (rc) return rc;
-    if (E == 0) return CBX_OK;
-    if (!image && !merge) {
-        *count = E;
-        if (E > capacity) return CBX_OK;
-    }
-    if (E >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "more than 2^31 - 1 pixel entries");
-
+    *count = E;
+    if (E == 0 || E > capacity) return CBX_OK;
     DevBuf<uint64_t> keys;
     DevBuf<T> vals;
     DevBuf<int32_t> lof;
@@ -856,63 +985,87 @@ static int emit_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_val, 
     launch_emit<T, true>(kernel, P, max_val, nullptr, off.ptr, keys.ptr, vals.ptr, want_lines ? lof.ptr : nullptr,
                          st);
     CBX_HIP(hipGetLastError());
+    k_raw_out<T><<<grid_for(E, blk), blk, 0, st>>>(keys.ptr, vals.ptr, lof.ptr, E, P.g.H, P.g.W, of, oy, ox, ov,
+                                                  want_lines ? ol : nullptr);
+    CBX_HIP(hipGetLastError());
+    CBX_HIP(hipStreamSynchronize(st));
+    return CBX_OK;
+}
 
-    if (!image && !merge) {
-        k_raw_out<T><<<grid_for(E, blk), blk, 0, st>>>(keys.ptr, vals.ptr, lof.ptr, E, P.g.H, P.g.W, of, oy, ox, ov,
-                                                      want_lines ? ol : nullptr);
-        CBX_HIP(hipGetLastError());
-        return CBX_OK;
+template <typename T, int OV, int TILE, int SEG, bool LIST>
+static void launch_ordered_k(int kernel, T *img, uint8_t *touched, int32_t *first, const SegmentPlan<T> &P,
+                             T max_val, hipStream_t st)
+{
+#define CBX_ORD(KK)                                                                                             \
+    k_tiles_ordered<T, KK, OV, TILE, SEG, LIST><<<(unsigned)P.n_tiles, 64, 0, st>>>(                            \
+        img, touched, first, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr)
+    switch (kernel) {
+    case K_BIWEIGHT: CBX_ORD(K_BIWEIGHT); break;
+    case K_GAUSSIAN: CBX_ORD(K_GAUSSIAN); break;
+    case K_PARABOLIC: CBX_ORD(K_PARABOLIC); break;
+    case K_RECTANGULAR: CBX_ORD(K_RECTANGULAR); break;
+    default: CBX_ORD(K_TRIANGULAR); break;
     }
+#undef CBX_ORD
+}
 
-    // stable radix sort of (pixel key, emission index): equal keys keep line order
-    const uint64_t max_key = (uint64_t)P.g.n_frames * (uint64_t)P.g.H * (uint64_t)P.g.W;
-    int end_bit = 1;
-    while (end_bit < 64 && (uint64_t(1) << end_bit) < max_key) end_bit++;
-    DevBuf<uint64_t> skeys;
-    DevBuf<uint32_t> perm, sperm;
-    CBX_HIP(skeys.alloc(E, st));
-    CBX_HIP(perm.alloc(E, st));
-    CBX_HIP(sperm.alloc(E, st));
-    k_iota_u32<<<grid_for(E, blk), blk, 0, st>>>(perm.ptr, E);
-    size_t tmp_bytes = 0;
-    CBX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.ptr, skeys.ptr, perm.ptr, sperm.ptr, (int)E, 0,
-                                               end_bit, st));
-    DevBuf<char> tmp;
-    CBX_HIP(tmp.alloc(tmp_bytes, st));
-    CBX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, keys.ptr, skeys.ptr, perm.ptr, sperm.ptr, (int)E,
-                                               0, end_bit, st));
-    if (image) {
-        if (overlap == 0)
-            k_reduce_runs<T, 0><<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, sperm.ptr, E, vals.ptr, nullptr, nullptr,
-                                                                  P.g.H, P.g.W, nullptr, nullptr, nullptr, nullptr,
-                                                                  nullptr, image);
-        else
-            k_reduce_runs<T, 1><<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, sperm.ptr, E, vals.ptr, nullptr, nullptr,
-                                                                  P.g.H, P.g.W, nullptr, nullptr, nullptr, nullptr,
-                                                                  nullptr, image);
-        CBX_HIP(hipGetLastError());
-        return CBX_OK;
+// Reference-order accumulation over a binned plan (tile 32, seg 16):
+// image != nullptr accumulates into the caller's frames; otherwise a merged
+// pixel list is compacted out of a dense scratch render.
+template <typename T>
+static int ordered_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_val, T *image, int32_t *of, int32_t *oy,
+                            int32_t *ox, T *ov, int32_t *ol, int64_t capacity, int64_t *count, hipStream_t st)
+{
+    if (count) *count = 0;
+    if (P.n_segs == 0) return CBX_OK;
+    k_sort_tile_items<<<(unsigned)P.n_tiles, 256, 0, st>>>(P.tile_off.ptr, P.items.ptr, P.flags.ptr);
+    CBX_HIP(hipGetLastError());
+    const int64_t npix = P.g.n_frames * (int64_t)P.g.H * P.g.W;
+    DevBuf<T> scratch;
+    DevBuf<uint8_t> touched;
+    DevBuf<int32_t> first;
+    T *img = image;
+    if (!image) {
+        CBX_HIP(scratch.alloc(npix, st));
+        CBX_HIP(touched.alloc(npix, st));
+        if (ol) CBX_HIP(first.alloc(npix, st));
+        // tiles without items never write their flags
+        CBX_HIP(hipMemsetAsync(touched.ptr, 0, npix, st));
+        img = scratch.ptr;
     }
-    DevBuf<int64_t> head, pos;
-    CBX_HIP(head.alloc(E + 1, st));
-    CBX_HIP(pos.alloc(E + 1, st));
-    CBX_HIP(hipMemsetAsync(head.ptr + E, 0, sizeof(int64_t), st));
-    k_heads<<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, E, head.ptr);
-    CBX_HIP(exclusive_scan_i64(head.ptr, pos.ptr, E + 1, st));
+    if (overlap == 0) {
+        if (image) launch_ordered_k<T, 0, 32, 16, false>(kernel, img, nullptr, nullptr, P, max_val, st);
+        else launch_ordered_k<T, 0, 32, 16, true>(kernel, img, touched.ptr, first.ptr, P, max_val, st);
+    } else {
+        if (image) launch_ordered_k<T, 1, 32, 16, false>(kernel, img, nullptr, nullptr, P, max_val, st);
+        else launch_ordered_k<T, 1, 32, 16, true>(kernel, img, touched.ptr, first.ptr, P, max_val, st);
+    }
+    CBX_HIP(hipGetLastError());
+    int fl = 0;
+    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    if (fl & FLAG_TILE_FULL)
+        return fail(CBX_ERUNTIME, "a tile holds more than 8192 segments; the ordered mode cannot sort it");
+    int rc = flags_to_status(fl);
+    if (rc || image) return rc;
+
+    const int blk = 256;
+    const int64_t rows = P.g.n_frames * (int64_t)P.g.H;
+    DevBuf<int64_t> rcnt, roff;
+    CBX_HIP(rcnt.alloc(rows + 1, st));
+    CBX_HIP(roff.alloc(rows + 1, st));
+    CBX_HIP(hipMemsetAsync(rcnt.ptr + rows, 0, sizeof(int64_t), st));
+    k_row_count<<<grid_for(rows, blk), blk, 0, st>>>(touched.ptr, rows, P.g.W, rcnt.ptr);
+    CBX_HIP(exclusive_scan_i64(rcnt.ptr, roff.ptr, rows + 1, st));
     int64_t U = 0;
-    CBX_HIP(hipMemcpyAsync(&U, pos.ptr + E, sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipMemcpyAsync(&U, roff.ptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
     CBX_HIP(hipStreamSynchronize(st));
     *count = U;
-    if (U > capacity) return CBX_OK;
-    if (overlap == 0)
-        k_reduce_runs<T, 0><<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, sperm.ptr, E, vals.ptr, lof.ptr, pos.ptr,
-                                                              P.g.H, P.g.W, of, oy, ox, ov, want_lines ? ol : nullptr,
-                                                              nullptr);
-    else
-        k_reduce_runs<T, 1><<<grid_for(E, blk), blk, 0, st>>>(skeys.ptr, sperm.ptr, E, vals.ptr, lof.ptr, pos.ptr,
-                                                              P.g.H, P.g.W, of, oy, ox, ov, want_lines ? ol : nullptr,
-                                                              nullptr);
+    if (U == 0 || U > capacity) return CBX_OK;
+    k_row_write<T><<<grid_for(rows, blk), blk, 0, st>>>(touched.ptr, img, first.ptr, rows, P.g.H, P.g.W, roff.ptr, of,
+                                                       oy, ox, ov, ol);
     CBX_HIP(hipGetLastError());
+    CBX_HIP(hipStreamSynchronize(st));
     return CBX_OK;
 }
 
@@ -928,11 +1081,15 @@ static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const
     if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
     hipStream_t st = (hipStream_t)stream;
     const bool det = opts && opts->deterministic;
+    if (det) {
+        tile = 32;
+        seg = 16;
+    }
     SegmentPlan<T> P;
-    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, !det, st))) return rc;
+    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
     if (P.n_segs == 0) return CBX_OK;
-    if (det) return emit_pipeline<T>(P, kernel, overlap, max_val, 1, out, nullptr, nullptr, nullptr, nullptr,
-                                     nullptr, 0, nullptr, st);
+    if (det) return ordered_pipeline<T>(P, kernel, overlap, max_val, out, nullptr, nullptr, nullptr, nullptr, nullptr,
+                                        0, nullptr, st);
     if (overlap == 0) launch_tiles_ov<T, 0>(kernel, out, P, max_val, st);
     else launch_tiles_ov<T, 1>(kernel, out, P, max_val, st);
     CBX_HIP(hipGetLastError());
@@ -951,8 +1108,9 @@ static int draw_pixels_impl(int64_t n_frames, int64_t H, int64_t W, const T *lin
     if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
     hipStream_t st = (hipStream_t)stream;
     SegmentPlan<T> P;
-    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, 64, 16, false, st))) return rc;
-    return emit_pipeline<T>(P, kernel, overlap, max_val, merge, nullptr, of, oy, ox, ov, ol, capacity, count, st);
+    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, 32, 16, merge != 0, st))) return rc;
+    if (merge) return ordered_pipeline<T>(P, kernel, overlap, max_val, nullptr, of, oy, ox, ov, ol, capacity, count, st);
+    return raw_list_pipeline<T>(P, kernel, max_val, of, oy, ox, ov, ol, capacity, count, st);
 }
 
 template <typename T, int K>

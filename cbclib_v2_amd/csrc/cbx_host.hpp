@@ -23,26 +23,25 @@ int fail(int code, const std::string &msg);
             return ::cbx::fail(CBX_ERUNTIME, std::string(#expr) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
-// Stream-ordered device allocation released on scope exit.
+// Device allocation released on scope exit (hipFree waits for the device,
+// so a buffer is never released under a kernel that still uses it).
 template <typename V>
 struct DevBuf {
     V *ptr = nullptr;
     size_t n = 0;
-    hipStream_t stream = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
-    hipError_t alloc(size_t count, hipStream_t s)
+    hipError_t alloc(size_t count, hipStream_t)
     {
         release();
-        stream = s;
         n = count;
         if (count == 0) return hipSuccess;
-        return hipMallocAsync((void **)&ptr, sizeof(V) * count, s);
+        return hipMalloc((void **)&ptr, sizeof(V) * count);
     }
     void release()
     {
-        if (ptr) (void)hipFreeAsync(ptr, stream);
+        if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         n = 0;
     }

@@ -28,10 +28,10 @@ static hipError_t exclusive_scan(const V *in, V *out, int64_t n, hipStream_t s)
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s);
     if (e != hipSuccess) return e;
     void *tmp = nullptr;
-    e = hipMallocAsync(&tmp, bytes ? bytes : 1, s);
+    e = hipMalloc(&tmp, bytes ? bytes : 1);
     if (e != hipSuccess) return e;
     e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, s);
-    hipError_t f = hipFreeAsync(tmp, s);
+    hipError_t f = hipFree(tmp);
     return (e != hipSuccess) ? e : f;
 }
 
