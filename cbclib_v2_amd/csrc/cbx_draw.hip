@@ -629,46 +629,48 @@ __global__ void k_row_write(const uint8_t *__restrict__ touched, const T *__rest
 // adjoint (sum overlap): per (segment, tile) partial gradients, then an
 // ordered per-line reduction
 // ---------------------------------------------------------------------------
+// Per-pixel gradient terms, in double: the triangular profile's derivative
+// carries the unit vector (p - q) / dist, which float loses near the line.
 template <typename T, int K>
 __device__ __forceinline__ void pixel_grad(const LineSetup<T> &L, int x, int y, T gv, T max_val, double *acc)
 {
-    const T tx = L.t[0], ty = L.t[1];
-    const T rx = (T)x - L.p0[0], ry = (T)y - L.p0[1];
-    const T mag = tx * tx + ty * ty;
-    T ts = (rx * tx + ry * ty) / mag;
-    ts = (ts < T(0)) ? T(0) : ((ts > T(1)) ? T(1) : ts);
-    const T dx = rx - ts * tx, dy = ry - ts * ty;
-    const T d2 = dx * dx + dy * dy;
-    const T w = L.w, w2 = L.w * L.w;
-    const T u = d2 / w2;
-    T dk_dd2, dk_dw;
+    const double tx = (double)L.t[0], ty = (double)L.t[1];
+    const double rx = (double)x - (double)L.p0[0], ry = (double)y - (double)L.p0[1];
+    const double mag = tx * tx + ty * ty;
+    double ts = (rx * tx + ry * ty) / mag;
+    ts = (ts < 0.0) ? 0.0 : ((ts > 1.0) ? 1.0 : ts);
+    const double dx = rx - ts * tx, dy = ry - ts * ty;
+    const double d2 = dx * dx + dy * dy;
+    const double w = (double)L.w, w2 = w * w;
+    const double u = d2 / w2;
+    double dk_dd2, dk_dw;
     if constexpr (K == K_TRIANGULAR) {
-        const T dist = m_sqrt(d2);
-        dk_dd2 = (dist > T(0)) ? T(-1) / (T(2) * w * dist) : T(0);
+        const double dist = sqrt(d2);
+        dk_dd2 = (dist > 0.0) ? -1.0 / (2.0 * w * dist) : 0.0;
         dk_dw = dist / w2;
     } else if constexpr (K == K_GAUSSIAN) {
-        const T k1 = T(-4.5) * T(0.3989422804014327) * (T)::exp(-4.5 * (double)u);
+        const double k1 = -4.5 * 0.3989422804014327 * ::exp(-4.5 * u);
         dk_dd2 = k1 / w2;
-        dk_dw = k1 * (T(-2) * u / w);
+        dk_dw = k1 * (-2.0 * u / w);
     } else if constexpr (K == K_BIWEIGHT) {
-        const T k1 = T(-1.875) * (T(1) - u);
+        const double k1 = -1.875 * (1.0 - u);
         dk_dd2 = k1 / w2;
-        dk_dw = k1 * (T(-2) * u / w);
+        dk_dw = k1 * (-2.0 * u / w);
     } else if constexpr (K == K_PARABOLIC) {
-        const T k1 = T(-0.75);
+        const double k1 = -0.75;
         dk_dd2 = k1 / w2;
-        dk_dw = k1 * (T(-2) * u / w);
+        dk_dw = k1 * (-2.0 * u / w);
     } else {
-        dk_dd2 = T(0);
-        dk_dw = T(0);
+        dk_dd2 = 0.0;
+        dk_dw = 0.0;
     }
-    const T sg = gv * max_val;
-    const T c = sg * dk_dd2 * T(-2);
-    acc[0] += (double)(c * (T(1) - ts) * dx);
-    acc[1] += (double)(c * (T(1) - ts) * dy);
-    acc[2] += (double)(c * ts * dx);
-    acc[3] += (double)(c * ts * dy);
-    acc[4] += (double)(sg * dk_dw);
+    const double sg = (double)gv * (double)max_val;
+    const double c = sg * dk_dd2 * -2.0;
+    acc[0] += c * (1.0 - ts) * dx;
+    acc[1] += c * (1.0 - ts) * dy;
+    acc[2] += c * ts * dx;
+    acc[3] += c * ts * dy;
+    acc[4] += sg * dk_dw;
 }
 
 template <typename T, int K, int TILE, int SEG, int NT, int CH>

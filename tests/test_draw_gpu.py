@@ -33,10 +33,10 @@ def case(seed, dtype=np.float32):
     return (F, H, W), lines.astype(dtype), idxs.astype(it)
 
 
-def gpu_image(shape, lines, idxs, kernel, overlap, dtype, init=0.0, **kw):
+def gpu_image(shape, lines, idxs, kernel, overlap, dtype, init=0.0, max_val=1.7, **kw):
     out = torch.full(shape, init, dtype=torch.float64 if dtype == np.float64 else torch.float32, device='cuda')
     ndimage.draw_lines(out, torch.from_numpy(lines).cuda(), None if idxs is None else torch.from_numpy(idxs).cuda(),
-                       1.7, kernel, overlap, **kw)
+                       max_val, kernel, overlap, **kw)
     return out.cpu().numpy()
 
 
@@ -121,9 +121,9 @@ def test_config1_parity(cuda, oracle):
     """Config 1: one 512x512 frame, 1000 segments, triangular, sum; plus its pixel list."""
     lines = config1()
     ref = oracle.draw_lines(np.zeros((512, 512), np.float32), lines, None, 1.0, 'triangular', 'sum')
-    det = gpu_image((512, 512), lines, None, 'triangular', 'sum', np.float32, deterministic=True)
+    det = gpu_image((512, 512), lines, None, 'triangular', 'sum', np.float32, max_val=1.0, deterministic=True)
     assert np.array_equal(det, ref)
-    fast = gpu_image((512, 512), lines, None, 'triangular', 'sum', np.float32)
+    fast = gpu_image((512, 512), lines, None, 'triangular', 'sum', np.float32, max_val=1.0)
     np.testing.assert_allclose(fast, ref, rtol=RTOL, atol=ATOL)
     raw = oracle.pixel_list((512, 512), lines, None, 1.0, 'triangular')
     pl = ndimage.draw_pixels((512, 512), torch.from_numpy(lines).cuda(), None, 1.0, 'triangular', merge=False)
@@ -135,7 +135,7 @@ def test_reference_module_on_box(cuda, ref_bresenham):
     """When the compiled reference travelled with the tree, compare against it directly."""
     lines = config1()
     ref = ref_bresenham.draw_lines(np.zeros((512, 512), np.float32), lines, None, 1.0, 'rectangular', 'sum', 1)
-    det = gpu_image((512, 512), lines, None, 'rectangular', 'sum', np.float32, deterministic=True)
+    det = gpu_image((512, 512), lines, None, 'rectangular', 'sum', np.float32, max_val=1.0, deterministic=True)
     assert np.array_equal(det, ref)
 
 
