@@ -1,0 +1,227 @@
+#!/usr/bin/env python3
+"""Benchmark of the streak-rasterization hot path on one MI355X.
+
+Default workload (plain forward + label, 64 frames of 2048 x 2048 float32):
+128 000 segments, 2 000 per frame, lengths U[20, 400] px, random angles,
+widths U[2, 6] px, gaussian profile, sum overlap; the render is thresholded
+at 0.5 and labelled with per-frame 8-connectivity (Structure([0, 1, 1], 2)).
+One step = zero the frames + draw_lines + label(out > 0.5), inputs resident
+in HBM.  value = segments per second of the whole step (Mlines/s).
+
+Prints ONE JSON line.  Extra fields: forward-only rate, label time, the
+roofline of the dominant kernel (the tile kernel, timed with HIP events on
+its launch stream) and the reference CPU path timed on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def fail_multi(n: int) -> None:
+    print(json.dumps({"error": f"--gpus {n}: this build measures one GPU only; the path shards by frame "
+                      "(see the sharding paragraph of the design notes) and no multi-GPU run is made"}))
+    sys.exit(2)
+
+
+def cpu_baseline(lines, idxs, frames, H, W, seconds):
+    """Reference CPU path (the compiled reference modules, else the oracle port) on a bounded sample."""
+    import numpy as np
+    from oracle import oracle as O
+    ref = O.ref_module('bresenham')
+    lab = O.ref_module('label')
+    cores = min(16, os.cpu_count() or 1)
+    per_frame = int(np.sum(idxs == 0))
+    sample_frames = 2
+    sel = idxs < sample_frames
+    ln, ix = lines[sel], idxs[sel]
+    out = np.zeros((sample_frames, H, W), np.float32)
+    kind = 'reference' if ref is not None else 'port'
+    runs, t_draw, t_label = 0, 0.0, 0.0
+    t_start = time.perf_counter()
+    while runs == 0 or time.perf_counter() - t_start < seconds:
+        out[...] = 0
+        t0 = time.perf_counter()
+        if ref is not None:
+            ref.draw_lines(out, ln, ix, 1.0, 'gaussian', 'sum', cores)
+        else:
+            O.draw_lines(out, ln, ix, 1.0, 'gaussian', 'sum')
+        t1 = time.perf_counter()
+        mask = out > 0.5
+        if lab is not None:
+            lab.label(mask, lab.Structure([0, 1, 1], 2), 1, cores)
+        else:
+            from cbclib_v2_amd.label import Structure
+            O.label(mask, Structure([0, 1, 1], 2).shifts(), 1)
+        t2 = time.perf_counter()
+        t_draw += t1 - t0
+        t_label += t2 - t1
+        runs += 1
+    n = runs * len(ln)
+    return {"value": round(n / (t_draw + t_label) / 1e6, 6), "unit": "Mlines/s", "cores": cores if kind == 'reference'
+            else 1, "kind": kind,
+            "sample": f"{sample_frames} of the {frames} frames ({len(ln)} segments, {per_frame} per frame), "
+                      f"draw_lines + label of out > 0.5, {runs} runs in {t_draw + t_label:.1f} s; "
+                      f"draw_lines alone {n / t_draw / 1e6:.4f} Mlines/s"}
+
+
+def sweep(out, lines, idxs, F, H, W, N):
+    """Tile-kernel time over launch shapes (tile, seg, threads) and value modes."""
+    import torch
+    from cbclib_v2_amd import _native, ndimage
+    rows = []
+    shapes = [(t, s, n, c) for t in (32, 64) for s in (8, 16) for n, c in ((64, 64), (128, 64), (256, 64), (256, 256))]
+    for exact in (False, True):
+        for tile, seg, threads, chunk in shapes:
+            if True:
+                if True:
+                    kw = dict(tile=tile, seg=seg, threads=threads, chunk=chunk, exact_values=exact)
+                    for _ in range(2):
+                        ndimage.draw_lines(out, lines, idxs, 1.0, 'gaussian', 'sum', **kw)
+                    torch.cuda.synchronize()
+                    _native.timing_reset()
+                    _native.timing_enable(True)
+                    for _ in range(5):
+                        ndimage.draw_lines(out, lines, idxs, 1.0, 'gaussian', 'sum', **kw)
+                    torch.cuda.synchronize()
+                    _native.timing_enable(False)
+                    ms, n = _native.timing_read('draw_tiles')
+                    avg = ms / max(n, 1)
+                    gbs = (8.0 * F * H * W + 24.0 * N) / (avg * 1e-3) / 1e9
+                    rows.append(dict(kw, ms=round(avg, 4), GBs=round(gbs, 1)))
+                    print(json.dumps(rows[-1]), flush=True)
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=10)
+    ap.add_argument('--warmup', type=int, default=3)
+    ap.add_argument('--frames', type=int, default=64)
+    ap.add_argument('--tile', type=int, default=0)
+    ap.add_argument('--seg', type=int, default=0)
+    ap.add_argument('--threads', type=int, default=0)
+    ap.add_argument('--sweep', action='store_true', help='time the tile kernel over launch shapes and exit')
+    ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
+    ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    args = ap.parse_args()
+    world = int(os.environ.get('WORLD_SIZE', '1'))
+    if args.gpus != 1 or world != 1:
+        fail_multi(max(args.gpus, world))
+
+    import numpy as np
+    import torch
+    from cbclib_v2_amd import _native, label as cl, ndimage
+    from tests.lines_gen import config2
+
+    dev = torch.device('cuda', 0)
+    F, H, W, per = args.frames, 2048, 2048, 2000
+    lines_np, idxs_np = config2(frames=F, per_frame=per, H=H, W=W)
+    N = lines_np.shape[0]
+    lines = torch.from_numpy(lines_np).to(dev)
+    idxs = torch.from_numpy(idxs_np).to(dev)
+    out = torch.empty((F, H, W), dtype=torch.float32, device=dev)
+    st = cl.Structure([0, 1, 1], 2)
+    kw = {}
+    if args.tile:
+        kw['tile'] = args.tile
+    if args.seg:
+        kw['seg'] = args.seg
+    if args.threads:
+        kw['threads'] = args.threads
+    if args.sweep:
+        sweep(out, lines, idxs, F, H, W, N)
+        return
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    acc = {'zero': 0.0, 'fwd': 0.0, 'label': 0.0}
+    n_labels = 0
+
+    def step(measure: bool):
+        nonlocal n_labels
+        if measure:
+            ev[0].record()
+        out.zero_()
+        if measure:
+            ev[1].record()
+        ndimage.draw_lines(out, lines, idxs, 1.0, 'gaussian', 'sum', **kw)
+        if measure:
+            ev[2].record()
+        res = cl.label_threshold(out, 0.5, st)
+        n_labels = int(res.index.numel())
+        if measure:
+            ev[3].record()
+            torch.cuda.synchronize()
+            acc['zero'] += ev[0].elapsed_time(ev[1])
+            acc['fwd'] += ev[1].elapsed_time(ev[2])
+            acc['label'] += ev[2].elapsed_time(ev[3])
+
+    for _ in range(args.warmup):
+        step(False)
+    torch.cuda.synchronize()
+    _native.timing_reset()
+    _native.timing_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(False)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _native.timing_enable(False)
+    tiles_ms, tiles_n = _native.timing_read('draw_tiles')
+    label_ms, label_n = _native.timing_read('label_total')
+    pass1_ms, pass1_n = _native.timing_read('segments_pass1')
+    # per-part breakdown on separate, untimed-for-value steps
+    for _ in range(2):
+        step(True)
+
+    ms_step = (t1 - t0) * 1e3 / args.steps
+    tile_avg = tiles_ms / max(tiles_n, 1)
+    fwd_bytes = 8.0 * F * H * W + 24.0 * N
+    achieved = fwd_bytes / (tile_avg * 1e-3) / 1e9
+    mask_density = float((out > 0.5).float().mean())
+    result = {
+        "metric": "draw_lines Mlines/s and fwd+bwd frames/s on 1 MI355X; % HBM peak; vs CPU",
+        "value": round(N / (ms_step * 1e-3) / 1e6, 4),
+        "unit": "Mlines/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plain forward + label: {F} frames of {H}x{W} f32, {N} segments ({per}/frame, "
+                               "L U[20,400], w U[2,6], random angles, seed 1), gaussian, sum; label(out > 0.5, "
+                               "Structure([0,1,1],2)); step = zero + draw_lines + label",
+                   "frames": F, "segments": N, "tile": args.tile or 64, "seg": args.seg or 16},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_tiles (draw_lines tile accumulation)",
+                     "bytes_per_launch": fwd_bytes, "avg_launch_ms": round(tile_avg, 4), "launches": tiles_n},
+        "fwd_Mlines_per_s": round(N / (acc['fwd'] / 2 * 1e-3) / 1e6, 4),
+        "frames_per_s": round(F / (ms_step * 1e-3), 2),
+        "breakdown_ms": {k: round(v / 2, 4) for k, v in acc.items()},
+        "segments_pass1_ms": round(pass1_ms / max(pass1_n, 1), 4),
+        "label_total_ms": round(label_ms / max(label_n, 1), 4),
+        "label": {"n_labels": n_labels, "mask_density": round(mask_density, 6),
+                  "algorithmic_GBs": round(8.0 * F * H * W / (label_ms / max(label_n, 1) * 1e-3) / 1e9, 2)},
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(lines_np, idxs_np, F, H, W, args.cpu_seconds)
+    else:
+        result["cpu_baseline"] = None
+    print(json.dumps(result))
+
+
+if __name__ == '__main__':
+    main()

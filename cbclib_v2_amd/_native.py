@@ -21,13 +21,15 @@ DTYPES = {'bool': 0, 'int32': 1, 'int64': 2}
 
 EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32', 'cbx_draw_lines_f64',
            'cbx_draw_pixels_f32', 'cbx_draw_pixels_f64', 'cbx_draw_lines_adjoint_f32',
-           'cbx_draw_lines_adjoint_f64', 'cbx_label', 'cbx_label_threshold_f32')
+           'cbx_draw_lines_adjoint_f64', 'cbx_label', 'cbx_label_threshold_f32', 'cbx_timing_enable',
+           'cbx_timing_reset', 'cbx_timing_read')
 
 
 class DrawOpts(ctypes.Structure):
-    """``cbx_draw_opts``: tile side, columns per walk checkpoint, deterministic flag."""
+    """``cbx_draw_opts``: tile side, columns per walk checkpoint, deterministic
+    flag, tile-kernel workgroup size, exact-value flag."""
     _fields_ = [('tile', ctypes.c_int32), ('seg', ctypes.c_int32), ('deterministic', ctypes.c_int32),
-                ('reserved', ctypes.c_int32)]
+                ('threads', ctypes.c_int32), ('exact_values', ctypes.c_int32), ('chunk', ctypes.c_int32)]
 
 
 _lib = None
@@ -59,6 +61,12 @@ def _declare(L: ctypes.CDLL) -> None:
     L.cbx_label.argtypes = [P, I32, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
     L.cbx_label_threshold_f32.restype = I32
     L.cbx_label_threshold_f32.argtypes = [P, F, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
+    L.cbx_timing_enable.restype = None
+    L.cbx_timing_enable.argtypes = [I32]
+    L.cbx_timing_reset.restype = None
+    L.cbx_timing_reset.argtypes = []
+    L.cbx_timing_read.restype = I32
+    L.cbx_timing_read.argtypes = [C, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
 
 
 def lib() -> ctypes.CDLL:
@@ -101,3 +109,20 @@ def overlap_id(overlap: str) -> int:
         # bresenham.cpp:148
         raise ValueError(f"Invalid overlap keyword: {overlap}")
     return OVERLAPS[overlap]
+
+
+def timing_enable(on: bool = True) -> None:
+    """Record HIP events around the library's main kernels (see cbx_timing_read)."""
+    lib().cbx_timing_enable(1 if on else 0)
+
+
+def timing_reset() -> None:
+    lib().cbx_timing_reset()
+
+
+def timing_read(name: str):
+    """(total milliseconds, launches) of kernel ``name`` since the last reset."""
+    ms = ctypes.c_double(0.0)
+    n = ctypes.c_int64(0)
+    lib().cbx_timing_read(name.encode(), ctypes.byref(ms), ctypes.byref(n))
+    return ms.value, n.value

@@ -45,6 +45,14 @@ __device__ __forceinline__ double m_ceil(double x) { return __builtin_ceil(x); }
 __device__ __forceinline__ float m_sqrt(float x) { return __builtin_sqrtf(x); }
 __device__ __forceinline__ double m_sqrt(double x) { return __builtin_sqrt(x); }
 
+// element i (0 or 1) of a two-element array without dynamic indexing, which
+// would move the array to scratch memory
+template <typename V>
+__device__ __forceinline__ V pick(const V (&v)[2], int i)
+{
+    return i ? v[1] : v[0];
+}
+
 // ---------------------------------------------------------------------------
 // correctly rounded quotient from a correctly rounded reciprocal
 // ---------------------------------------------------------------------------
@@ -65,6 +73,14 @@ __device__ __forceinline__ double div_rn(double a, double b, double rb)
     return (__builtin_fabs(q) < 1e-290) ? a / b : q1;
 }
 
+// Same without the guard, for arguments known to give a normal quotient.
+__device__ __forceinline__ float div_rn_nz(float a, float b, float rb)
+{
+    const float q = a * rb;
+    const float r = __builtin_fmaf(-q, b, a);
+    return __builtin_fmaf(r, rb, q);
+}
+
 // ---------------------------------------------------------------------------
 // per-line constants (everything the walk, the fills and the error need)
 // ---------------------------------------------------------------------------
@@ -76,12 +92,14 @@ struct alignas(16) LineSetup {
     T LL;          // RN(len * len) (TangentError::error_at)
     T w, w2;       // width and RN(w * w)
     double rw2;    // RN64(1 / (double)w2)
+    float rw2f;    // RN32(1 / w2) for the fast error estimate
     int32_t m1[2]; // walk end (exclusive, "any coordinate equal" test)
     int32_t m0[2]; // walk start
     int32_t frame;
     int8_t axis;   // long axis (0 = x, 1 = y)
     int8_t st[2];  // step per axis (+1 / -1)
     int8_t valid;  // 0: the line emits nothing
+    int8_t safe;   // 1: quotients of its errors by len cannot be subnormal (see line_setup)
     T e0[2];       // tangent and normal errors at m0
     T p0[2];       // first end point (for the adjoint's closed-form derivatives)
 };
@@ -115,6 +133,7 @@ __device__ __forceinline__ int line_setup(const T *ln, LineSetup<T> &L)
     L.LL = len * len;
     L.w2 = w * w;
     L.rw2 = 1.0 / (double)L.w2;
+    L.rw2f = (float)(1.0 / (double)L.w2);
     int axis = (m_abs(L.t[0]) < m_abs(L.t[1])) ? 1 : 0;
     L.axis = (int8_t)axis;
     long offset = (long)(m_ceil(w) + T(1));
@@ -141,6 +160,12 @@ __device__ __forceinline__ int line_setup(const T *ln, LineSetup<T> &L)
     L.e0[0] = Te;
     L.e0[1] = Ne;
     L.valid = 1;
+    // With every end point coordinate 0 or at least 2^-20 in magnitude (and
+    // w >= 2^-20), each walk error is 0 or far above the subnormal range, so
+    // the reciprocal-based division needs no tiny-quotient guard.
+    const T lim = T(1.0 / 1048576.0);
+    auto ok = [&](T v) { return v == T(0) || m_abs(v) >= lim; };
+    L.safe = (int8_t)(ok(x0) && ok(y0) && ok(x1) && ok(y1) && w >= lim);
     return 0;
 }
 
@@ -209,29 +234,37 @@ __device__ __forceinline__ double nsq_of(const LineSetup<double> &L, double Ne)
     return qn * qn;
 }
 
-__device__ __forceinline__ float err_of(const LineSetup<float> &L, float Te, float Ne)
+// tangential overshoot max(-T, T - len^2, 0) (TangentError::error_at)
+template <typename T>
+__device__ __forceinline__ T tangent_at(const LineSetup<T> &L, T Te)
 {
-    float a = -Te, b = Te - L.LL;
-    float m = (a < b) ? b : a;
-    float at = (m < 0.0f) ? 0.0f : m;
-    float qt = div_rn(at, L.len, L.rlen);
-    float nsq = nsq_of(L, Ne);
+    const T a = -Te, b = Te - L.LL;
+    const T m = (a < b) ? b : a;
+    return (m < T(0)) ? T(0) : m;
+}
+
+// exact error from the tangential overshoot and the normal error's square
+__device__ __forceinline__ float err_exact(const LineSetup<float> &L, float at, float nsq)
+{
+    const float qt = div_rn(at, L.len, L.rlen);
     // RN64(qt^2 + nsq): the square of a float is exact in double, so one FMA
     // rounds exactly like the reference's double addition
-    double e = __builtin_fma((double)qt, (double)qt, (double)nsq);
+    const double e = __builtin_fma((double)qt, (double)qt, (double)nsq);
     return (float)div_rn(e, (double)L.w2, L.rw2);
 }
 
-__device__ __forceinline__ double err_of(const LineSetup<double> &L, double Te, double Ne)
+__device__ __forceinline__ double err_exact(const LineSetup<double> &L, double at, double nsq)
 {
-    double a = -Te, b = Te - L.LL;
-    double m = (a < b) ? b : a;
-    double at = (m < 0.0) ? 0.0 : m;
-    double qt = div_rn(at, L.len, L.rlen);
-    double nsq = nsq_of(L, Ne);
-    double qt2 = qt * qt;
-    double e = qt2 + nsq;
+    const double qt = div_rn(at, L.len, L.rlen);
+    const double qt2 = qt * qt;
+    const double e = qt2 + nsq;
     return div_rn(e, L.w2, L.rw2);
+}
+
+template <typename T>
+__device__ __forceinline__ T err_of(const LineSetup<T> &L, T Te, T Ne)
+{
+    return err_exact(L, tangent_at(L, Te), nsq_of(L, Ne));
 }
 
 // ---------------------------------------------------------------------------
@@ -301,6 +334,57 @@ template <int K>
 __device__ __forceinline__ double pixel_value(double err, double max_val)
 {
     return max_val * profile<K>(m_sqrt(err));
+}
+
+// Fast profile values from u = err = x^2 (float arithmetic, a few ulp from
+// the reference's values; used by the tile path, never by the exact modes).
+template <int K>
+__device__ __forceinline__ float profile_fast(float u)
+{
+    if constexpr (K == K_BIWEIGHT) {
+        const float m = 1.0f - u;
+        return 0.9375f * m * m;
+    } else if constexpr (K == K_GAUSSIAN) {
+        return 0.3989422804014327f * __expf(-4.5f * u);
+    } else if constexpr (K == K_PARABOLIC) {
+        return 0.75f * (1.0f - u);
+    } else if constexpr (K == K_RECTANGULAR) {
+        return 1.0f;
+    } else {
+        const float m = 1.0f - __builtin_amdgcn_sqrtf(u);
+        return (m < 0.0f) ? 0.0f : m;
+    }
+}
+
+// Decides whether a pixel is emitted (exactly as the reference: error <= 1)
+// and computes its value.  FAST: the error is estimated with reciprocal
+// products and recomputed exactly only within 4e-6 of the threshold (the
+// estimate is within 1e-6 of the exact value); values from profile_fast.
+template <bool FAST, int K>
+__device__ __forceinline__ bool pixel_eval(const LineSetup<float> &L, float Te, float nsq, float max_val, float &v)
+{
+    const float at = tangent_at(L, Te);
+    if constexpr (FAST) {
+        const float qt = at * L.rlen;
+        float e = __builtin_fmaf(qt, qt, nsq) * L.rw2f;
+        if (__builtin_fabsf(e - 1.0f) <= 4e-6f) e = err_exact(L, at, nsq);
+        if (!(e <= 1.0f)) return false;
+        v = max_val * profile_fast<K>(e);
+    } else {
+        const float e = err_exact(L, at, nsq);
+        if (!(e <= 1.0f)) return false;
+        v = pixel_value<K>(e, max_val);
+    }
+    return true;
+}
+
+template <bool FAST, int K>
+__device__ __forceinline__ bool pixel_eval(const LineSetup<double> &L, double Te, double nsq, double max_val, double &v)
+{
+    const double e = err_exact(L, tangent_at(L, Te), nsq);
+    if (!(e <= 1.0)) return false;
+    v = pixel_value<K>(e, max_val);
+    return true;
 }
 
 // dK/du at u = err = (dist / w)^2 for the adjoint; the triangular profile is

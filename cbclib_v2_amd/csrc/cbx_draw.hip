@@ -94,8 +94,8 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
 
     const int a = L.axis, b = 1 - a;
     const int da = (a == 0) ? g.W : g.H, db = (a == 0) ? g.H : g.W;
-    const int sa = L.st[a];
-    const double ta = (double)m_abs(L.t[a]);
+    const int sa = pick(L.st, a);
+    const double ta = (double)m_abs(pick(L.t, a));
     const double wl = (double)L.w * (double)L.len;
 
     WalkState<T> s;
@@ -134,14 +134,14 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
 
     // every step moves the walk; a walk takes at most one short-axis-only
     // step per long-axis step, so this budget is never reached by a valid line
-    const int64_t budget = 4 * ((int64_t)abs(L.m1[a] - L.m0[a]) + abs(L.m1[b] - L.m0[b])) + 64;
+    const int64_t budget = 4 * ((int64_t)abs(pick(L.m1, a) - pick(L.m0, a)) + abs(pick(L.m1, b) - pick(L.m0, b))) + 64;
     for (int64_t it = 0;; it++) {
         if (it == budget) {
             atomicOr(flags, FLAG_STUCK);
             break;
         }
         if (walk_done(L, s)) break;
-        const int c = s.c[a];
+        const int c = pick(s.c, a);
         if ((sa > 0) ? (c >= da) : (c < 0)) break; // no in-frame column is left
         bool nx, ny;
         walk_next(L, s.Ne, nx, ny);
@@ -166,7 +166,7 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
             }
             if ((a == 0) ? nx : ny) {
                 nem++;
-                const int sb = s.c[b];
+                const int sb = pick(s.c, b);
                 smin = min(smin, sb);
                 smax = max(smax, sb);
                 maxN = fmax(maxN, fabs((double)s.Ne));
@@ -205,22 +205,22 @@ template <typename T, int SEG>
 __device__ __forceinline__ void walk_columns(const LineSetup<T> &L, const Segment<T> &sg, ColRec<T> *rr, int *flags)
 {
     for (int k = 0; k < SEG; k++) rr[k].s = INT_MIN;
-    const int a = L.axis, b = 1 - a, sa = L.st[a];
+    const int a = L.axis, b = 1 - a, sa = pick(L.st, a);
     WalkState<T> s;
     s.c[0] = sg.c[0];
     s.c[1] = sg.c[1];
     s.Te = sg.Te;
     s.Ne = sg.Ne;
-    const int c0 = s.c[a];
+    const int c0 = pick(s.c, a);
     int it = 0;
     for (; it < 4 * SEG + 16; it++) {
         if (walk_done(L, s)) break;
-        const int k = (s.c[a] - c0) * sa;
+        const int k = (pick(s.c, a) - c0) * sa;
         if (k >= sg.ncols) break;
         bool nx, ny;
         walk_next(L, s.Ne, nx, ny);
         if ((a == 0) ? nx : ny) {
-            rr[k].s = s.c[b];
+            rr[k].s = pick(s.c, b);
             rr[k].Te = s.Te;
             rr[k].Ne = s.Ne;
         }
@@ -238,8 +238,8 @@ __device__ __forceinline__ void fill_column(const LineSetup<T> &L, const ColRec<
 {
     const int b = 1 - L.axis;
     if (cr.s >= s_lo && cr.s < s_hi) emit(cr.s, cr.Te, cr.Ne);
-    const T tb = L.t[b], nb = L.nv[b];
-    const int sb = L.st[b];
+    const T tb = pick(L.t, b), nb = pick(L.nv, b);
+    const int sb = pick(L.st, b);
 #pragma unroll
     for (int half = 0; half < 2; half++) {
         const int dir = half ? -sb : sb;
@@ -273,17 +273,72 @@ struct AccType<double, 1> {
 // ---------------------------------------------------------------------------
 // 4. tile kernel (image mode)
 // ---------------------------------------------------------------------------
-template <typename T, int K, int OV, int TILE, int SEG, int NT, int CH>
+// The fills of one column half for the float fast path: constants held in
+// registers, the tile window turned into a trip count, no division guard
+// (only for lines with L.safe).  Accumulates into the LDS tile.
+template <int K, int OV, typename Acc>
+__device__ __forceinline__ void fill_half_fast(Acc *img, const LineSetup<float> &L, int cs, float Te, float Ne,
+                                               int half, int cbase, int sstride, int s_lo, int s_hi, float max_val)
+{
+    const float len = L.len, rlen = L.rlen, LL = L.LL, w2 = L.w2, rw2f = L.rw2f;
+    const int b = 1 - L.axis;
+    const int sb = pick(L.st, b);
+    const int dir = half ? -sb : sb;
+    const float tb = pick(L.t, b), nb = pick(L.nv, b);
+    const float dT = (dir > 0) ? tb : -tb, dN = (dir > 0) ? nb : -nb;
+    auto put = [&](int s, float T, float nsq) {
+        const float a0 = -T, a1 = T - LL;
+        float at = (a0 < a1) ? a1 : a0;
+        at = (at < 0.0f) ? 0.0f : at;
+        const float qt = at * rlen;
+        float e = __builtin_fmaf(qt, qt, nsq) * rw2f;
+        if (__builtin_fabsf(e - 1.0f) <= 4e-6f) e = err_exact(L, at, nsq);
+        if (e <= 1.0f) {
+            const float v = max_val * profile_fast<K>(e);
+            Acc *px = &img[cbase + (s - s_lo) * sstride];
+            if constexpr (OV == 0) {
+                atomicAdd(px, v);
+            } else {
+                if (v == v) atomicMax(px, ord_enc(v));
+            }
+        }
+    };
+    if (half == 0 && cs >= s_lo && cs < s_hi) {
+        const float q = div_rn_nz(Ne, len, rlen);
+        put(cs, Te, q * q);
+    }
+    // steps to the last in-window pixel in the direction of travel, and the
+    // first step that lands inside the window
+    const int nsteps = (dir > 0) ? (s_hi - 1 - cs) : (cs - s_lo);
+    const int jin = (dir > 0) ? (s_lo - cs) : (cs - s_hi + 1);
+    for (int j = 1; j <= nsteps; j++) {
+        Te += dT;
+        Ne += dN;
+        const float q = div_rn_nz(Ne, len, rlen);
+        const float nsq = q * q;
+        if (!(nsq < w2)) break;
+        if (j >= jin) put(cs + j * dir, Te, nsq);
+    }
+}
+
+// Tile kernel.  The workgroup takes its tile's items in chunks of CH <= NT:
+// phase 1 gives each of CH threads one item (the walk replay, column records
+// into LDS); phase 2 gives each thread one (item, column, half) unit of the
+// fills at a time.
+template <typename T, int K, int OV, int TILE, int SEG, int NT, int CH, bool FAST>
 __global__ void __launch_bounds__(NT)
 k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const int *__restrict__ items,
         const Segment<T> *__restrict__ segs, const LineSetup<T> *__restrict__ setup, T max_val,
         int *__restrict__ flags)
 {
     using Acc = typename AccType<T, OV>::type;
-    constexpr int NW = NT / 64;
     __shared__ Acc img[TILE * TILE];
-    __shared__ ColRec<T> rec[NW][CH * SEG];
-    __shared__ ItemHdr hdr[NW][CH];
+    static_assert(CH <= NT, "one phase-1 thread per item");
+    __shared__ int32_t rs[CH * SEG];
+    __shared__ T rT[CH * SEG];
+    __shared__ T rN[CH * SEG];
+    __shared__ int32_t hline[CH];
+    __shared__ int32_t hc0[CH];
 
     const int tile = blockIdx.x;
     const int beg = tile_off[tile], end = tile_off[tile + 1];
@@ -294,66 +349,148 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     const int ty = r / g.ntx, tx = r - ty * g.ntx;
     const int X0 = tx * TILE, Y0 = ty * TILE;
     const int X1 = min(X0 + TILE, g.W), Y1 = min(Y0 + TILE, g.H);
+    const int tid = threadIdx.x;
 
-    for (int i = threadIdx.x; i < TILE * TILE; i += NT) {
+    for (int i = tid; i < TILE * TILE; i += NT) {
         if constexpr (OV == 0) img[i] = T(-0.0); // additive identity, keeps -0.0 pixels intact
         else img[i] = Acc(0);                     // below every encoded value
     }
-    __syncthreads();
 
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    ColRec<T> *R = rec[wave];
-    for (int first = beg + wave * CH; first < end; first += NW * CH) {
+    for (int first = beg; first < end; first += CH) {
         const int n = min(CH, end - first);
-        if (lane < n) {
-            const int sid = items[first + lane];
-            const Segment<T> sg = segs[sid];
+        __syncthreads();
+        // phase 1: replay the walk of one item from its checkpoint and record
+        // the centre and errors of every emitted column
+        if (tid < n) {
+            const Segment<T> sg = segs[items[first + tid]];
             const LineSetup<T> L = setup[sg.line];
-            hdr[wave][lane].seg = sid;
-            hdr[wave][lane].line = sg.line;
-            walk_columns<T, SEG>(L, sg, R + lane * SEG, flags);
+            const int a = L.axis, sa = pick(L.st, a);
+            hline[tid] = sg.line;
+            const int c0 = pick(sg.c, a);
+            hc0[tid] = c0;
+            int32_t *ps = &rs[tid * SEG];
+            T *pT = &rT[tid * SEG];
+            T *pN = &rN[tid * SEG];
+#pragma unroll
+            for (int k = 0; k < SEG; k++) ps[k] = INT_MIN;
+            WalkState<T> s;
+            s.c[0] = sg.c[0];
+            s.c[1] = sg.c[1];
+            s.Te = sg.Te;
+            s.Ne = sg.Ne;
+            int it = 0;
+            for (; it < 4 * SEG + 16; it++) {
+                if (walk_done(L, s)) break;
+                const int k = (pick(s.c, a) - c0) * sa;
+                if (k >= sg.ncols) break;
+                bool nx, ny;
+                walk_next(L, s.Ne, nx, ny);
+                if ((a == 0) ? nx : ny) {
+                    ps[k] = pick(s.c, 1 - a);
+                    pT[k] = s.Te;
+                    pN[k] = s.Ne;
+                }
+                walk_step(L, s, nx, ny);
+            }
+            if (it == 4 * SEG + 16) atomicOr(flags, FLAG_STUCK);
         }
-        wave_sync();
-        for (int u = lane; u < n * SEG; u += 64) {
-            const int i = u / SEG, k = u - i * SEG;
-            const ColRec<T> cr = R[u];
-            if (cr.s == INT_MIN) continue;
-            const Segment<T> &sg = segs[hdr[wave][i].seg];
-            const LineSetup<T> &L = setup[hdr[wave][i].line];
-            const int a = L.axis;
-            const int c = sg.c[a] + k * L.st[a];
+        __syncthreads();
+        // phase 2: one thread per (item, column, half); the centre goes with
+        // the half that steps in +st[b] (bresenham.hpp:364-380)
+        for (int u = tid; u < n * SEG * 2; u += NT) {
+            const int i = u / (2 * SEG), rem = u - i * (2 * SEG);
+            const int k = rem >> 1, half = rem & 1;
+            const int cs = rs[i * SEG + k];
+            if (cs == INT_MIN) continue;
+            const LineSetup<T> L = setup[hline[i]];
+            const int a = L.axis, b = 1 - a;
+            const int c = hc0[i] + k * pick(L.st, a);
             const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
-            fill_column(L, cr, s_lo, s_hi, [&](int s, T Te, T Ne) {
-                const T err = err_of(L, Te, Ne);
-                if (!(err <= T(1))) return;
-                const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
-                const T v = pixel_value<K>(err, max_val);
-                Acc *px = &img[(y - Y0) * TILE + (x - X0)];
+            // LDS stride of the short axis and offset of the long-axis column
+            const int sstride = (a == 0) ? TILE : 1;
+            const int cbase = (a == 0) ? (c - X0) : (c - Y0) * TILE;
+            T Te = rT[i * SEG + k], Ne = rN[i * SEG + k];
+            if constexpr (FAST && sizeof(T) == 4) {
+                if (L.safe) {
+                    fill_half_fast<K, OV>(img, L, cs, Te, Ne, half, cbase, sstride, s_lo, s_hi, max_val);
+                    continue;
+                }
+            }
+            auto put = [&](int s, T Te_, T nsq) {
+                T v;
+                if (!pixel_eval<FAST, K>(L, Te_, nsq, max_val, v)) return;
+                Acc *px = &img[cbase + (s - s_lo) * sstride];
                 if constexpr (OV == 0) {
                     atomicAdd(px, v);
                 } else {
                     if (v == v) atomicMax(px, ord_enc(v));
                 }
-            });
+            };
+            if (half == 0 && cs >= s_lo && cs < s_hi) put(cs, Te, nsq_of(L, Ne));
+            const int sb = pick(L.st, b);
+            const int dir = half ? -sb : sb;
+            const T tb = pick(L.t, b), nb = pick(L.nv, b);
+            const T dT = (dir > 0) ? tb : -tb, dN = (dir > 0) ? nb : -nb;
+            int s = cs;
+            for (;;) {
+                s += dir;
+                Te += dT;
+                Ne += dN;
+                const T nsq = nsq_of(L, Ne);
+                if (!(nsq < L.w2)) break;
+                if (s >= s_lo && s < s_hi) put(s, Te, nsq);
+                else if ((dir > 0) ? (s >= s_hi) : (s < s_lo)) break;
+            }
         }
-        wave_sync();
     }
     __syncthreads();
 
-    // flush: one coalesced read-modify-write of the tile
+    // flush: one coalesced read-modify-write of the tile, 4 pixels per access
+    // where the tile row lies inside the frame
     T *fo = out + (f * g.H) * (int64_t)g.W;
-    for (int i = threadIdx.x; i < TILE * TILE; i += NT) {
+    constexpr int V = 4;
+    const bool full = (X1 - X0 == TILE) && ((g.W & (V - 1)) == 0) && (((uintptr_t)out & 15) == 0);
+    for (int i = tid * V; i < TILE * TILE; i += NT * V) {
         const int y = Y0 + i / TILE, x = X0 + (i % TILE);
-        if (y >= Y1 || x >= X1) continue;
+        if (y >= Y1) continue;
         T *o = fo + (int64_t)y * g.W + x;
-        if constexpr (OV == 0) {
-            *o = *o + img[i];
+        if (full) {
+            if constexpr (OV == 0) {
+                using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
+                if constexpr (sizeof(T) == 4) {
+                    float4 ov = *reinterpret_cast<float4 *>(o);
+                    ov.x = ov.x + img[i];
+                    ov.y = ov.y + img[i + 1];
+                    ov.z = ov.z + img[i + 2];
+                    ov.w = ov.w + img[i + 3];
+                    *reinterpret_cast<float4 *>(o) = ov;
+                } else {
+                    (void)sizeof(Vec);
+                    for (int j = 0; j < V; j++) o[j] = o[j] + img[i + j];
+                }
+            } else {
+                for (int j = 0; j < V; j++) {
+                    const Acc e = img[i + j];
+                    if (e != Acc(0)) {
+                        const T m = ord_dec(e);
+                        const T ov = o[j];
+                        o[j] = (ov < m) ? m : ov; // std::max(out, value)
+                    }
+                }
+            }
         } else {
-            const Acc e = img[i];
-            if (e != Acc(0)) {
-                const T m = ord_dec(e);
-                const T ov = *o;
-                *o = (ov < m) ? m : ov; // std::max(out, value)
+            for (int j = 0; j < V; j++) {
+                if (x + j >= X1) break;
+                if constexpr (OV == 0) {
+                    o[j] = o[j] + img[i + j];
+                } else {
+                    const Acc e = img[i + j];
+                    if (e != Acc(0)) {
+                        const T m = ord_dec(e);
+                        const T ov = o[j];
+                        o[j] = (ov < m) ? m : ov;
+                    }
+                }
             }
         }
     }
@@ -372,7 +509,7 @@ k_emit(const Segment<T> *__restrict__ segs, int64_t n_segs, const LineSetup<T> *
     if (i >= n_segs) return;
     const Segment<T> sg = segs[i];
     const LineSetup<T> L = setup[sg.line];
-    const int a = L.axis, b = 1 - a, sa = L.st[a];
+    const int a = L.axis, b = 1 - a, sa = pick(L.st, a);
     const int db = (b == 0) ? W : H;
     const int64_t base = WRITE ? off[i] : 0;
     int n = 0;
@@ -382,20 +519,20 @@ k_emit(const Segment<T> *__restrict__ segs, int64_t n_segs, const LineSetup<T> *
     s.c[1] = sg.c[1];
     s.Te = sg.Te;
     s.Ne = sg.Ne;
-    const int c0 = s.c[a];
+    const int c0 = pick(s.c, a);
     int it = 0;
     for (; it < 4 * 64 + 16; it++) {
         if (walk_done(L, s)) break;
-        const int k = (s.c[a] - c0) * sa;
+        const int k = (pick(s.c, a) - c0) * sa;
         if (k >= sg.ncols) break;
         bool nx, ny;
         walk_next(L, s.Ne, nx, ny);
         if ((a == 0) ? nx : ny) {
             ColRec<T> cr;
-            cr.s = s.c[b];
+            cr.s = pick(s.c, b);
             cr.Te = s.Te;
             cr.Ne = s.Ne;
-            const int c = s.c[a];
+            const int c = pick(s.c, a);
             fill_column(L, cr, 0, db, [&](int sp, T Te, T Ne) {
                 const T err = err_of(L, Te, Ne);
                 if (!(err <= T(1))) return;
@@ -539,7 +676,7 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
                     const LineSetup<T> &L = setup[hdr[j].line];
                     const int line = hdr[j].line;
                     const int a = L.axis, b = 1 - a;
-                    const int c = sg.c[a] + k * L.st[a];
+                    const int c = pick(sg.c, a) + k * pick(L.st, a);
                     const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
                     auto put = [&](int s, T Te, T Ne) {
                         const T err = err_of(L, Te, Ne);
@@ -558,8 +695,8 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
                     };
                     // lane pair (k, 0): centre and the +step half; (k, 1): the -step half
                     if (half == 0 && cr.s >= s_lo && cr.s < s_hi) put(cr.s, cr.Te, cr.Ne);
-                    const int dir = half ? -L.st[b] : L.st[b];
-                    const T dT = (dir > 0) ? L.t[b] : -L.t[b], dN = (dir > 0) ? L.nv[b] : -L.nv[b];
+                    const int dir = half ? -pick(L.st, b) : pick(L.st, b);
+                    const T dT = (dir > 0) ? pick(L.t, b) : -pick(L.t, b), dN = (dir > 0) ? pick(L.nv, b) : -pick(L.nv, b);
                     int s = cr.s;
                     T Te = cr.Te, Ne = cr.Ne;
                     for (;;) {
@@ -727,7 +864,7 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
                     const Segment<T> &sg = segs[hdr[wave][i].seg];
                     const LineSetup<T> &L = setup[hdr[wave][i].line];
                     const int a = L.axis;
-                    const int c = sg.c[a] + k * L.st[a];
+                    const int c = pick(sg.c, a) + k * pick(L.st, a);
                     const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
                     fill_column(L, cr, s_lo, s_hi, [&](int s, T Te, T Ne) {
                         const T err = err_of(L, Te, Ne);
@@ -847,6 +984,8 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t 
         CBX_HIP(hipMemsetAsync(P.tile_cnt.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
     }
     const int blk = 256;
+    {
+        KernelTimer timer1("segments_pass1", st);
     if (bin)
         k_segments<T, I, false, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
             lines, idxs, n_lines, g, P.setup.ptr, P.seg_cnt.ptr, P.tile_cnt.ptr, nullptr, nullptr, nullptr, nullptr,
@@ -855,6 +994,7 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t 
         k_segments<T, I, false, false><<<grid_for(n_lines, blk), blk, 0, st>>>(
             lines, idxs, n_lines, g, P.setup.ptr, P.seg_cnt.ptr, nullptr, nullptr, nullptr, nullptr, nullptr,
             nullptr, P.flags.ptr);
+    }
     CBX_HIP(hipGetLastError());
     CBX_HIP(exclusive_scan_i32(P.seg_cnt.ptr, P.seg_off.ptr, n_lines + 1, st));
     if (bin) CBX_HIP(exclusive_scan_i32(P.tile_cnt.ptr, P.tile_off.ptr, P.n_tiles + 1, st));
@@ -902,31 +1042,46 @@ static int check_opts(int tile, int seg)
 }
 
 template <typename T, int K, int OV, int TILE, int SEG>
-static void launch_tiles(T *out, const SegmentPlan<T> &P, T max_val, hipStream_t st)
+static void launch_tiles(T *out, const SegmentPlan<T> &P, T max_val, int threads, int chunk, bool fast,
+                         hipStream_t st)
 {
-    constexpr int NT = 256, CH = (SEG == 16) ? 16 : 32;
-    k_tiles<T, K, OV, TILE, SEG, NT, CH><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-        out, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr);
+#define CBX_TILES(NT, CH, FST)                                                                                  \
+    k_tiles<T, K, OV, TILE, SEG, NT, CH, FST><<<(unsigned)P.n_tiles, NT, 0, st>>>(                             \
+        out, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr)
+    if (fast) {
+        if (threads == 64) CBX_TILES(64, 64, true);
+        else if (threads == 128) CBX_TILES(128, 64, true);
+        else if (chunk == 64) CBX_TILES(256, 64, true);
+        else CBX_TILES(256, 256, true);
+    } else {
+        if (threads == 64) CBX_TILES(64, 64, false);
+        else if (threads == 128) CBX_TILES(128, 64, false);
+        else if (chunk == 64) CBX_TILES(256, 64, false);
+        else CBX_TILES(256, 256, false);
+    }
+#undef CBX_TILES
 }
 
 template <typename T, int K, int OV>
-static void launch_tiles_k(T *out, const SegmentPlan<T> &P, T max_val, hipStream_t st)
+static void launch_tiles_k(T *out, const SegmentPlan<T> &P, T max_val, int threads, int chunk, bool fast,
+                           hipStream_t st)
 {
-    if (P.g.tile == 64 && P.g.seg == 16) launch_tiles<T, K, OV, 64, 16>(out, P, max_val, st);
-    else if (P.g.tile == 64) launch_tiles<T, K, OV, 64, 8>(out, P, max_val, st);
-    else if (P.g.seg == 16) launch_tiles<T, K, OV, 32, 16>(out, P, max_val, st);
-    else launch_tiles<T, K, OV, 32, 8>(out, P, max_val, st);
+    if (P.g.tile == 64 && P.g.seg == 16) launch_tiles<T, K, OV, 64, 16>(out, P, max_val, threads, chunk, fast, st);
+    else if (P.g.tile == 64) launch_tiles<T, K, OV, 64, 8>(out, P, max_val, threads, chunk, fast, st);
+    else if (P.g.seg == 16) launch_tiles<T, K, OV, 32, 16>(out, P, max_val, threads, chunk, fast, st);
+    else launch_tiles<T, K, OV, 32, 8>(out, P, max_val, threads, chunk, fast, st);
 }
 
 template <typename T, int OV>
-static void launch_tiles_ov(int kernel, T *out, const SegmentPlan<T> &P, T max_val, hipStream_t st)
+static void launch_tiles_ov(int kernel, T *out, const SegmentPlan<T> &P, T max_val, int threads, int chunk,
+                            bool fast, hipStream_t st)
 {
     switch (kernel) {
-    case K_BIWEIGHT: launch_tiles_k<T, K_BIWEIGHT, OV>(out, P, max_val, st); break;
-    case K_GAUSSIAN: launch_tiles_k<T, K_GAUSSIAN, OV>(out, P, max_val, st); break;
-    case K_PARABOLIC: launch_tiles_k<T, K_PARABOLIC, OV>(out, P, max_val, st); break;
-    case K_RECTANGULAR: launch_tiles_k<T, K_RECTANGULAR, OV>(out, P, max_val, st); break;
-    default: launch_tiles_k<T, K_TRIANGULAR, OV>(out, P, max_val, st); break;
+    case K_BIWEIGHT: launch_tiles_k<T, K_BIWEIGHT, OV>(out, P, max_val, threads, chunk, fast, st); break;
+    case K_GAUSSIAN: launch_tiles_k<T, K_GAUSSIAN, OV>(out, P, max_val, threads, chunk, fast, st); break;
+    case K_PARABOLIC: launch_tiles_k<T, K_PARABOLIC, OV>(out, P, max_val, threads, chunk, fast, st); break;
+    case K_RECTANGULAR: launch_tiles_k<T, K_RECTANGULAR, OV>(out, P, max_val, threads, chunk, fast, st); break;
+    default: launch_tiles_k<T, K_TRIANGULAR, OV>(out, P, max_val, threads, chunk, fast, st); break;
     }
 }
 
@@ -1080,6 +1235,8 @@ static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const
     int tile, seg;
     resolve_opts(opts, tile, seg);
     if ((rc = check_opts(tile, seg))) return rc;
+    if (opts && !(opts->threads == 0 || opts->threads == 64 || opts->threads == 128 || opts->threads == 256))
+        return fail(CBX_EVALUE, "threads must be 64, 128 or 256");
     if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
     hipStream_t st = (hipStream_t)stream;
     const bool det = opts && opts->deterministic;
@@ -1092,8 +1249,14 @@ static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const
     if (P.n_segs == 0) return CBX_OK;
     if (det) return ordered_pipeline<T>(P, kernel, overlap, max_val, out, nullptr, nullptr, nullptr, nullptr, nullptr,
                                         0, nullptr, st);
-    if (overlap == 0) launch_tiles_ov<T, 0>(kernel, out, P, max_val, st);
-    else launch_tiles_ov<T, 1>(kernel, out, P, max_val, st);
+    {
+        KernelTimer timer("draw_tiles", st);
+        const int threads = (opts && opts->threads) ? opts->threads : 256;
+        const bool fast = !(opts && opts->exact_values);
+        const int chunk = (opts && opts->chunk) ? opts->chunk : 64;
+        if (overlap == 0) launch_tiles_ov<T, 0>(kernel, out, P, max_val, threads, chunk, fast, st);
+        else launch_tiles_ov<T, 1>(kernel, out, P, max_val, threads, chunk, fast, st);
+    }
     CBX_HIP(hipGetLastError());
     return CBX_OK;
 }
@@ -1160,6 +1323,7 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     // every slot is written by exactly one tile (P.n_items of them)
     DevBuf<double> partial;
     CBX_HIP(partial.alloc(5 * P.n_items, st));
+    KernelTimer timer("adjoint_tiles", st);
     switch (kernel) {
     case K_BIWEIGHT: launch_adjoint<T, K_BIWEIGHT>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
     case K_GAUSSIAN: launch_adjoint<T, K_GAUSSIAN>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;

@@ -48,6 +48,34 @@ struct DevBuf {
     ~DevBuf() { release(); }
 };
 
+// Optional per-kernel timing with HIP events recorded on the launch stream
+// (enabled by cbx_timing_enable); read back by cbx_timing_read.
+bool timing_on();
+void timing_record(const char *name, hipEvent_t begin, hipEvent_t end);
+
+struct KernelTimer {
+    const char *name;
+    hipStream_t stream;
+    hipEvent_t a = nullptr, b = nullptr;
+    KernelTimer(const char *n, hipStream_t s) : name(n), stream(s)
+    {
+        if (!timing_on()) return;
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+            a = b = nullptr;
+            return;
+        }
+        (void)hipEventRecord(a, stream);
+    }
+    void stop()
+    {
+        if (!a) return;
+        (void)hipEventRecord(b, stream);
+        timing_record(name, a, b);
+        a = b = nullptr;
+    }
+    ~KernelTimer() { stop(); }
+};
+
 hipError_t exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s);
 hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
 

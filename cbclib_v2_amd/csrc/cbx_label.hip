@@ -54,6 +54,7 @@ struct Elem<CBX_INT64> {
 };
 
 struct ThresholdF32 {
+    static constexpr bool binary = true;
     const float *img;
     float vmin;
     __device__ __forceinline__ bool on(int64_t i) const { return img[i] > vmin; }
@@ -62,6 +63,7 @@ struct ThresholdF32 {
 
 template <typename V>
 struct ValueIn {
+    static constexpr bool binary = sizeof(V) == 1;
     const V *v;
     __device__ __forceinline__ bool on(int64_t i) const { return v[i] != V(0); }
     __device__ __forceinline__ bool same(int64_t i, int64_t k) const { return v[i] == v[k]; }
@@ -168,6 +170,395 @@ __global__ void k_final(int64_t P, int32_t *__restrict__ par, const int32_t *__r
     par[i] = (r >= 0 && keep[r]) ? rank[r] + 1 : 0;
 }
 
+// ---------------------------------------------------------------------------
+// Band path: 2D structures (or per-frame stacks, leading radii 0) whose
+// shifts lie in the 3x3 neighbourhood.  A band is up to BAND_PX pixels of
+// whole rows of one frame, so bands in order are pixels in raster order.
+//   L1 k_band_local   union-find of the band in LDS; a band-local root is its
+//                     component's smallest index; local ids k = rank of the
+//                     local roots in the band.  Writes the local ids of the
+//                     band's first and last rows, and one node per local
+//                     root (node = band * cap + k) of a global union-find.
+//   L2 k_band_merge   unions the nodes across each band boundary (the
+//                     smaller node, i.e. the smaller raster index, is root).
+//   L3 k_band_resolve / k_band_count / k_band_rank / k_node_label: global
+//                     roots per band, their ranks (scan over bands), labels
+//                     of every node.
+//   L4 k_band_final   recomputes the band's local union-find (same result)
+//                     and writes each pixel's label.
+// HBM traffic: the input twice, the labels once, the band borders (int16).
+// ---------------------------------------------------------------------------
+constexpr int BAND_PX = 8192;
+constexpr int BAND_WORDS = BAND_PX / 32;
+constexpr int BAND_NT = 256;
+static_assert(BAND_WORDS <= BAND_NT, "one root-mask word per thread");
+
+struct BandGeom {
+    int64_t F;
+    int32_t H, W;
+    int32_t B;               // rows per band
+    int32_t bands_per_frame; // ceil(H / B)
+    int64_t n_bands;
+    int32_t cap;             // node slots per band
+    int32_t prev;            // bit 0: (0,-1), 1: (-1,-1), 2: (-1,0), 3: (-1,+1)
+};
+
+__device__ __forceinline__ int lds_find(volatile int32_t *par, int i)
+{
+    int p = par[i];
+    while (p != i) {
+        i = p;
+        p = par[i];
+    }
+    return i;
+}
+
+__device__ __forceinline__ void lds_union(int32_t *par, int a, int b)
+{
+    volatile int32_t *vp = par;
+    while (true) {
+        a = lds_find(vp, a);
+        b = lds_find(vp, b);
+        if (a == b) return;
+        if (a < b) {
+            const int t = a;
+            a = b;
+            b = t;
+        }
+        const int old = atomicCAS(&par[a], a, b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+__device__ __forceinline__ void global_union(int32_t *par, int32_t a, int32_t b)
+{
+    while (true) {
+        int32_t pa = __hip_atomic_load(&par[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (pa != a) {
+            a = pa;
+            pa = __hip_atomic_load(&par[a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        int32_t pb = __hip_atomic_load(&par[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (pb != b) {
+            b = pb;
+            pb = __hip_atomic_load(&par[b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (a == b) return;
+        if (a < b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        const int32_t old = atomicCAS(&par[a], a, b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+// exclusive scan of one int per thread over a 256-thread block
+__device__ __forceinline__ int block_exclusive_scan(int v, int *tmp, int &total)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63) tmp[wave] = x;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < wave; w++) off += tmp[w];
+    total = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+    __syncthreads();
+    return off + x - v;
+}
+
+struct BandShared {
+    int32_t par[BAND_PX];
+    uint32_t words[BAND_WORDS];
+    int32_t wpref[BAND_WORDS];
+    int32_t tmp[4];
+    int32_t nroots;
+};
+
+__device__ __forceinline__ int band_kid(const BandShared &S, int r)
+{
+    return S.wpref[r >> 5] + __popc(S.words[r >> 5] & ((1u << (r & 31)) - 1u));
+}
+
+// Local union-find of one band in LDS; leaves par[i] = local root (or -1),
+// the root bitmask, its word prefix and the root count.
+template <typename In>
+__device__ void band_ccl(In in, const BandGeom &g, int64_t band, BandShared &S, int &npx, int &rows, int64_t &gbase)
+{
+    const int64_t f = band / g.bands_per_frame;
+    const int bb = (int)(band - f * g.bands_per_frame);
+    const int r0 = bb * g.B;
+    rows = min(g.B, g.H - r0);
+    npx = rows * g.W;
+    gbase = (f * g.H + r0) * (int64_t)g.W;
+    const int W = g.W, tid = threadIdx.x;
+    for (int i = tid; i < npx; i += BAND_NT) S.par[i] = in.on(gbase + i) ? i : -1;
+    __syncthreads();
+    for (int i = tid; i < npx; i += BAND_NT) {
+        if (S.par[i] < 0) continue;
+        const int x = i % W, r = i / W;
+        if ((g.prev & 1) && x > 0 && S.par[i - 1] >= 0 && in.same(gbase + i, gbase + i - 1)) lds_union(S.par, i, i - 1);
+        if (r > 0) {
+            if ((g.prev & 2) && x > 0 && S.par[i - W - 1] >= 0 && in.same(gbase + i, gbase + i - W - 1))
+                lds_union(S.par, i, i - W - 1);
+            if ((g.prev & 4) && S.par[i - W] >= 0 && in.same(gbase + i, gbase + i - W)) lds_union(S.par, i, i - W);
+            if ((g.prev & 8) && x + 1 < W && S.par[i - W + 1] >= 0 && in.same(gbase + i, gbase + i - W + 1))
+                lds_union(S.par, i, i - W + 1);
+        }
+    }
+    __syncthreads();
+    for (int i = tid; i < npx; i += BAND_NT)
+        if (S.par[i] >= 0) S.par[i] = lds_find(S.par, i);
+    __syncthreads();
+    const int nwords = (npx + 31) >> 5;
+    uint32_t bits = 0;
+    if (tid < nwords) {
+        for (int j = 0; j < 32; j++) {
+            const int i = (tid << 5) + j;
+            if (i < npx && S.par[i] == i) bits |= 1u << j;
+        }
+        S.words[tid] = bits;
+    }
+    int total = 0;
+    const int pre = block_exclusive_scan(tid < nwords ? __popc(bits) : 0, S.tmp, total);
+    if (tid < nwords) S.wpref[tid] = pre;
+    if (tid == 0) S.nroots = total;
+    __syncthreads();
+}
+
+template <typename In, bool SIZES>
+__global__ void __launch_bounds__(BAND_NT)
+k_band_local(In in, BandGeom g, int32_t *__restrict__ nroots, int16_t *__restrict__ top, int16_t *__restrict__ bot,
+             int32_t *__restrict__ gpar, int32_t *__restrict__ lsize, int32_t *__restrict__ gsize)
+{
+    __shared__ BandShared S;
+    __shared__ int32_t cnt[SIZES ? BAND_PX : 1];
+    const int64_t band = blockIdx.x;
+    int npx, rows;
+    int64_t gbase;
+    band_ccl(in, g, band, S, npx, rows, gbase);
+    const int tid = threadIdx.x, W = g.W;
+    const int64_t node0 = band * g.cap;
+    if (tid == 0) nroots[band] = S.nroots;
+    for (int i = tid; i < npx; i += BAND_NT) {
+        if (S.par[i] != i) continue;
+        const int64_t node = node0 + band_kid(S, i);
+        gpar[node] = (int32_t)node;
+        if (SIZES) gsize[node] = 0;
+    }
+    if (SIZES) {
+        for (int k = tid; k < S.nroots; k += BAND_NT) cnt[k] = 0;
+        __syncthreads();
+        for (int i = tid; i < npx; i += BAND_NT)
+            if (S.par[i] >= 0) atomicAdd(&cnt[band_kid(S, S.par[i])], 1);
+        __syncthreads();
+        for (int k = tid; k < S.nroots; k += BAND_NT) lsize[node0 + k] = cnt[k];
+    }
+    const int last = (rows - 1) * W;
+    for (int x = tid; x < W; x += BAND_NT) {
+        const int pt = S.par[x], pb = S.par[last + x];
+        top[band * W + x] = (int16_t)(pt >= 0 ? band_kid(S, pt) : -1);
+        bot[band * W + x] = (int16_t)(pb >= 0 ? band_kid(S, pb) : -1);
+    }
+}
+
+template <typename In>
+__global__ void __launch_bounds__(BAND_NT)
+k_band_merge(In in, BandGeom g, const int16_t *__restrict__ top, const int16_t *__restrict__ bot, int32_t *gpar)
+{
+    const int64_t band = blockIdx.x;
+    const int64_t f = band / g.bands_per_frame;
+    const int bb = (int)(band - f * g.bands_per_frame);
+    if (bb == 0) return;
+    const int x = blockIdx.y * BAND_NT + threadIdx.x;
+    const int W = g.W;
+    if (x >= W) return;
+    const int t = top[band * W + x];
+    if (t < 0) return;
+    const int32_t nt = (int32_t)(band * g.cap + t);
+    const int64_t gt = (f * g.H + (int64_t)bb * g.B) * W + x;
+#pragma unroll
+    for (int dx = -1; dx <= 1; dx++) {
+        if (!(g.prev & (dx < 0 ? 2 : (dx == 0 ? 4 : 8)))) continue;
+        const int xb = x + dx;
+        if (xb < 0 || xb >= W) continue;
+        const int v = bot[(band - 1) * W + xb];
+        if (v < 0) continue;
+        if (!in.same(gt, gt - W + dx)) continue;
+        global_union(gpar, nt, (int32_t)((band - 1) * g.cap + v));
+    }
+}
+
+template <bool SIZES>
+__global__ void __launch_bounds__(BAND_NT)
+k_band_resolve(BandGeom g, const int32_t *__restrict__ nroots, int32_t *gpar, const int32_t *__restrict__ lsize,
+               int32_t *gsize)
+{
+    const int64_t band = blockIdx.x;
+    const int n = nroots[band];
+    for (int k = threadIdx.x; k < n; k += BAND_NT) {
+        const int32_t node = (int32_t)(band * g.cap + k);
+        int32_t r = node, p = gpar[r];
+        while (p != r) {
+            r = p;
+            p = gpar[r];
+        }
+        gpar[node] = r;
+        if (SIZES) atomicAdd(&gsize[r], lsize[node]);
+    }
+}
+
+__global__ void __launch_bounds__(BAND_NT)
+k_band_rank(BandGeom g, const int32_t *__restrict__ nroots, const int32_t *__restrict__ gpar,
+            const int32_t *__restrict__ gsize, int64_t npts, const int32_t *__restrict__ band_off,
+            int32_t *__restrict__ band_cnt, int32_t *__restrict__ lab)
+{
+    __shared__ int32_t tmp[4];
+    const int64_t band = blockIdx.x;
+    const int n = nroots[band];
+    int base = band_off ? band_off[band] : 0;
+    for (int k0 = 0; k0 < n; k0 += BAND_NT) {
+        const int k = k0 + threadIdx.x;
+        const int32_t node = (int32_t)(band * g.cap + k);
+        const bool keep = k < n && gpar[node] == node && (!gsize || gsize[node] >= npts);
+        int total = 0;
+        const int pre = block_exclusive_scan(keep ? 1 : 0, tmp, total);
+        if (band_off && k < n && gpar[node] == node) lab[node] = keep ? base + pre + 1 : 0;
+        base += total;
+    }
+    if (!band_off && threadIdx.x == 0) band_cnt[band] = base;
+}
+
+__global__ void __launch_bounds__(BAND_NT)
+k_node_label(BandGeom g, const int32_t *__restrict__ nroots, const int32_t *__restrict__ gpar, int32_t *lab)
+{
+    const int64_t band = blockIdx.x;
+    const int n = nroots[band];
+    for (int k = threadIdx.x; k < n; k += BAND_NT) {
+        const int32_t node = (int32_t)(band * g.cap + k);
+        const int32_t r = gpar[node];
+        if (r != node) lab[node] = lab[r];
+    }
+}
+
+template <typename In>
+__global__ void __launch_bounds__(BAND_NT)
+k_band_final(In in, BandGeom g, const int32_t *__restrict__ lab, int32_t *__restrict__ out)
+{
+    __shared__ BandShared S;
+    const int64_t band = blockIdx.x;
+    int npx, rows;
+    int64_t gbase;
+    band_ccl(in, g, band, S, npx, rows, gbase);
+    const int64_t node0 = band * g.cap;
+    for (int i = threadIdx.x; i < npx; i += BAND_NT) {
+        const int p = S.par[i];
+        out[gbase + i] = (p >= 0) ? lab[node0 + band_kid(S, p)] : 0;
+    }
+}
+
+// Returns -1 if the structure is not handled by the band path.
+static int band_prev_mask(int ndim, const int64_t *shape, const int64_t *shifts, int64_t n_shifts)
+{
+    if (ndim < 2) return -1;
+    int mask = 0;
+    for (int64_t j = 0; j < n_shifts; j++) {
+        const int64_t *s = shifts + j * ndim;
+        for (int d = 0; d < ndim - 2; d++)
+            if (s[d] != 0) return -1;
+        const int64_t dy = s[ndim - 2], dx = s[ndim - 1];
+        if (dy < -1 || dy > 1 || dx < -1 || dx > 1) return -1;
+        if (dy == 0 && dx == 0) continue;
+        // keep the raster-earlier half; its mirror must be present too
+        bool mirrored = false;
+        for (int64_t m = 0; m < n_shifts; m++) {
+            const int64_t *t = shifts + m * ndim;
+            bool z = true;
+            for (int d = 0; d < ndim - 2; d++) z &= t[d] == 0;
+            if (z && t[ndim - 2] == -dy && t[ndim - 1] == -dx) mirrored = true;
+        }
+        if (!mirrored) return -1;
+        if (dy == 0 && dx == -1) mask |= 1;
+        if (dy == -1 && dx == -1) mask |= 2;
+        if (dy == -1 && dx == 0) mask |= 4;
+        if (dy == -1 && dx == 1) mask |= 8;
+    }
+    (void)shape;
+    return mask;
+}
+
+template <typename In>
+static int label_band(In in, int ndim, const int64_t *shape, int prev, int64_t npts, int32_t *out, int64_t *n_labels,
+                      hipStream_t st)
+{
+    BandGeom g;
+    g.H = (int32_t)shape[ndim - 2];
+    g.W = (int32_t)shape[ndim - 1];
+    g.F = 1;
+    for (int d = 0; d < ndim - 2; d++) g.F *= shape[d];
+    g.B = max(1, min(g.H, BAND_PX / g.W));
+    g.bands_per_frame = (g.H + g.B - 1) / g.B;
+    g.n_bands = g.F * g.bands_per_frame;
+    // node slots per band = the most components a band can hold: for binary
+    // inputs a left (up) neighbour merges horizontally (vertically) adjacent
+    // pixels, so a row (column) holds at most ceil(W/2) (ceil(B/2)) of them;
+    // equal-value inputs can have a component per pixel
+    int cap = g.B * g.W;
+    if (In::binary && (prev & 1)) cap = min(cap, g.B * ((g.W + 1) / 2));
+    if (In::binary && (prev & 4)) cap = min(cap, g.W * ((g.B + 1) / 2));
+    g.cap = cap;
+    g.prev = prev;
+    const int64_t n_nodes = g.n_bands * g.cap;
+    if (n_nodes >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "label: too many bands");
+    const bool sizes = npts > 1;
+
+    DevBuf<int32_t> nroots, gpar, lab, lsize, gsize, band_cnt, band_off;
+    DevBuf<int16_t> top, bot;
+    CBX_HIP(nroots.alloc(g.n_bands, st));
+    CBX_HIP(gpar.alloc(n_nodes, st));
+    CBX_HIP(lab.alloc(n_nodes, st));
+    CBX_HIP(top.alloc(g.n_bands * g.W, st));
+    CBX_HIP(bot.alloc(g.n_bands * g.W, st));
+    CBX_HIP(band_cnt.alloc(g.n_bands + 1, st));
+    CBX_HIP(band_off.alloc(g.n_bands + 1, st));
+    if (sizes) {
+        CBX_HIP(lsize.alloc(n_nodes, st));
+        CBX_HIP(gsize.alloc(n_nodes, st));
+    }
+    KernelTimer timer("label_total", st);
+    const unsigned nb = (unsigned)g.n_bands;
+    if (sizes)
+        k_band_local<In, true><<<nb, BAND_NT, 0, st>>>(in, g, nroots.ptr, top.ptr, bot.ptr, gpar.ptr, lsize.ptr, gsize.ptr);
+    else
+        k_band_local<In, false><<<nb, BAND_NT, 0, st>>>(in, g, nroots.ptr, top.ptr, bot.ptr, gpar.ptr, nullptr, nullptr);
+    k_band_merge<In><<<dim3(nb, (g.W + BAND_NT - 1) / BAND_NT), BAND_NT, 0, st>>>(in, g, top.ptr, bot.ptr, gpar.ptr);
+    if (sizes) k_band_resolve<true><<<nb, BAND_NT, 0, st>>>(g, nroots.ptr, gpar.ptr, lsize.ptr, gsize.ptr);
+    else k_band_resolve<false><<<nb, BAND_NT, 0, st>>>(g, nroots.ptr, gpar.ptr, nullptr, nullptr);
+    CBX_HIP(hipMemsetAsync(band_cnt.ptr + g.n_bands, 0, sizeof(int32_t), st));
+    k_band_rank<<<nb, BAND_NT, 0, st>>>(g, nroots.ptr, gpar.ptr, sizes ? gsize.ptr : nullptr, npts, nullptr,
+                                        band_cnt.ptr, nullptr);
+    CBX_HIP(exclusive_scan_i32(band_cnt.ptr, band_off.ptr, g.n_bands + 1, st));
+    k_band_rank<<<nb, BAND_NT, 0, st>>>(g, nroots.ptr, gpar.ptr, sizes ? gsize.ptr : nullptr, npts, band_off.ptr,
+                                        nullptr, lab.ptr);
+    k_node_label<<<nb, BAND_NT, 0, st>>>(g, nroots.ptr, gpar.ptr, lab.ptr);
+    k_band_final<In><<<nb, BAND_NT, 0, st>>>(in, g, lab.ptr, out);
+    CBX_HIP(hipGetLastError());
+    timer.stop();
+    int32_t n = 0;
+    CBX_HIP(hipMemcpyAsync(&n, band_off.ptr + g.n_bands, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    *n_labels = n;
+    return CBX_OK;
+}
+
 template <typename In>
 static int label_run(In in, int ndim, const int64_t *shape, const int64_t *shifts, int64_t n_shifts, int64_t npts,
                      int32_t *out, int64_t *n_labels, hipStream_t st)
@@ -186,6 +577,11 @@ static int label_run(In in, int ndim, const int64_t *shape, const int64_t *shift
     }
     if (P >= (int64_t(1) << 31) - 1) return fail(CBX_EVALUE, "label supports fewer than 2^31 - 1 elements");
     if (P == 0) return CBX_OK;
+    {
+        const int prev = band_prev_mask(ndim, shape, shifts, n_shifts);
+        if (prev >= 0 && shape[ndim - 1] <= BAND_PX && shape[ndim - 1] >= 1)
+            return label_band(in, ndim, shape, prev, npts, out, n_labels, st);
+    }
 
     // keep the shifts that point to a raster-earlier neighbour when the set is
     // symmetric (every union is then made once); otherwise keep them all
@@ -239,6 +635,7 @@ static int label_run(In in, int ndim, const int64_t *shape, const int64_t *shift
         CBX_HIP(hipMemcpyAsync(dsh.ptr, sh.data(), sizeof(int32_t) * sh.size(), hipMemcpyHostToDevice, st));
         CBX_HIP(hipMemcpyAsync(doff.ptr, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice, st));
     }
+    KernelTimer timer("label_total", st);
     k_init<<<grid, blk, 0, st>>>(in, P, out);
     if (g.n_shifts) k_union<<<grid, blk, 0, st>>>(in, P, g, dsh.ptr, doff.ptr, out);
     k_flatten<<<grid, blk, 0, st>>>(P, out);
@@ -257,6 +654,7 @@ static int label_run(In in, int ndim, const int64_t *shape, const int64_t *shift
     CBX_HIP(exclusive_scan_i32(keep.ptr, rank.ptr, P + 1, st));
     k_final<<<grid, blk, 0, st>>>(P, out, keep.ptr, rank.ptr);
     CBX_HIP(hipGetLastError());
+    timer.stop();
     int32_t n = 0;
     CBX_HIP(hipMemcpyAsync(&n, rank.ptr + P, sizeof(int32_t), hipMemcpyDeviceToHost, st));
     CBX_HIP(hipStreamSynchronize(st));

@@ -4,7 +4,10 @@
 #include <hipcub/hipcub.hpp>
 
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <vector>
 
 #include "cbx_host.hpp"
 
@@ -18,6 +21,47 @@ int fail(int code, const std::string &msg)
 {
     set_error(msg);
     return code;
+}
+
+struct TimingState {
+    std::mutex mu;
+    bool on = false;
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::map<std::string, std::pair<double, int64_t>> total;
+};
+
+static TimingState &timing()
+{
+    static TimingState t;
+    return t;
+}
+
+bool timing_on() { return timing().on; }
+
+void timing_record(const char *name, hipEvent_t a, hipEvent_t b)
+{
+    TimingState &t = timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.pending.push_back({name, a, b});
+}
+
+static void timing_drain(TimingState &t)
+{
+    for (auto &p : t.pending) {
+        float ms = 0.0f;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            auto &e = t.total[p.name];
+            e.first += ms;
+            e.second += 1;
+        }
+        (void)hipEventDestroy(p.a);
+        (void)hipEventDestroy(p.b);
+    }
+    t.pending.clear();
 }
 
 template <typename V>
@@ -48,6 +92,32 @@ extern "C" {
 const char *cbx_last_error(void) { return cbx::g_last_error.c_str(); }
 
 const char *cbx_version(void) { return "cbx 0.1.0 (gfx950)"; }
+
+void cbx_timing_enable(int on)
+{
+    cbx::TimingState &t = cbx::timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    t.on = on != 0;
+}
+
+void cbx_timing_reset(void)
+{
+    cbx::TimingState &t = cbx::timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    cbx::timing_drain(t);
+    t.total.clear();
+}
+
+int cbx_timing_read(const char *name, double *total_ms, int64_t *count)
+{
+    cbx::TimingState &t = cbx::timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    cbx::timing_drain(t);
+    auto it = t.total.find(name ? name : "");
+    *total_ms = (it == t.total.end()) ? 0.0 : it->second.first;
+    *count = (it == t.total.end()) ? 0 : it->second.second;
+    return CBX_OK;
+}
 
 int cbx_kernel_id(const char *name)
 {

@@ -56,18 +56,23 @@ def _check_idxs(idxs: Any, n_lines: int, n_frames: int) -> None:
         raise ValueError(f"idxs has an invalid size ({n} != {n_lines})")
 
 
-def _opts(tile: Optional[int], seg: Optional[int], deterministic: bool):
-    return _native.DrawOpts(int(tile or 0), int(seg or 0), 1 if deterministic else 0, 0)
+def _opts(tile: Optional[int], seg: Optional[int], deterministic: bool, threads: Optional[int] = None,
+          exact_values: bool = False, chunk: Optional[int] = None):
+    return _native.DrawOpts(int(tile or 0), int(seg or 0), 1 if deterministic else 0, int(threads or 0),
+                            1 if exact_values else 0, int(chunk or 0))
 
 
 def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, kernel: str = 'rectangular',
                overlap: str = 'sum', *, tile: Optional[int] = None, seg: Optional[int] = None,
-               deterministic: bool = False):
+               deterministic: bool = False, threads: Optional[int] = None, exact_values: bool = False,
+               chunk: Optional[int] = None):
     """Draw thick lines with a profile into ``out`` (in place) and return ``out``.
 
     Extra keyword-only arguments (no reference counterpart): ``tile`` (32 or
-    64, pixel tile per workgroup), ``seg`` (8 or 16, walk checkpoint spacing)
-    and ``deterministic`` (reference summation order, bit-exact).
+    64, pixel tile per workgroup), ``seg`` (8 or 16, walk checkpoint spacing),
+    ``threads`` (64, 128 or 256, tile workgroup size), ``deterministic``
+    (reference summation order: bit-exact images) and ``exact_values``
+    (reference value arithmetic in the tile path).
     """
     device.require_rocm('draw_lines')
     kid = _native.kernel_id(kernel)
@@ -89,7 +94,7 @@ def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, ker
     t_idxs = None if idxs is None else to_device(idxs, torch.int64 if f64 else torch.int32)
     L = _native.lib()
     fn = L.cbx_draw_lines_f64 if f64 else L.cbx_draw_lines_f32
-    opts = _opts(tile, seg, deterministic)
+    opts = _opts(tile, seg, deterministic, threads, exact_values, chunk)
     _native.check(fn(t_out.data_ptr(), n_frames, H, W, t_lines.data_ptr(), n_lines, ptr(t_idxs), float(max_val),
                      kid, oid, ctypes.byref(opts), stream_ptr()))
     return finish()

@@ -65,6 +65,9 @@ CBX_API int cbx_kernel_id(const char *name);
  * = 1 accumulates every pixel in ascending line order starting from the value
  * already in `out`, which is the single-threaded reference's order: the
  * result is then bit-identical to bresenham.draw_lines(..., num_threads=1).
+ * Otherwise (the tile path) the pixel set is still the reference's exactly;
+ * values are computed in float (a few ulp from the reference's) unless
+ * `exact_values` = 1, and summed in LDS in a different order.
  * The replacement of the reference's `grid` argument
  * (cuda_draw_lines.pyi:8-10).
  */
@@ -72,7 +75,9 @@ typedef struct cbx_draw_opts {
     int32_t tile;
     int32_t seg;
     int32_t deterministic;
-    int32_t reserved;
+    int32_t threads;      /* workgroup size of the tile kernel: 64, 128 or 256 (0 = 256) */
+    int32_t exact_values; /* 1: the tile path computes values with the reference's arithmetic */
+    int32_t chunk;        /* items per phase of the tile kernel: 64 or the workgroup size (0 = 64) */
 } cbx_draw_opts;
 
 /*
@@ -146,6 +151,18 @@ CBX_API int cbx_label(const void *inp, int dtype, int ndim, const int64_t *shape
  */
 CBX_API int cbx_label_threshold_f32(const float *image, float vmin, int ndim, const int64_t *shape, const int64_t *shifts,
                             int64_t n_shifts, int64_t npts, int32_t *out, int64_t *n_labels, void *stream);
+
+/*
+ * Instrumentation (no reference counterpart; the reference has no timers):
+ * per-kernel durations measured with HIP events recorded on the launch
+ * stream around the library's main kernels ("draw_tiles",
+ * "adjoint_tiles", "segments_pass1", "label_union", ...).  cbx_timing_read
+ * waits for the recorded events and returns the accumulated milliseconds and
+ * the number of launches since the last reset.
+ */
+CBX_API void cbx_timing_enable(int on);
+CBX_API void cbx_timing_reset(void);
+CBX_API int cbx_timing_read(const char *name, double *total_ms, int64_t *count);
 
 #ifdef __cplusplus
 }

@@ -101,15 +101,19 @@ def test_image_deterministic_bit_exact(cuda, oracle, dtype, overlap, kernel):
 @pytest.mark.parametrize('dtype', [np.float32, np.float64])
 @pytest.mark.parametrize('overlap', ['sum', 'max'])
 @pytest.mark.parametrize('kernel', KERNELS)
-@pytest.mark.parametrize('tile,seg', [(64, 16), (32, 8), (64, 8), (32, 16)])
-def test_image_tiles(cuda, oracle, dtype, overlap, kernel, tile, seg):
+@pytest.mark.parametrize('tile,seg,threads', [(64, 16, 256), (32, 8, 64), (64, 8, 128), (32, 16, 256)])
+@pytest.mark.parametrize('exact', [False, True])
+def test_image_tiles(cuda, oracle, dtype, overlap, kernel, tile, seg, threads, exact):
+    """Tile path: same support as the reference always; values exact per term with
+    exact_values (max overlap then bit-exact), within RTOL/ATOL otherwise."""
     for seed in range(3):
         shape, lines, idxs = case(100 + seed, dtype)
         init = -0.25 if overlap == 'max' else 0.0
         ref = np.full(shape, init, dtype)
         oracle.draw_lines(ref, lines, idxs, 1.7, kernel, overlap)
-        got = gpu_image(shape, lines, idxs, kernel, overlap, dtype, init, tile=tile, seg=seg)
-        if overlap == 'max' and kernel != 'gaussian':
+        got = gpu_image(shape, lines, idxs, kernel, overlap, dtype, init, tile=tile, seg=seg, threads=threads,
+                        exact_values=exact)
+        if overlap == 'max' and kernel != 'gaussian' and (exact or dtype == np.float64):
             assert np.array_equal(got, ref), f"seed {seed}"
         else:
             np.testing.assert_allclose(got, ref, rtol=RTOL, atol=ATOL * 1.7)

@@ -71,3 +71,24 @@ def test_label_numpy_result(cuda, ref_label):
 def test_label_errors(cuda):
     with pytest.raises(ValueError):
         cl.label(torch.zeros(4, 4, dtype=torch.bool, device='cuda'), cl.Structure([1, 1, 1], 1))
+
+
+@pytest.mark.parametrize('shape', [(64, 2048), (3, 40, 1024), (12, 8192), (2, 5, 8193), (700, 13)])
+@pytest.mark.parametrize('conn', [1, 2])
+@pytest.mark.parametrize('kind', ['bool', 'int32', 'int64'])
+def test_label_bands(cuda, oracle, shape, conn, kind):
+    """Many bands per frame (band = whole rows of <= 8192 pixels), the border
+    merge, equal-value inputs, the npts filter, and the general fallback for
+    rows wider than a band."""
+    rng = np.random.default_rng(sum(shape) + conn)
+    if kind == 'bool':
+        inp = rng.random(shape) > 0.55
+    else:
+        inp = rng.integers(0, 3, shape).astype(np.int32 if kind == 'int32' else np.int64)
+    radii = [0] * (len(shape) - 2) + [1, 1]
+    st = cl.Structure(radii, conn)
+    for npts in (1, 4):
+        ref, n = oracle.label(inp, st.shifts(), npts)
+        res = cl.label(torch.from_numpy(inp).cuda(), st, npts)
+        assert int(res.index.numel()) == n, (shape, conn, kind, npts)
+        assert np.array_equal(res.labels.cpu().numpy(), ref), (shape, conn, kind, npts)
