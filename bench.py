@@ -203,7 +203,7 @@ def main() -> None:
         "config": {"workload": f"plain forward + label: {F} frames of {H}x{W} f32, {N} segments ({per}/frame, "
                                "L U[20,400], w U[2,6], random angles, seed 1), gaussian, sum; label(out > 0.5, "
                                "Structure([0,1,1],2)); step = zero + draw_lines + label",
-                   "frames": F, "segments": N, "tile": args.tile or 64, "seg": args.seg or 16},
+                   "frames": F, "segments": N, "tile": args.tile or 64, "seg": args.seg or 8},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_tiles (draw_lines tile accumulation)",

@@ -22,7 +22,8 @@ DTYPES = {'bool': 0, 'int32': 1, 'int64': 2}
 EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32', 'cbx_draw_lines_f64',
            'cbx_draw_pixels_f32', 'cbx_draw_pixels_f64', 'cbx_draw_lines_adjoint_f32',
            'cbx_draw_lines_adjoint_f64', 'cbx_label', 'cbx_label_threshold_f32', 'cbx_timing_enable',
-           'cbx_timing_reset', 'cbx_timing_read')
+           'cbx_timing_reset', 'cbx_timing_read', 'cbx_workspace_release', 'cbx_workspace_bytes',
+           'cbx_label_force_path')
 
 
 class DrawOpts(ctypes.Structure):
@@ -67,6 +68,12 @@ def _declare(L: ctypes.CDLL) -> None:
     L.cbx_timing_reset.argtypes = []
     L.cbx_timing_read.restype = I32
     L.cbx_timing_read.argtypes = [C, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+    L.cbx_label_force_path.restype = None
+    L.cbx_label_force_path.argtypes = [I32]
+    L.cbx_workspace_release.restype = I64
+    L.cbx_workspace_release.argtypes = []
+    L.cbx_workspace_bytes.restype = I64
+    L.cbx_workspace_bytes.argtypes = []
 
 
 def lib() -> ctypes.CDLL:
@@ -126,3 +133,18 @@ def timing_read(name: str):
     n = ctypes.c_int64(0)
     lib().cbx_timing_read(name.encode(), ctypes.byref(ms), ctypes.byref(n))
     return ms.value, n.value
+
+
+def workspace_release() -> int:
+    """Free the library's cached intermediate device memory; returns the bytes freed."""
+    return int(lib().cbx_workspace_release())
+
+
+def workspace_bytes() -> int:
+    """Bytes of intermediate device memory the library keeps cached for reuse."""
+    return int(lib().cbx_workspace_bytes())
+
+
+def label_force_path(path: int) -> None:
+    """Label algorithm: 0 fastest applicable, 1 no run-length path, 2 element union-find only."""
+    lib().cbx_label_force_path(int(path))

@@ -1030,8 +1030,10 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t 
 
 static void resolve_opts(const cbx_draw_opts *o, int &tile, int &seg)
 {
+    // 64 x 64 tiles with a checkpoint every 8 columns: the fastest launch
+    // shape measured on the 64-frame 2048 x 2048 workload
     tile = (o && o->tile) ? o->tile : 64;
-    seg = (o && o->seg) ? o->seg : 16;
+    seg = (o && o->seg) ? o->seg : 8;
 }
 
 static int check_opts(int tile, int seg)
@@ -1351,6 +1353,7 @@ int cbx_draw_lines_f32(float *out, int64_t n_frames, int64_t height, int64_t wid
                        int64_t n_lines, const int32_t *idxs, float max_val, int kernel, int overlap,
                        const cbx_draw_opts *opts, void *stream)
 {
+    cbx::Workspace ws((hipStream_t)stream);
     return cbx::draw_lines_impl<float, int32_t>(out, n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
                                                 overlap, opts, stream);
 }
@@ -1359,6 +1362,7 @@ int cbx_draw_lines_f64(double *out, int64_t n_frames, int64_t height, int64_t wi
                        int64_t n_lines, const int64_t *idxs, double max_val, int kernel, int overlap,
                        const cbx_draw_opts *opts, void *stream)
 {
+    cbx::Workspace ws((hipStream_t)stream);
     return cbx::draw_lines_impl<double, int64_t>(out, n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
                                                  overlap, opts, stream);
 }
@@ -1368,6 +1372,7 @@ int cbx_draw_pixels_f32(int64_t n_frames, int64_t height, int64_t width, const f
                         int32_t *y, int32_t *x, float *value, int32_t *line, int64_t capacity, int64_t *count,
                         void *stream)
 {
+    cbx::Workspace ws((hipStream_t)stream);
     return cbx::draw_pixels_impl<float, int32_t>(n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
                                                  overlap, merge, frame, y, x, value, line, capacity, count, stream);
 }
@@ -1377,6 +1382,7 @@ int cbx_draw_pixels_f64(int64_t n_frames, int64_t height, int64_t width, const d
                         int32_t *y, int32_t *x, double *value, int32_t *line, int64_t capacity, int64_t *count,
                         void *stream)
 {
+    cbx::Workspace ws((hipStream_t)stream);
     return cbx::draw_pixels_impl<double, int64_t>(n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
                                                   overlap, merge, frame, y, x, value, line, capacity, count, stream);
 }
@@ -1385,6 +1391,7 @@ int cbx_draw_lines_adjoint_f32(const float *grad_out, int64_t n_frames, int64_t 
                                const float *lines, int64_t n_lines, const int32_t *idxs, float max_val, int kernel,
                                float *grad_lines, const cbx_draw_opts *opts, void *stream)
 {
+    cbx::Workspace ws((hipStream_t)stream);
     return cbx::adjoint_impl<float, int32_t>(grad_out, n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
                                              grad_lines, opts, stream);
 }
@@ -1393,6 +1400,7 @@ int cbx_draw_lines_adjoint_f64(const double *grad_out, int64_t n_frames, int64_t
                                const double *lines, int64_t n_lines, const int64_t *idxs, double max_val,
                                int kernel, double *grad_lines, const cbx_draw_opts *opts, void *stream)
 {
+    cbx::Workspace ws((hipStream_t)stream);
     return cbx::adjoint_impl<double, int64_t>(grad_out, n_frames, height, width, lines, n_lines, idxs, max_val,
                                               kernel, grad_lines, opts, stream);
 }

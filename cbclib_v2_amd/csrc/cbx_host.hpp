@@ -23,12 +23,39 @@ int fail(int code, const std::string &msg);
             return ::cbx::fail(CBX_ERUNTIME, std::string(#expr) + ": " + hipGetErrorString(e_));       \
     } while (0)
 
-// Device allocation released on scope exit (hipFree waits for the device,
-// so a buffer is never released under a kernel that still uses it).
+// Intermediate device memory of one library call.  An entry point opens a
+// Workspace on its stream; every DevBuf allocated on that host thread while it
+// is open is carved (bump allocation, 256-byte aligned) out of device memory
+// cached per (device, stream) and reused by the next call on that stream.
+// Reuse is safe because work on one stream runs in order: the next call's
+// kernels start after the previous call's have finished.  The cache only
+// grows (a call that needed a second block has the blocks merged into one at
+// the start of the next call) and is released by cbx_workspace_release().
+// A cache already open on another host thread is not shared: that call opens
+// a cache of its own.  Without an open Workspace, DevBuf falls back to
+// hipMalloc / hipFree.
+class Workspace {
+  public:
+    explicit Workspace(hipStream_t s);
+    ~Workspace();
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+    hipError_t take(void **p, size_t bytes);
+    hipError_t status() const { return status_; }
+    static Workspace *current();
+
+  private:
+    struct Cache *cache_ = nullptr;
+    Workspace *outer_ = nullptr;
+    size_t block_ = 0, off_ = 0;
+    hipError_t status_ = hipSuccess;
+};
+
 template <typename V>
 struct DevBuf {
     V *ptr = nullptr;
     size_t n = 0;
+    bool owned = false;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
@@ -37,13 +64,18 @@ struct DevBuf {
         release();
         n = count;
         if (count == 0) return hipSuccess;
+        if (Workspace *ws = Workspace::current()) return ws->take((void **)&ptr, sizeof(V) * count);
+        owned = true;
         return hipMalloc((void **)&ptr, sizeof(V) * count);
     }
     void release()
     {
-        if (ptr) (void)hipFree(ptr);
+        // hipFree waits for the device, so an owned buffer is never released
+        // under a kernel that still uses it; workspace memory is not freed here
+        if (ptr && owned) (void)hipFree(ptr);
         ptr = nullptr;
         n = 0;
+        owned = false;
     }
     ~DevBuf() { release(); }
 };

@@ -29,6 +29,9 @@
 namespace cbx {
 
 constexpr int LBL_MAX_DIM = 7;
+
+// 0: fastest applicable path; 1: skip the run path; 2: general path only
+static int g_label_path = 0;
 constexpr int LBL_MAX_SHIFTS = 256;
 
 struct LabelGeom {
@@ -59,6 +62,9 @@ struct ThresholdF32 {
     float vmin;
     __device__ __forceinline__ bool on(int64_t i) const { return img[i] > vmin; }
     __device__ __forceinline__ bool same(int64_t, int64_t) const { return true; }
+    using Raw = float;
+    __device__ __forceinline__ Raw load(int64_t i) const { return img[i]; }
+    __device__ __forceinline__ bool test(Raw r) const { return r > vmin; }
 };
 
 template <typename V>
@@ -66,7 +72,11 @@ struct ValueIn {
     static constexpr bool binary = sizeof(V) == 1;
     const V *v;
     __device__ __forceinline__ bool on(int64_t i) const { return v[i] != V(0); }
-    __device__ __forceinline__ bool same(int64_t i, int64_t k) const { return v[i] == v[k]; }
+    // bool inputs: every non-zero byte is the same value (true)
+    __device__ __forceinline__ bool same(int64_t i, int64_t k) const { return binary || v[i] == v[k]; }
+    using Raw = V;
+    __device__ __forceinline__ Raw load(int64_t i) const { return v[i]; }
+    __device__ __forceinline__ bool test(Raw r) const { return r != V(0); }
 };
 
 template <typename In>
@@ -465,6 +475,421 @@ k_band_final(In in, BandGeom g, const int32_t *__restrict__ lab, int32_t *__rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Run path: binary inputs (bool arrays, thresholded images) with a 2D
+// structure (or a per-frame stack, leading radii 0) that joins horizontal
+// neighbours, (0,-1) and its mirror, and any of the three upper neighbours.
+// A run is a maximal horizontal stretch of "on" pixels of one row.  With
+// horizontal connectivity a run lies inside one component; runs are numbered
+// in raster order of their first pixels, so the smallest run id of a
+// component is the run holding its smallest flat index, and the union-find
+// (smaller id is root) yields the reference's label order directly.
+//   R1 k_run_pack     one wave per row: the row as 64-bit words (bit x%64 of
+//                     word x/64 = pixel x is on) and its run count.
+//   R2 scan           first run id of every row.
+//   R3 k_run_init     par[r] = r (size[r] = 0).
+//   R4 k_run_union    one wave per row y (y-1 in the same frame): each
+//                     contact (a pixel of row y and an upper neighbour in
+//                     row y-1, both on) at which the run of row y or the run
+//                     of row y-1 starts is unioned.  That covers every pair
+//                     of touching runs: moving a contact one pixel to the left
+//                     keeps it a contact until one of the two runs starts.
+//   R5 k_run_resolve  par[r] = root (read-only root walks, so every entry
+//                     ends as its root); without a size limit also the number of
+//                     roots per row.  (npts > 1: k_run_sizes adds each run's
+//                     length to its root, k_run_count counts kept roots.)
+//   R6 scan           first label of every row.
+//   R7 k_run_rank     each root gets its label, stored as ~label in par.
+//   R8 k_run_label    each run gets its root's ~label.
+//   R9 k_run_final    one wave per row: every pixel's label (0 if off).
+// HBM traffic: the input once, the labels once; the bit rows (W/8 bytes per
+// row) and the run arrays are small enough to stay in the caches.
+// ---------------------------------------------------------------------------
+struct RunGeom {
+    int64_t rows; // frames * H
+    int32_t H, W;
+    int32_t nw;   // 64-bit words per row
+    int32_t prev; // bit 1: (-1,-1), 2: (-1,0), 3: (-1,+1) neighbours
+};
+
+constexpr int RUN_NT = 256;
+constexpr int RUN_ROWS = RUN_NT / 64; // rows per workgroup, one per wave
+
+__device__ __forceinline__ uint64_t le_mask(int b) { return (b >= 63) ? ~0ull : ((2ull << b) - 1ull); }
+
+__device__ __forceinline__ int wave_sum(int v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ int wave_excl_scan(int v, int lane)
+{
+    int x = v;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    return x - v;
+}
+
+// run starts of word w given word w-1 (pixel x starts a run if it is on and
+// x - 1 is off or outside the row)
+__device__ __forceinline__ uint64_t run_starts(uint64_t w, uint64_t left) { return w & ~((w << 1) | (left >> 63)); }
+
+__device__ __forceinline__ int32_t run_find(int32_t *par, int32_t x)
+{
+    // path halving: a non-root's parent is only ever replaced by one of its
+    // ancestors, and roots are only changed by the CAS in run_union
+    while (true) {
+        const int32_t p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (p == x) return x;
+        const int32_t gp = __hip_atomic_load(&par[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (gp == p) return p;
+        __hip_atomic_store(&par[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = gp;
+    }
+}
+
+__device__ __forceinline__ void run_union(int32_t *par, int32_t a, int32_t b)
+{
+    while (true) {
+        a = run_find(par, a);
+        b = run_find(par, b);
+        if (a == b) return;
+        if (a < b) {
+            const int32_t t = a;
+            a = b;
+            b = t;
+        }
+        // link the larger root a under the smaller root b
+        const int32_t old = atomicCAS(&par[a], a, b);
+        if (old == a) return;
+        a = old;
+    }
+}
+
+template <typename In>
+__global__ void __launch_bounds__(RUN_NT)
+k_run_pack(In in, RunGeom g, uint64_t *__restrict__ bits, int32_t *__restrict__ cnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const int64_t base = row * g.W;
+    uint64_t *brow = bits + row * g.nw;
+    int runs = 0;
+    uint64_t carry = 0;
+    for (int c0 = 0; c0 < g.nw; c0 += 64) {
+        const int nwc = min(64, g.nw - c0);
+        uint64_t mine = 0;
+        for (int k0 = 0; k0 < nwc; k0 += 16) {
+            typename In::Raw v[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int64_t x = (int64_t)(c0 + k0 + k) * 64 + lane;
+                v[k] = in.load(base + min(x, (int64_t)g.W - 1));
+            }
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                const int64_t x = (int64_t)(c0 + k0 + k) * 64 + lane;
+                const uint64_t m = __ballot(x < g.W && in.test(v[k]));
+                if (lane == k0 + k) mine = m;
+            }
+        }
+        uint64_t left = __shfl_up(mine, 1, 64);
+        if (lane == 0) left = carry;
+        runs += __popcll(run_starts(mine, left));
+        if (lane < nwc) brow[c0 + lane] = mine;
+        carry = __shfl(mine, nwc - 1, 64);
+    }
+    runs = wave_sum(runs);
+    if (lane == 0) cnt[row] = runs;
+}
+
+template <bool SIZES>
+__global__ void __launch_bounds__(RUN_NT)
+k_run_init(RunGeom g, const int32_t *__restrict__ roff, int32_t *__restrict__ par, int32_t *__restrict__ size)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const int32_t off = roff[row], n = roff[row + 1] - off;
+    for (int k = lane; k < n; k += 64) {
+        par[off + k] = off + k;
+        if (SIZES) size[off + k] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(RUN_NT)
+k_run_union(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff, int32_t *par)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows || row % g.H == 0) return;
+    const int nw = g.nw;
+    const uint64_t *cur = bits + row * nw, *up = cur - nw;
+    const int32_t cbase = roff[row], pbase = roff[row - 1];
+    int ccarry = 0, pcarry = 0;
+    for (int c0 = 0; c0 < nw; c0 += 64) {
+        const int w = c0 + lane;
+        const bool act = w < nw;
+        const uint64_t cw = act ? cur[w] : 0, cwl = (act && w > 0) ? cur[w - 1] : 0;
+        const uint64_t pw = act ? up[w] : 0, pwl = (act && w > 0) ? up[w - 1] : 0;
+        const uint64_t pwr = (act && w + 1 < nw) ? up[w + 1] : 0;
+        const uint64_t cst = run_starts(cw, cwl), pst = run_starts(pw, pwl);
+        const int cn = __popcll(cst), pn = __popcll(pst);
+        const int cpre = ccarry + wave_excl_scan(cn, lane), ppre = pcarry + wave_excl_scan(pn, lane);
+#pragma unroll
+        for (int d = -1; d <= 1; d++) {
+            if (!(g.prev & (d < 0 ? 2 : (d == 0 ? 4 : 8)))) continue;
+            // sp / sps: bit p = pixel / run start of row y-1 at p + d
+            uint64_t sp, sps;
+            if (d == 0) {
+                sp = pw;
+                sps = pst;
+            } else if (d > 0) {
+                sp = (pw >> 1) | (pwr << 63);
+                sps = (pst >> 1) | ((pwr & ~(pw >> 63) & 1ull) << 63);
+            } else {
+                sp = (pw << 1) | (pwl >> 63);
+                sps = (pst << 1) | ((pwl >> 63) & ~(pwl >> 62) & 1ull);
+            }
+            uint64_t ev = cw & sp & (cst | sps);
+            while (ev) {
+                const int p = __builtin_ctzll(ev);
+                ev &= ev - 1;
+                const uint64_t le = le_mask(p);
+                const int32_t a = cbase + cpre + __popcll(cst & le) - 1;
+                // runs of row y-1 starting at or before p + d
+                int nb = ppre + __popcll(pst & le);
+                if (d > 0) nb += (int)((sps >> p) & 1ull);
+                if (d < 0) nb -= (int)((pst >> p) & 1ull);
+                run_union(par, a, pbase + nb - 1);
+            }
+        }
+        ccarry += wave_sum(cn);
+        pcarry += wave_sum(pn);
+    }
+}
+
+template <bool COUNT>
+__global__ void __launch_bounds__(RUN_NT)
+k_run_resolve(RunGeom g, const int32_t *__restrict__ roff, int32_t *par, int32_t *__restrict__ kcnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const int32_t off = roff[row], n = roff[row + 1] - off;
+    int roots = 0;
+    for (int k = lane; k < n; k += 64) {
+        const int32_t r = off + k;
+        // read-only walk: a halving store here could replace another run's
+        // freshly written root by a mere ancestor
+        int32_t root = r, p = __hip_atomic_load(&par[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (p != root) {
+            root = p;
+            p = __hip_atomic_load(&par[root], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (root == r) roots++;
+        else __hip_atomic_store(&par[r], root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (COUNT) {
+        roots = wave_sum(roots);
+        if (lane == 0) kcnt[row] = roots;
+    }
+}
+
+__global__ void __launch_bounds__(RUN_NT)
+k_run_sizes(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff,
+            const int32_t *__restrict__ par, int32_t *size)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const uint64_t *cur = bits + row * g.nw;
+    const int32_t cbase = roff[row];
+    int ccarry = 0;
+    for (int c0 = 0; c0 < g.nw; c0 += 64) {
+        const int w = c0 + lane;
+        const bool act = w < g.nw;
+        const uint64_t cw = act ? cur[w] : 0, cwl = (act && w > 0) ? cur[w - 1] : 0;
+        const uint64_t cst = run_starts(cw, cwl);
+        const int cn = __popcll(cst);
+        const int cpre = ccarry + wave_excl_scan(cn, lane);
+        // pieces of runs in this word: the runs starting here, and the run
+        // continuing from word w-1
+        uint64_t piece = cst | (cw & 1ull);
+        while (piece) {
+            const int s = __builtin_ctzll(piece);
+            piece &= piece - 1;
+            const uint64_t rest = ~(cw >> s);
+            const int len = rest ? __builtin_ctzll(rest) : 64;
+            const int32_t r = cbase + cpre + __popcll(cst & le_mask(s)) - 1;
+            atomicAdd(&size[par[r]], len);
+        }
+        ccarry += wave_sum(cn);
+    }
+}
+
+__global__ void __launch_bounds__(RUN_NT)
+k_run_count(RunGeom g, const int32_t *__restrict__ roff, const int32_t *__restrict__ par,
+            const int32_t *__restrict__ size, int64_t npts, int32_t *__restrict__ kcnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const int32_t off = roff[row], n = roff[row + 1] - off;
+    int kept = 0;
+    for (int k = lane; k < n; k += 64) {
+        const int32_t r = off + k;
+        if (par[r] == r && size[r] >= npts) kept++;
+    }
+    kept = wave_sum(kept);
+    if (lane == 0) kcnt[row] = kept;
+}
+
+__global__ void __launch_bounds__(RUN_NT)
+k_run_rank(RunGeom g, const int32_t *__restrict__ roff, const int32_t *__restrict__ koff, int32_t *par,
+           const int32_t *__restrict__ size, int64_t npts)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const int32_t off = roff[row], n = roff[row + 1] - off;
+    int32_t base = koff[row];
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int k0 = 0; k0 < n; k0 += 64) {
+        const int k = k0 + lane;
+        const int32_t r = off + k;
+        const bool root = k < n && par[r] == r;
+        const bool keep = root && (!size || size[r] >= npts);
+        const uint64_t m = __ballot(keep);
+        if (root) par[r] = ~(keep ? base + __popcll(m & below) + 1 : 0);
+        base += __popcll(m);
+    }
+}
+
+__global__ void __launch_bounds__(RUN_NT)
+k_run_label(RunGeom g, const int32_t *__restrict__ roff, int32_t *par)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const int32_t off = roff[row], n = roff[row + 1] - off;
+    for (int k = lane; k < n; k += 64) {
+        const int32_t v = par[off + k];
+        if (v >= 0) par[off + k] = par[v]; // a root's entry already holds ~label
+    }
+}
+
+__global__ void __launch_bounds__(RUN_NT)
+k_run_final(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff,
+            const int32_t *__restrict__ par, int32_t *__restrict__ out, int vec)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const int nw = g.nw;
+    const uint64_t *cur = bits + row * nw;
+    int32_t *orow = out + row * g.W;
+    const int32_t cbase = roff[row];
+    int ccarry = 0;
+    for (int c0 = 0; c0 < nw; c0 += 64) {
+        const int w = c0 + lane;
+        const bool act = w < nw;
+        const uint64_t cw = act ? cur[w] : 0, cwl = (act && w > 0) ? cur[w - 1] : 0;
+        const uint64_t cst = run_starts(cw, cwl);
+        const int cn = __popcll(cst);
+        const int cpre = ccarry + wave_excl_scan(cn, lane);
+        const int nwc = min(64, nw - c0);
+        // 256 pixels (4 words) per step: lane l takes pixels 4(l%16)..+3 of
+        // word l/16
+        for (int j0 = 0; j0 < nwc; j0 += 4) {
+            const int jj = j0 + (lane >> 4);
+            const uint64_t wj = __shfl(cw, jj, 64);
+            const uint64_t sj = __shfl(cst, jj, 64);
+            const int pj = __shfl(cpre, jj, 64);
+            const int b0 = (lane & 15) * 4;
+            const int64_t x0 = (int64_t)(c0 + jj) * 64 + b0;
+            const uint32_t nib = (uint32_t)(wj >> b0) & 0xFu;
+            int v[4] = {0, 0, 0, 0};
+            if (nib) {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if ((nib >> i) & 1u) v[i] = ~par[cbase + pj + __popcll(sj & le_mask(b0 + i)) - 1];
+            }
+            if (vec && x0 + 3 < g.W) {
+                *reinterpret_cast<int4 *>(orow + x0) = make_int4(v[0], v[1], v[2], v[3]);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (x0 + i < g.W) orow[x0 + i] = v[i];
+            }
+        }
+        ccarry += wave_sum(cn);
+    }
+}
+
+template <typename In>
+static int label_runs(In in, int ndim, const int64_t *shape, int prev, int64_t npts, int32_t *out, int64_t *n_labels,
+                      hipStream_t st)
+{
+    RunGeom g;
+    g.H = (int32_t)shape[ndim - 2];
+    g.W = (int32_t)shape[ndim - 1];
+    int64_t F = 1;
+    for (int d = 0; d < ndim - 2; d++) F *= shape[d];
+    g.rows = F * g.H;
+    g.nw = (g.W + 63) / 64;
+    g.prev = prev;
+    const int64_t R = g.rows;
+    // horizontal neighbours join: at most ceil(W / 2) runs per row
+    const int64_t max_runs = R * ((g.W + 1) / 2);
+    if (max_runs >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "label: too many elements for the run path");
+    const bool sizes = npts > 1;
+
+    DevBuf<uint64_t> bits;
+    DevBuf<int32_t> rcnt, roff, par, kcnt, koff, size;
+    CBX_HIP(bits.alloc(R * g.nw, st));
+    CBX_HIP(rcnt.alloc(R + 1, st));
+    CBX_HIP(roff.alloc(R + 1, st));
+    CBX_HIP(par.alloc(max_runs, st));
+    CBX_HIP(kcnt.alloc(R + 1, st));
+    CBX_HIP(koff.alloc(R + 1, st));
+    if (sizes) CBX_HIP(size.alloc(max_runs, st));
+    CBX_HIP(hipMemsetAsync(rcnt.ptr + R, 0, sizeof(int32_t), st));
+    CBX_HIP(hipMemsetAsync(kcnt.ptr + R, 0, sizeof(int32_t), st));
+    const unsigned grid = (unsigned)((R + RUN_ROWS - 1) / RUN_ROWS);
+    KernelTimer timer("label_total", st);
+    k_run_pack<In><<<grid, RUN_NT, 0, st>>>(in, g, bits.ptr, rcnt.ptr);
+    CBX_HIP(exclusive_scan_i32(rcnt.ptr, roff.ptr, R + 1, st));
+    if (sizes) k_run_init<true><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, size.ptr);
+    else k_run_init<false><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, nullptr);
+    if (prev & 14) k_run_union<<<grid, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr);
+    if (sizes) {
+        k_run_resolve<false><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, nullptr);
+        k_run_sizes<<<grid, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr, size.ptr);
+        k_run_count<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, size.ptr, npts, kcnt.ptr);
+    } else {
+        k_run_resolve<true><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, kcnt.ptr);
+    }
+    CBX_HIP(exclusive_scan_i32(kcnt.ptr, koff.ptr, R + 1, st));
+    k_run_rank<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, koff.ptr, par.ptr, sizes ? size.ptr : nullptr, npts);
+    k_run_label<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr);
+    const int vec = (g.W % 4 == 0 && ((uintptr_t)out & 15) == 0) ? 1 : 0;
+    k_run_final<<<grid, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr, out, vec);
+    CBX_HIP(hipGetLastError());
+    timer.stop();
+    int32_t n = 0;
+    CBX_HIP(hipMemcpyAsync(&n, koff.ptr + R, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    *n_labels = n;
+    return CBX_OK;
+}
+
 // Returns -1 if the structure is not handled by the band path.
 static int band_prev_mask(int ndim, const int64_t *shape, const int64_t *shifts, int64_t n_shifts)
 {
@@ -560,7 +985,7 @@ static int label_band(In in, int ndim, const int64_t *shape, int prev, int64_t n
 }
 
 template <typename In>
-static int label_run(In in, int ndim, const int64_t *shape, const int64_t *shifts, int64_t n_shifts, int64_t npts,
+static int label_dispatch(In in, int ndim, const int64_t *shape, const int64_t *shifts, int64_t n_shifts, int64_t npts,
                      int32_t *out, int64_t *n_labels, hipStream_t st)
 {
     if (!n_labels) return fail(CBX_EVALUE, "n_labels must not be NULL");
@@ -579,7 +1004,10 @@ static int label_run(In in, int ndim, const int64_t *shape, const int64_t *shift
     if (P == 0) return CBX_OK;
     {
         const int prev = band_prev_mask(ndim, shape, shifts, n_shifts);
-        if (prev >= 0 && shape[ndim - 1] <= BAND_PX && shape[ndim - 1] >= 1)
+        const int forced = g_label_path;
+        if (prev >= 0 && In::binary && (prev & 1) && forced == 0)
+            return label_runs(in, ndim, shape, prev, npts, out, n_labels, st);
+        if (prev >= 0 && shape[ndim - 1] <= BAND_PX && shape[ndim - 1] >= 1 && forced <= 1)
             return label_band(in, ndim, shape, prev, npts, out, n_labels, st);
     }
 
@@ -670,15 +1098,16 @@ int cbx_label(const void *inp, int dtype, int ndim, const int64_t *shape, const 
               int64_t npts, int32_t *out, int64_t *n_labels, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
+    cbx::Workspace ws(st);
     switch (dtype) {
     case CBX_BOOL:
-        return cbx::label_run(cbx::ValueIn<uint8_t>{(const uint8_t *)inp}, ndim, shape, shifts, n_shifts, npts, out,
+        return cbx::label_dispatch(cbx::ValueIn<uint8_t>{(const uint8_t *)inp}, ndim, shape, shifts, n_shifts, npts, out,
                               n_labels, st);
     case CBX_INT32:
-        return cbx::label_run(cbx::ValueIn<int32_t>{(const int32_t *)inp}, ndim, shape, shifts, n_shifts, npts, out,
+        return cbx::label_dispatch(cbx::ValueIn<int32_t>{(const int32_t *)inp}, ndim, shape, shifts, n_shifts, npts, out,
                               n_labels, st);
     case CBX_INT64:
-        return cbx::label_run(cbx::ValueIn<int64_t>{(const int64_t *)inp}, ndim, shape, shifts, n_shifts, npts, out,
+        return cbx::label_dispatch(cbx::ValueIn<int64_t>{(const int64_t *)inp}, ndim, shape, shifts, n_shifts, npts, out,
                               n_labels, st);
     default:
         return cbx::fail(CBX_EVALUE, "unsupported label input dtype");
@@ -688,8 +1117,11 @@ int cbx_label(const void *inp, int dtype, int ndim, const int64_t *shape, const 
 int cbx_label_threshold_f32(const float *image, float vmin, int ndim, const int64_t *shape, const int64_t *shifts,
                             int64_t n_shifts, int64_t npts, int32_t *out, int64_t *n_labels, void *stream)
 {
-    return cbx::label_run(cbx::ThresholdF32{image, vmin}, ndim, shape, shifts, n_shifts, npts, out, n_labels,
+    cbx::Workspace ws((hipStream_t)stream);
+    return cbx::label_dispatch(cbx::ThresholdF32{image, vmin}, ndim, shape, shifts, n_shifts, npts, out, n_labels,
                           (hipStream_t)stream);
 }
+
+void cbx_label_force_path(int path) { cbx::g_label_path = (path >= 0 && path <= 2) ? path : 0; }
 
 } // extern "C"

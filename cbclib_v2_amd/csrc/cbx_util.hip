@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
+#include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -64,6 +65,130 @@ static void timing_drain(TimingState &t)
     t.pending.clear();
 }
 
+// ---------------------------------------------------------------------------
+// workspace caches
+// ---------------------------------------------------------------------------
+struct Cache {
+    int device;
+    hipStream_t stream;
+    bool busy;
+    std::vector<std::pair<void *, size_t>> blocks;
+};
+
+static std::mutex g_ws_mu;
+static std::vector<Cache *> g_ws;
+static thread_local Workspace *g_ws_current = nullptr;
+static const size_t WS_ALIGN = 256;
+static const size_t WS_MIN_BLOCK = size_t(64) << 20;
+
+Workspace *Workspace::current() { return g_ws_current; }
+
+Workspace::Workspace(hipStream_t s)
+{
+    int dev = 0;
+    status_ = hipGetDevice(&dev);
+    if (status_ != hipSuccess) return;
+    {
+        std::lock_guard<std::mutex> g(g_ws_mu);
+        for (Cache *c : g_ws)
+            if (c->device == dev && c->stream == s && !c->busy) {
+                cache_ = c;
+                break;
+            }
+        if (!cache_) {
+            cache_ = new Cache{dev, s, false, {}};
+            g_ws.push_back(cache_);
+        }
+        cache_->busy = true;
+    }
+    if (cache_->blocks.size() > 1) {
+        // merge: wait for the stream's earlier work, then one block of the total
+        size_t total = 0;
+        for (auto &b : cache_->blocks) total += b.second;
+        status_ = hipStreamSynchronize(s);
+        for (auto &b : cache_->blocks) (void)hipFree(b.first);
+        cache_->blocks.clear();
+        if (status_ == hipSuccess) {
+            void *p = nullptr;
+            status_ = hipMalloc(&p, total);
+            if (status_ == hipSuccess) cache_->blocks.push_back({p, total});
+        }
+    }
+    outer_ = g_ws_current;
+    g_ws_current = this;
+}
+
+Workspace::~Workspace()
+{
+    if (!cache_) return;
+    g_ws_current = outer_;
+    std::lock_guard<std::mutex> g(g_ws_mu);
+    cache_->busy = false;
+}
+
+hipError_t Workspace::take(void **p, size_t bytes)
+{
+    *p = nullptr;
+    if (status_ != hipSuccess) return status_;
+    bytes = (bytes + WS_ALIGN - 1) / WS_ALIGN * WS_ALIGN;
+    if (bytes == 0) bytes = WS_ALIGN;
+    auto &bl = cache_->blocks;
+    while (block_ < bl.size()) {
+        if (bl[block_].second - off_ >= bytes) {
+            *p = (char *)bl[block_].first + off_;
+            off_ += bytes;
+            return hipSuccess;
+        }
+        block_++;
+        off_ = 0;
+    }
+    size_t have = 0;
+    for (auto &b : bl) have += b.second;
+    const size_t size = std::max(bytes, std::max(WS_MIN_BLOCK, have));
+    void *q = nullptr;
+    hipError_t e = hipMalloc(&q, size);
+    if (e != hipSuccess) return e;
+    bl.push_back({q, size});
+    block_ = bl.size() - 1;
+    off_ = bytes;
+    *p = q;
+    return hipSuccess;
+}
+
+static size_t workspace_release()
+{
+    std::lock_guard<std::mutex> g(g_ws_mu);
+    size_t freed = 0;
+    int dev0 = 0;
+    (void)hipGetDevice(&dev0);
+    for (auto it = g_ws.begin(); it != g_ws.end();) {
+        Cache *c = *it;
+        if (c->busy) {
+            ++it;
+            continue;
+        }
+        (void)hipSetDevice(c->device);
+        (void)hipDeviceSynchronize();
+        for (auto &b : c->blocks) {
+            (void)hipFree(b.first);
+            freed += b.second;
+        }
+        delete c;
+        it = g_ws.erase(it);
+    }
+    (void)hipSetDevice(dev0);
+    return freed;
+}
+
+static size_t workspace_bytes()
+{
+    std::lock_guard<std::mutex> g(g_ws_mu);
+    size_t n = 0;
+    for (Cache *c : g_ws)
+        for (auto &b : c->blocks) n += b.second;
+    return n;
+}
+
 template <typename V>
 static hipError_t exclusive_scan(const V *in, V *out, int64_t n, hipStream_t s)
 {
@@ -71,12 +196,10 @@ static hipError_t exclusive_scan(const V *in, V *out, int64_t n, hipStream_t s)
     size_t bytes = 0;
     hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s);
     if (e != hipSuccess) return e;
-    void *tmp = nullptr;
-    e = hipMalloc(&tmp, bytes ? bytes : 1);
+    DevBuf<uint8_t> tmp;
+    e = tmp.alloc(bytes ? bytes : 1, s);
     if (e != hipSuccess) return e;
-    e = hipcub::DeviceScan::ExclusiveSum(tmp, bytes, in, out, (int)n, s);
-    hipError_t f = hipFree(tmp);
-    return (e != hipSuccess) ? e : f;
+    return hipcub::DeviceScan::ExclusiveSum(tmp.ptr, bytes, in, out, (int)n, s);
 }
 
 hipError_t exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s) { return exclusive_scan(in, out, n, s); }
@@ -118,6 +241,10 @@ int cbx_timing_read(const char *name, double *total_ms, int64_t *count)
     *count = (it == t.total.end()) ? 0 : it->second.second;
     return CBX_OK;
 }
+
+int64_t cbx_workspace_release(void) { return (int64_t)cbx::workspace_release(); }
+
+int64_t cbx_workspace_bytes(void) { return (int64_t)cbx::workspace_bytes(); }
 
 int cbx_kernel_id(const char *name)
 {

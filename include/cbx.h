@@ -153,6 +153,16 @@ CBX_API int cbx_label_threshold_f32(const float *image, float vmin, int ndim, co
                             int64_t n_shifts, int64_t npts, int32_t *out, int64_t *n_labels, void *stream);
 
 /*
+ * Path selection for tests and measurements (no reference counterpart):
+ * 0 (default) picks the fastest applicable label algorithm (run-length
+ * union-find for binary inputs with horizontal connectivity, band union-find
+ * for 3x3 2D structures, element union-find otherwise); 1 never uses the
+ * run-length path; 2 uses the element union-find only.  All give the same
+ * labels.  Process-wide; not for concurrent use with running label calls.
+ */
+CBX_API void cbx_label_force_path(int path);
+
+/*
  * Instrumentation (no reference counterpart; the reference has no timers):
  * per-kernel durations measured with HIP events recorded on the launch
  * stream around the library's main kernels ("draw_tiles",
@@ -163,6 +173,18 @@ CBX_API int cbx_label_threshold_f32(const float *image, float vmin, int ndim, co
 CBX_API void cbx_timing_enable(int on);
 CBX_API void cbx_timing_reset(void);
 CBX_API int cbx_timing_read(const char *name, double *total_ms, int64_t *count);
+
+/*
+ * Workspace (no reference counterpart; the reference allocates per call on
+ * the host).  Intermediate device buffers of every entry point come from a
+ * cache kept per (device, stream) and reused by later calls on the same
+ * stream, so steady-state calls make no hipMalloc / hipFree.  The cache only
+ * grows.  cbx_workspace_release waits for the devices and frees every cache
+ * not in use by a running call, returning the bytes freed;
+ * cbx_workspace_bytes returns the bytes held.
+ */
+CBX_API int64_t cbx_workspace_release(void);
+CBX_API int64_t cbx_workspace_bytes(void);
 
 #ifdef __cplusplus
 }

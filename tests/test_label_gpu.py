@@ -1,12 +1,30 @@
 """label() on the GPU against the oracle (pinned to the compiled reference and
-to scipy): label maps and counts bit-exact."""
+to scipy): label maps and counts bit-exact.
+
+Every case runs under each of the three label algorithms (``path``): 0 the
+fastest applicable (run-length union-find for binary inputs with horizontal
+connectivity), 1 without the run-length path (band union-find for 3x3 2D
+structures), 2 the element union-find only.
+"""
+import ctypes
+
 import numpy as np
 import pytest
 import torch
 
+from cbclib_v2_amd import _native
 from cbclib_v2_amd import label as cl
 
 pytestmark = pytest.mark.gpu
+
+PATHS = [0, 1, 2]
+
+
+@pytest.fixture(params=PATHS)
+def path(request, cuda):
+    _native.label_force_path(request.param)
+    yield request.param
+    _native.label_force_path(0)
 
 
 def masks(seed):
@@ -25,7 +43,7 @@ def masks(seed):
     return inp, cl.Structure(radii, conn), int(rng.integers(1, 4))
 
 
-def test_label_random(cuda, oracle):
+def test_label_random(path, oracle):
     for seed in range(60):
         inp, st, npts = masks(seed)
         ref, n = oracle.label(inp, st.shifts(), npts)
@@ -34,7 +52,7 @@ def test_label_random(cuda, oracle):
         assert np.array_equal(res.labels.cpu().numpy(), ref), f"seed {seed}"
 
 
-def test_label_per_frame_stack(cuda, oracle):
+def test_label_per_frame_stack(path, oracle):
     """Config-2 pattern: a stack of frames, per-frame 8-connectivity Structure([0,1,1],2)."""
     rng = np.random.default_rng(5)
     inp = rng.random((6, 120, 140)) > 0.62
@@ -45,7 +63,7 @@ def test_label_per_frame_stack(cuda, oracle):
     assert np.array_equal(res.labels.cpu().numpy(), ref)
 
 
-def test_label_threshold_fused(cuda, oracle):
+def test_label_threshold_fused(path, oracle):
     rng = np.random.default_rng(6)
     img = rng.random((4, 100, 130)).astype(np.float32)
     st = cl.Structure([0, 1, 1], 2)
@@ -76,7 +94,7 @@ def test_label_errors(cuda):
 @pytest.mark.parametrize('shape', [(64, 2048), (3, 40, 1024), (12, 8192), (2, 5, 8193), (700, 13)])
 @pytest.mark.parametrize('conn', [1, 2])
 @pytest.mark.parametrize('kind', ['bool', 'int32', 'int64'])
-def test_label_bands(cuda, oracle, shape, conn, kind):
+def test_label_bands(path, oracle, shape, conn, kind):
     """Many bands per frame (band = whole rows of <= 8192 pixels), the border
     merge, equal-value inputs, the npts filter, and the general fallback for
     rows wider than a band."""
@@ -90,5 +108,52 @@ def test_label_bands(cuda, oracle, shape, conn, kind):
     for npts in (1, 4):
         ref, n = oracle.label(inp, st.shifts(), npts)
         res = cl.label(torch.from_numpy(inp).cuda(), st, npts)
-        assert int(res.index.numel()) == n, (shape, conn, kind, npts)
-        assert np.array_equal(res.labels.cpu().numpy(), ref), (shape, conn, kind, npts)
+        assert int(res.index.numel()) == n, (shape, conn, kind, npts, path)
+        assert np.array_equal(res.labels.cpu().numpy(), ref), (shape, conn, kind, npts, path)
+
+
+def raw_label(inp: np.ndarray, shifts: np.ndarray, npts: int):
+    """cbx_label through the C ABI with an arbitrary (symmetric) shift set."""
+    t_in = torch.from_numpy(inp).cuda()
+    out = torch.empty(inp.shape, dtype=torch.int32, device='cuda')
+    shp = np.asarray(inp.shape, dtype=np.int64)
+    sh = np.ascontiguousarray(shifts, dtype=np.int64)
+    n = ctypes.c_int64(0)
+    _native.check(_native.lib().cbx_label(t_in.data_ptr(), _native.DTYPES['bool'], inp.ndim, shp.ctypes.data,
+                                          sh.ctypes.data, sh.shape[0], npts, out.data_ptr(), ctypes.byref(n), None))
+    torch.cuda.synchronize()
+    return out.cpu().numpy(), n.value
+
+
+@pytest.mark.parametrize('upper', [(), ((-1, -1),), ((-1, 1),), ((-1, -1), (-1, 1)), ((-1, 0),),
+                                   ((-1, -1), (-1, 0), (-1, 1))])
+def test_label_structures(path, oracle, upper):
+    """Horizontal neighbours plus any subset of the upper three (and mirrors):
+    the contact rules of the run-length path for diagonal-only sets."""
+    rng = np.random.default_rng(len(upper) * 7 + sum(dx for _, dx in upper))
+    sh = [(0, 0), (0, -1), (0, 1)] + [s for u in upper for s in (u, (-u[0], -u[1]))]
+    shifts = np.array(sh, np.int64)
+    for shape, dens in (((37, 200), 0.5), ((5, 64), 0.8), ((130, 129), 0.3), ((1, 1000), 0.6)):
+        inp = rng.random(shape) < dens
+        for npts in (1, 3):
+            ref, n = oracle.label(inp, shifts, npts)
+            got, m = raw_label(inp, shifts, npts)
+            assert m == n and np.array_equal(got, ref), (shape, upper, npts, path)
+
+
+def test_label_large_binary(path, oracle):
+    """A C2-sized frame pair (2 x 2048 x 2048, long runs, mostly background)."""
+    if path == 2:
+        pytest.skip("element union-find on 8 Mpx is covered by the smaller cases")
+    rng = np.random.default_rng(12)
+    img = np.zeros((2, 2048, 2048), np.float32)
+    for f in range(2):
+        for _ in range(300):
+            y, x = rng.integers(0, 2048, 2)
+            h, w = rng.integers(1, 40, 2)
+            img[f, y:y + h, x:x + w] = rng.random()
+    st = cl.Structure([0, 1, 1], 2)
+    ref, n = oracle.label(img > 0.3, st.shifts(), 1)
+    res = cl.label_threshold(torch.from_numpy(img).cuda(), 0.3, st)
+    assert int(res.index.numel()) == n
+    assert np.array_equal(res.labels.cpu().numpy(), ref)
