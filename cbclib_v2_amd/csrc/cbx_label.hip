@@ -487,13 +487,17 @@ k_band_final(In in, BandGeom g, const int32_t *__restrict__ lab, int32_t *__rest
 //   R1 k_run_pack     one wave per row: the row as 64-bit words (bit x%64 of
 //                     word x/64 = pixel x is on) and its run count.
 //   R2 scan           first run id of every row.
-//   R3 k_run_init     par[r] = r (size[r] = 0).
-//   R4 k_run_union    one wave per row y (y-1 in the same frame): each
-//                     contact (a pixel of row y and an upper neighbour in
-//                     row y-1, both on) at which the run of row y or the run
-//                     of row y-1 starts is unioned.  That covers every pair
-//                     of touching runs: moving a contact one pixel to the left
+//   R3 k_run_band     union-find of the runs of a band of RUN_BAND rows in
+//                     LDS (one workgroup per band); par[r] = band-local root.
+//                     Contacts (a pixel of row y and an upper neighbour in
+//                     row y-1, both on) are unioned where the run of row y or
+//                     the run of row y-1 starts.  That covers every pair of
+//                     touching runs: moving a contact one pixel to the left
 //                     keeps it a contact until one of the two runs starts.
+//   R4 k_run_border   the same for the first row of each band and the row
+//                     above it, lock-free in global memory.
+//                     (k_run_init: par[r] = r when no upper neighbour is in
+//                     the structure; size[r] = 0 for npts > 1.)
 //   R5 k_run_resolve  par[r] = root (read-only root walks, so every entry
 //                     ends as its root); without a size limit also the number of
 //                     roots per row.  (npts > 1: k_run_sizes adds each run's
@@ -623,12 +627,13 @@ k_run_init(RunGeom g, const int32_t *__restrict__ roff, int32_t *__restrict__ pa
     }
 }
 
-__global__ void __launch_bounds__(RUN_NT)
-k_run_union(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff, int32_t *par)
+// Contacts between row y and row y-1 (same frame) for one wave: calls
+// join(a, b) with the run ids of each contact pair that the event rule
+// keeps (a run of row y, b of row y-1; b < a).
+template <typename Join>
+__device__ __forceinline__ void run_contacts(const RunGeom &g, const uint64_t *__restrict__ bits,
+                                             const int32_t *__restrict__ roff, int64_t row, int lane, Join join)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
-    if (row >= g.rows || row % g.H == 0) return;
     const int nw = g.nw;
     const uint64_t *cur = bits + row * nw, *up = cur - nw;
     const int32_t cbase = roff[row], pbase = roff[row - 1];
@@ -667,12 +672,65 @@ k_run_union(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restr
                 int nb = ppre + __popcll(pst & le);
                 if (d > 0) nb += (int)((sps >> p) & 1ull);
                 if (d < 0) nb -= (int)((pst >> p) & 1ull);
-                run_union(par, a, pbase + nb - 1);
+                join(a, pbase + nb - 1);
             }
         }
         ccarry += wave_sum(cn);
         pcarry += wave_sum(pn);
     }
+}
+
+// One workgroup per band of RUN_BAND rows of one frame.  The band's runs are
+// the id range [roff[y0], roff[y0 + rows]); if it fits RUN_LCAP they are
+// unioned in LDS and every run's band-local root is written to par (a
+// band-local root is the smallest id of its band component, so the forest
+// stays min-rooted); otherwise par is initialised and the band's contacts
+// are unioned in global memory.  Contacts across band borders are left to
+// k_run_border.
+constexpr int RUN_BAND = 32;
+constexpr int RUN_LCAP = 8192;
+
+__global__ void __launch_bounds__(RUN_NT)
+k_run_band(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff, int32_t *par)
+{
+    __shared__ int32_t lpar[RUN_LCAP];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int bpf = (g.H + RUN_BAND - 1) / RUN_BAND;
+    const int64_t f = blockIdx.x / bpf;
+    const int y0 = (int)(blockIdx.x - f * bpf) * RUN_BAND;
+    const int rows = min(RUN_BAND, g.H - y0);
+    const int64_t row0 = f * g.H + y0;
+    const int32_t r0 = roff[row0], nb = roff[row0 + rows] - r0;
+    if (nb <= RUN_LCAP) {
+        for (int k = threadIdx.x; k < nb; k += RUN_NT) lpar[k] = k;
+        __syncthreads();
+        for (int y = 1 + wave; y < rows; y += RUN_ROWS)
+            run_contacts(g, bits, roff, row0 + y, lane, [&](int32_t a, int32_t b) { lds_union(lpar, a - r0, b - r0); });
+        __syncthreads();
+        for (int k = threadIdx.x; k < nb; k += RUN_NT) par[r0 + k] = r0 + lds_find(lpar, k);
+    } else {
+        for (int k = threadIdx.x; k < nb; k += RUN_NT)
+            __hip_atomic_store(&par[r0 + k], r0 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        __syncthreads();
+        for (int y = 1 + wave; y < rows; y += RUN_ROWS)
+            run_contacts(g, bits, roff, row0 + y, lane, [&](int32_t a, int32_t b) { run_union(par, a, b); });
+    }
+}
+
+// Contacts between the first row of a band and the row above it (same frame).
+__global__ void __launch_bounds__(RUN_NT)
+k_run_border(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff, int64_t n_bands,
+             int32_t *par)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t band = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (band >= n_bands) return;
+    const int bpf = (g.H + RUN_BAND - 1) / RUN_BAND;
+    const int64_t f = band / bpf;
+    const int y0 = (int)(band - f * bpf) * RUN_BAND;
+    if (y0 == 0) return;
+    run_contacts(g, bits, roff, f * g.H + y0, lane, [&](int32_t a, int32_t b) { run_union(par, a, b); });
 }
 
 template <bool COUNT>
@@ -867,8 +925,14 @@ static int label_runs(In in, int ndim, const int64_t *shape, int prev, int64_t n
     k_run_pack<In><<<grid, RUN_NT, 0, st>>>(in, g, bits.ptr, rcnt.ptr);
     CBX_HIP(exclusive_scan_i32(rcnt.ptr, roff.ptr, R + 1, st));
     if (sizes) k_run_init<true><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, size.ptr);
-    else k_run_init<false><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, nullptr);
-    if (prev & 14) k_run_union<<<grid, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr);
+    if (prev & 14) {
+        const int64_t n_bands = (int64_t)(R / g.H) * ((g.H + RUN_BAND - 1) / RUN_BAND);
+        k_run_band<<<(unsigned)n_bands, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr);
+        k_run_border<<<(unsigned)((n_bands + RUN_ROWS - 1) / RUN_ROWS), RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr,
+                                                                                         n_bands, par.ptr);
+    } else if (!sizes) {
+        k_run_init<false><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, nullptr);
+    }
     if (sizes) {
         k_run_resolve<false><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, nullptr);
         k_run_sizes<<<grid, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr, size.ptr);
