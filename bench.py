@@ -177,7 +177,7 @@ def main() -> None:
     _native.timing_enable(False)
     tiles_ms, tiles_n = _native.timing_read('draw_tiles')
     label_ms, label_n = _native.timing_read('label_total')
-    pass1_ms, pass1_n = _native.timing_read('segments_pass1')
+    walk_ms, walk_n = _native.timing_read('segments_walk')
     # per-part breakdown on separate, untimed-for-value steps
     for _ in range(2):
         step(True)
@@ -211,7 +211,7 @@ def main() -> None:
         "fwd_Mlines_per_s": round(N / (acc['fwd'] / 2 * 1e-3) / 1e6, 4),
         "frames_per_s": round(F / (ms_step * 1e-3), 2),
         "breakdown_ms": {k: round(v / 2, 4) for k, v in acc.items()},
-        "segments_pass1_ms": round(pass1_ms / max(pass1_n, 1), 4),
+        "segments_walk_ms": round(walk_ms / max(walk_n, 1), 4),
         "label_total_ms": round(label_ms / max(label_n, 1), 4),
         "label": {"n_labels": n_labels, "mask_density": round(mask_density, 6),
                   "algorithmic_GBs": round(8.0 * F * H * W / (label_ms / max(label_n, 1) * 1e-3) / 1e9, 2)},

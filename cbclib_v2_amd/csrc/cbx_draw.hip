@@ -58,40 +58,48 @@ struct Geometry {
 };
 
 // ---------------------------------------------------------------------------
-// 1/3. segment pre-pass
+// segment pre-passes
 // ---------------------------------------------------------------------------
-template <typename T, typename I, bool WRITE, bool BIN>
-__global__ void __launch_bounds__(256)
-k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_lines, Geometry g,
-           LineSetup<T> *__restrict__ setup, int *__restrict__ seg_cnt, int *__restrict__ tile_cnt,
-           const int *__restrict__ seg_off, Segment<T> *__restrict__ segs, const int *__restrict__ tile_off,
-           int *__restrict__ tile_cur, int *__restrict__ items, int *__restrict__ flags)
+// Line constants and frame check of line l; false if the line draws nothing.
+template <typename T, typename I>
+__device__ __forceinline__ bool setup_line(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t l,
+                                           const Geometry &g, LineSetup<T> &L, int *__restrict__ flags)
 {
-    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n_lines) return;
-    LineSetup<T> L;
-    if (!WRITE) {
-        int rc = line_setup(lines + 5 * l, L);
-        int64_t f = idxs ? (int64_t)idxs[l] : ((g.n_frames == 1) ? 0 : l);
-        if (f < 0 || f >= g.n_frames) {
-            atomicOr(flags, FLAG_INDEX);
-            L.valid = 0;
-        }
-        if (rc) {
-            atomicOr(flags, rc);
-            L.valid = 0;
-        }
-        L.frame = (int32_t)f;
-        setup[l] = L;
-        if (!L.valid) {
-            seg_cnt[l] = 0;
-            return;
-        }
-    } else {
-        L = setup[l];
-        if (!L.valid) return;
+    int rc = line_setup(lines + 5 * l, L);
+    int64_t f = idxs ? (int64_t)idxs[l] : ((g.n_frames == 1) ? 0 : l);
+    if (f < 0 || f >= g.n_frames) {
+        atomicOr(flags, FLAG_INDEX);
+        L.valid = 0;
     }
+    if (rc) {
+        atomicOr(flags, rc);
+        L.valid = 0;
+    }
+    L.frame = (int32_t)f;
+    return L.valid;
+}
 
+// Most segments a line's walk can make: one per SEG-column block of the
+// long-axis range [m0, m1] inside the frame (the walk is monotone along the
+// long axis and never passes m1).
+template <typename T>
+__device__ __forceinline__ int segment_bound(const LineSetup<T> &L, const Geometry &g)
+{
+    if (!L.valid) return 0;
+    const int a = L.axis, da = (a == 0) ? g.W : g.H;
+    const int c0 = pick(L.m0, a), c1 = pick(L.m1, a);
+    const int lo = max(min(c0, c1), 0), hi = min(max(c0, c1), da - 1);
+    return (lo > hi) ? 0 : hi / g.seg - lo / g.seg + 1;
+}
+
+// Replays line l's walk (bresenham.hpp:352-383) and cuts it into segments of
+// <= SEG long-axis columns inside the frame.  For each segment whose fills
+// can reach the frame, on_seg(cur, tl) gets the checkpoint (with the reach
+// range lo..hi set) and the tile column tl along the long axis.
+template <typename T, typename OnSeg>
+__device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, const Geometry &g,
+                                              int *__restrict__ flags, OnSeg on_seg)
+{
     const int a = L.axis, b = 1 - a;
     const int da = (a == 0) ? g.W : g.H, db = (a == 0) ? g.H : g.W;
     const int sa = pick(L.st, a);
@@ -104,10 +112,8 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
     s.Te = L.e0[0];
     s.Ne = L.e0[1];
 
-    const int base = WRITE ? seg_off[l] : 0;
-    int nseg = 0;
     bool open = false;
-    int blk = 0, tl = 0, nem = 0, smin = 0, smax = 0;
+    int b_lo = 0, b_hi = 0, tl = 0, nem = 0, smin = 0, smax = 0;
     double maxN = 0.0;
     Segment<T> cur;
 
@@ -119,17 +125,7 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
         if (lo > hi) return;
         cur.lo = lo;
         cur.hi = hi;
-        int sid = base + nseg;
-        if (WRITE) segs[sid] = cur;
-        if (BIN) {
-            for (int tb = lo / g.tile; tb <= hi / g.tile; tb++) {
-                int tx = (a == 0) ? tl : tb, ty = (a == 0) ? tb : tl;
-                int64_t tid = ((int64_t)L.frame * g.nty + ty) * g.ntx + tx;
-                if (!WRITE) atomicAdd(&tile_cnt[tid], 1);
-                else items[tile_off[tid] + atomicAdd(&tile_cur[tid], 1)] = sid;
-            }
-        }
-        nseg++;
+        on_seg(cur, tl);
     };
 
     // every step moves the walk; a walk takes at most one short-axis-only
@@ -146,11 +142,11 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
         bool nx, ny;
         walk_next(L, s.Ne, nx, ny);
         if (c >= 0 && c < da) {
-            const int bk = c / g.seg;
-            if (!open || bk != blk) {
+            if (!open || c < b_lo || c >= b_hi) {
                 if (open) close();
                 open = true;
-                blk = bk;
+                b_lo = (c / g.seg) * g.seg;
+                b_hi = min(b_lo + g.seg, da);
                 tl = c / g.tile;
                 nem = 0;
                 maxN = 0.0;
@@ -161,8 +157,7 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
                 cur.c[1] = s.c[1];
                 cur.Te = s.Te;
                 cur.Ne = s.Ne;
-                const int b0 = bk * g.seg, b1 = min(b0 + g.seg, da);
-                cur.ncols = (sa > 0) ? (b1 - c) : (c - b0 + 1);
+                cur.ncols = (sa > 0) ? (b_hi - c) : (c - b_lo + 1);
             }
             if ((a == 0) ? nx : ny) {
                 nem++;
@@ -175,7 +170,137 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
         walk_step(L, s, nx, ny);
     }
     if (open) close();
+}
+
+__device__ __forceinline__ int64_t tile_index(const Geometry &g, int frame, int axis, int tl, int tb)
+{
+    const int tx = (axis == 0) ? tl : tb, ty = (axis == 0) ? tb : tl;
+    return ((int64_t)frame * g.nty + ty) * g.ntx + tx;
+}
+
+// Ordered plan (pixel lists, deterministic images, adjoint): segments of a
+// line are consecutive and lines are in input order.  Pass 1 (WRITE = false)
+// counts segments per line and items per tile; pass 2 writes them at the
+// scanned offsets.
+template <typename T, typename I, bool WRITE, bool BIN>
+__global__ void __launch_bounds__(256)
+k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_lines, Geometry g,
+           LineSetup<T> *__restrict__ setup, int *__restrict__ seg_cnt, int *__restrict__ tile_cnt,
+           const int *__restrict__ seg_off, Segment<T> *__restrict__ segs, const int *__restrict__ tile_off,
+           int *__restrict__ tile_cur, int *__restrict__ items, int *__restrict__ flags)
+{
+    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= n_lines) return;
+    LineSetup<T> L;
+    if (!WRITE) {
+        const bool ok = setup_line(lines, idxs, l, g, L, flags);
+        setup[l] = L;
+        if (!ok) {
+            seg_cnt[l] = 0;
+            return;
+        }
+    } else {
+        L = setup[l];
+        if (!L.valid) return;
+    }
+    const int base = WRITE ? seg_off[l] : 0;
+    int nseg = 0;
+    walk_segments(L, l, g, flags, [&](const Segment<T> &cur, int tl) {
+        const int sid = base + nseg;
+        if (WRITE) segs[sid] = cur;
+        if (BIN) {
+            for (int tb = cur.lo / g.tile; tb <= cur.hi / g.tile; tb++) {
+                const int64_t tid = tile_index(g, L.frame, L.axis, tl, tb);
+                if (!WRITE) atomicAdd(&tile_cnt[tid], 1);
+                else items[tile_off[tid] + atomicAdd(&tile_cur[tid], 1)] = sid;
+            }
+        }
+        nseg++;
+    });
     if (!WRITE) seg_cnt[l] = nseg;
+}
+
+// Unordered plan (the tile path, whose result does not depend on segment
+// order): one walk per line.
+//   k_setup_lines   line constants, flags, and the total of the per-line
+//                   segment bounds (counters[0]);
+//   k_walk_lines    each wave takes slots for its lines' bounds with one
+//                   atomic (counters[1]), walks, writes the segments into
+//                   them (unused slots get ncols = 0) and counts tile items;
+//   k_bin_slots     one lane per slot: writes the per-tile item lists.
+template <typename T, typename I>
+__global__ void __launch_bounds__(256)
+k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_lines, Geometry g,
+              LineSetup<T> *__restrict__ setup, unsigned long long *__restrict__ counters, int *__restrict__ flags)
+{
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int ub = 0;
+    if (l < n_lines) {
+        LineSetup<T> L;
+        setup_line(lines, idxs, l, g, L, flags);
+        setup[l] = L;
+        ub = segment_bound(L, g);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) ub += __shfl_xor(ub, o, 64);
+    if ((threadIdx.x & 63) == 0 && ub) atomicAdd(&counters[0], (unsigned long long)ub);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
+             unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt, int *__restrict__ flags)
+{
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    LineSetup<T> L;
+    int ub = 0;
+    if (l < n_lines) {
+        L = setup[l];
+        ub = segment_bound(L, g);
+    }
+    // wave prefix of the bounds, one slot reservation per wave
+    int pre = ub;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int y = __shfl_up(pre, d, 64);
+        if (lane >= d) pre += y;
+    }
+    const int total = __shfl(pre, 63, 64);
+    pre -= ub;
+    unsigned long long wbase = 0;
+    if (lane == 0 && total) wbase = atomicAdd(&counters[1], (unsigned long long)total);
+    wbase = __shfl(wbase, 0, 64);
+    if (ub == 0) return;
+    const int64_t base = (int64_t)wbase + pre;
+    int nseg = 0;
+    walk_segments(L, l, g, flags, [&](const Segment<T> &cur, int tl) {
+        if (nseg < ub) segs[base + nseg] = cur;
+        for (int tb = cur.lo / g.tile; tb <= cur.hi / g.tile; tb++)
+            atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb)], 1);
+        nseg++;
+    });
+    if (nseg > ub) atomicOr(flags, FLAG_STUCK); // cannot happen: the bound holds for every walk
+    for (int k = nseg; k < ub; k++) segs[base + k].ncols = 0;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256)
+k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup, const Segment<T> *__restrict__ segs,
+            const int *__restrict__ tile_off, int *__restrict__ tile_cur, int *__restrict__ items)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    const Segment<T> &sg = segs[i];
+    if (sg.ncols <= 0) return;
+    const int lo = sg.lo, hi = sg.hi;
+    const int32_t line = sg.line;
+    const int axis = setup[line].axis, frame = setup[line].frame;
+    const int tl = pick(sg.c, axis) / g.tile;
+    for (int tb = lo / g.tile; tb <= hi / g.tile; tb++) {
+        const int64_t tid = tile_index(g, frame, axis, tl, tb);
+        items[tile_off[tid] + atomicAdd(&tile_cur[tid], 1)] = (int)i;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1028,6 +1153,73 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t 
     return CBX_OK;
 }
 
+// Unordered plan for the tile path: one walk per line (k_setup_lines,
+// k_walk_lines, k_bin_slots).  Segments live in slots (n_segs = slots; unused
+// slots have ncols = 0 and are in no item list).  Two host syncs: the slot
+// total, then the item total.
+template <typename T, typename I>
+static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t n_lines, int64_t n_frames,
+                                int64_t H, int64_t W, int tile, int seg, hipStream_t st)
+{
+    Geometry &g = P.g;
+    g.n_frames = n_frames;
+    g.H = (int32_t)H;
+    g.W = (int32_t)W;
+    g.tile = tile;
+    g.seg = seg;
+    g.ntx = (int32_t)((W + tile - 1) / tile);
+    g.nty = (int32_t)((H + tile - 1) / tile);
+    P.n_lines = n_lines;
+    P.n_tiles = n_frames * g.ntx * g.nty;
+    if (P.n_tiles >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "too many tiles");
+
+    DevBuf<unsigned long long> counters;
+    CBX_HIP(P.setup.alloc(n_lines, st));
+    CBX_HIP(P.flags.alloc(1, st));
+    CBX_HIP(counters.alloc(2, st));
+    CBX_HIP(P.tile_cnt.alloc(P.n_tiles + 1, st));
+    CBX_HIP(P.tile_off.alloc(P.n_tiles + 1, st));
+    CBX_HIP(hipMemsetAsync(P.flags.ptr, 0, sizeof(int), st));
+    CBX_HIP(hipMemsetAsync(counters.ptr, 0, 2 * sizeof(unsigned long long), st));
+    CBX_HIP(hipMemsetAsync(P.tile_cnt.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
+    const int blk = 256;
+    k_setup_lines<T, I><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, n_lines, g, P.setup.ptr, counters.ptr,
+                                                               P.flags.ptr);
+    CBX_HIP(hipGetLastError());
+    unsigned long long slots = 0;
+    int fl = 0;
+    CBX_HIP(hipMemcpyAsync(&slots, counters.ptr, sizeof(slots), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    int rc = flags_to_status(fl);
+    if (rc) return rc;
+    if (slots >= (1ull << 31)) return fail(CBX_EVALUE, "too many segments (2^31)");
+    P.n_segs = (int64_t)slots;
+    if (P.n_segs == 0) return CBX_OK;
+    CBX_HIP(P.segs.alloc(P.n_segs, st));
+    {
+        KernelTimer timer("segments_walk", st);
+        k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, counters.ptr,
+                                                                P.tile_cnt.ptr, P.flags.ptr);
+    }
+    CBX_HIP(hipGetLastError());
+    CBX_HIP(exclusive_scan_i32(P.tile_cnt.ptr, P.tile_off.ptr, P.n_tiles + 1, st));
+    int n_items = 0;
+    CBX_HIP(hipMemcpyAsync(&n_items, P.tile_off.ptr + P.n_tiles, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    if ((rc = flags_to_status(fl))) return rc;
+    if (n_items < 0) return fail(CBX_ERUNTIME, "tile item count overflow");
+    P.n_items = n_items;
+    CBX_HIP(P.items.alloc(P.n_items, st));
+    CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
+    CBX_HIP(hipMemsetAsync(P.tile_cur.ptr, 0, sizeof(int) * P.n_tiles, st));
+    k_bin_slots<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr, P.tile_off.ptr,
+                                                            P.tile_cur.ptr, P.items.ptr);
+    CBX_HIP(hipGetLastError());
+    return CBX_OK;
+}
+
 static void resolve_opts(const cbx_draw_opts *o, int &tile, int &seg)
 {
     // 64 x 64 tiles with a checkpoint every 8 columns: the fastest launch
@@ -1247,10 +1439,14 @@ static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const
         seg = 16;
     }
     SegmentPlan<T> P;
-    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
+    if (det) {
+        if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
+        if (P.n_segs == 0) return CBX_OK;
+        return ordered_pipeline<T>(P, kernel, overlap, max_val, out, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
+                                   nullptr, st);
+    }
+    if ((rc = build_plan_unordered<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, st))) return rc;
     if (P.n_segs == 0) return CBX_OK;
-    if (det) return ordered_pipeline<T>(P, kernel, overlap, max_val, out, nullptr, nullptr, nullptr, nullptr, nullptr,
-                                        0, nullptr, st);
     {
         KernelTimer timer("draw_tiles", st);
         const int threads = (opts && opts->threads) ? opts->threads : 256;
