@@ -173,6 +173,13 @@ __device__ __forceinline__ int line_setup(const T *ln, LineSetup<T> &L)
 // walk state and one step of LineIterator (bresenham.hpp:99-112, 149-154,
 // 198-222)
 // ---------------------------------------------------------------------------
+// s_waitcnt vmcnt(0) (gfx9 encoding: expcnt and lgkmcnt left at their
+// maxima).  Placed before a loop whose first iteration reads loaded values, it
+// settles them there; otherwise the compiler's wait for them sits in the loop
+// and, being a count, also waits on every iteration for memory operations the
+// loop itself issued (returning atomics, stores).
+__device__ __forceinline__ void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 template <typename T>
 struct WalkState {
     int32_t c[2];

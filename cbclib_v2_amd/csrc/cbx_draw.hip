@@ -48,6 +48,7 @@ struct Segment {
     int32_t ncols;  // long-axis columns covered from c[axis] in the walk direction
     T Te, Ne;       // walk errors at that position
     int32_t lo, hi; // short-axis pixel range the fills can reach, clipped to the frame
+    int32_t rank[2]; // unordered plan: places in the item lists of its first two tiles
 };
 
 struct Geometry {
@@ -131,6 +132,7 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
     // every step moves the walk; a walk takes at most one short-axis-only
     // step per long-axis step, so this budget is never reached by a valid line
     const int64_t budget = 4 * ((int64_t)abs(pick(L.m1, a) - pick(L.m0, a)) + abs(pick(L.m1, b) - pick(L.m0, b))) + 64;
+    wait_vm_loads();
     for (int64_t it = 0;; it++) {
         if (it == budget) {
             atomicOr(flags, FLAG_STUCK);
@@ -226,7 +228,8 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
 //                   segment bounds (counters[0]);
 //   k_walk_lines    each wave takes slots for its lines' bounds with one
 //                   atomic (counters[1]), walks, writes the segments into
-//                   them (unused slots get ncols = 0) and counts tile items;
+//                   them (unused slots get ncols = 0) and counts tile items
+//                   (the count atomics also give each item its place);
 //   k_bin_slots     one lane per slot: writes the per-tile item lists.
 template <typename T, typename I>
 __global__ void __launch_bounds__(256)
@@ -249,7 +252,8 @@ k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
-             unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt, int *__restrict__ flags)
+             unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt, int *__restrict__ tile_cnt2,
+             int *__restrict__ flags)
 {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -274,32 +278,53 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
     if (ub == 0) return;
     const int64_t base = (int64_t)wbase + pre;
     int nseg = 0;
+    // a segment is stored when the next one closes (or after the walk), so
+    // the walk runs on while its count atomics are in flight
+    Segment<T> pending;
     walk_segments(L, l, g, flags, [&](const Segment<T> &cur, int tl) {
-        if (nseg < ub) segs[base + nseg] = cur;
-        for (int tb = cur.lo / g.tile; tb <= cur.hi / g.tile; tb++)
-            atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb)], 1);
+        if (nseg > 0 && nseg <= ub) segs[base + nseg - 1] = pending;
+        // the counting atomics hand out each piece's place in its tile's list:
+        // the first two tiles of a segment in class 1 (kept in the segment),
+        // further ones in class 2 (placed by k_bin_slots)
+        pending = cur;
+        const int tb0 = cur.lo / g.tile, tb1 = cur.hi / g.tile;
+        // straight-line, fixed destinations: nothing reads the returned
+        // places until the segment is stored
+        pending.rank[0] = atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb0)], 1);
+        if (tb1 > tb0) pending.rank[1] = atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb0 + 1)], 1);
+        for (int tb = tb0 + 2; tb <= tb1; tb++) atomicAdd(&tile_cnt2[tile_index(g, L.frame, L.axis, tl, tb)], 1);
         nseg++;
     });
+    if (nseg > 0 && nseg <= ub) segs[base + nseg - 1] = pending;
     if (nseg > ub) atomicOr(flags, FLAG_STUCK); // cannot happen: the bound holds for every walk
     for (int k = nseg; k < ub; k++) segs[base + k].ncols = 0;
 }
 
+__global__ void k_tile_total(int64_t n, const int *__restrict__ a, const int *__restrict__ b, int *__restrict__ sum)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) sum[i] = a[i] + b[i];
+}
+
+// Tile t's list: class-1 pieces at [0, cnt1[t]) by their ranks, then class 2.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup, const Segment<T> *__restrict__ segs,
-            const int *__restrict__ tile_off, int *__restrict__ tile_cur, int *__restrict__ items)
+            const int *__restrict__ tile_off, const int *__restrict__ cnt1, int *__restrict__ cur2,
+            int *__restrict__ items)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_slots) return;
-    const Segment<T> &sg = segs[i];
+    const Segment<T> sg = segs[i];
     if (sg.ncols <= 0) return;
-    const int lo = sg.lo, hi = sg.hi;
-    const int32_t line = sg.line;
-    const int axis = setup[line].axis, frame = setup[line].frame;
+    const int axis = setup[sg.line].axis, frame = setup[sg.line].frame;
     const int tl = pick(sg.c, axis) / g.tile;
-    for (int tb = lo / g.tile; tb <= hi / g.tile; tb++) {
+    const int tb0 = sg.lo / g.tile, tb1 = sg.hi / g.tile;
+    items[tile_off[tile_index(g, frame, axis, tl, tb0)] + sg.rank[0]] = (int)i;
+    if (tb1 > tb0) items[tile_off[tile_index(g, frame, axis, tl, tb0 + 1)] + sg.rank[1]] = (int)i;
+    for (int tb = tb0 + 2; tb <= tb1; tb++) {
         const int64_t tid = tile_index(g, frame, axis, tl, tb);
-        items[tile_off[tid] + atomicAdd(&tile_cur[tid], 1)] = (int)i;
+        items[tile_off[tid] + cnt1[tid] + atomicAdd(&cur2[tid], 1)] = (int)i;
     }
 }
 
@@ -1197,13 +1222,18 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     P.n_segs = (int64_t)slots;
     if (P.n_segs == 0) return CBX_OK;
     CBX_HIP(P.segs.alloc(P.n_segs, st));
+    DevBuf<int> cnt2, total;
+    CBX_HIP(cnt2.alloc(P.n_tiles + 1, st));
+    CBX_HIP(total.alloc(P.n_tiles + 1, st));
+    CBX_HIP(hipMemsetAsync(cnt2.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
     {
         KernelTimer timer("segments_walk", st);
         k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, counters.ptr,
-                                                                P.tile_cnt.ptr, P.flags.ptr);
+                                                                P.tile_cnt.ptr, cnt2.ptr, P.flags.ptr);
     }
     CBX_HIP(hipGetLastError());
-    CBX_HIP(exclusive_scan_i32(P.tile_cnt.ptr, P.tile_off.ptr, P.n_tiles + 1, st));
+    k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, cnt2.ptr, total.ptr);
+    CBX_HIP(exclusive_scan_i32(total.ptr, P.tile_off.ptr, P.n_tiles + 1, st));
     int n_items = 0;
     CBX_HIP(hipMemcpyAsync(&n_items, P.tile_off.ptr + P.n_tiles, sizeof(int), hipMemcpyDeviceToHost, st));
     CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1215,7 +1245,7 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
     CBX_HIP(hipMemsetAsync(P.tile_cur.ptr, 0, sizeof(int) * P.n_tiles, st));
     k_bin_slots<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr, P.tile_off.ptr,
-                                                            P.tile_cur.ptr, P.items.ptr);
+                                                            P.tile_cnt.ptr, P.tile_cur.ptr, P.items.ptr);
     CBX_HIP(hipGetLastError());
     return CBX_OK;
 }
