@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <type_traits>
@@ -311,7 +312,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup, const Segment<T> *__restrict__ segs,
             const int *__restrict__ tile_off, const int *__restrict__ cnt1, int *__restrict__ cur2,
-            int *__restrict__ items)
+            int2 *__restrict__ pairs)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_slots) return;
@@ -320,11 +321,12 @@ k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup,
     const int axis = setup[sg.line].axis, frame = setup[sg.line].frame;
     const int tl = pick(sg.c, axis) / g.tile;
     const int tb0 = sg.lo / g.tile, tb1 = sg.hi / g.tile;
-    items[tile_off[tile_index(g, frame, axis, tl, tb0)] + sg.rank[0]] = (int)i;
-    if (tb1 > tb0) items[tile_off[tile_index(g, frame, axis, tl, tb0 + 1)] + sg.rank[1]] = (int)i;
+    const int2 item = make_int2((int)i, sg.line);
+    pairs[tile_off[tile_index(g, frame, axis, tl, tb0)] + sg.rank[0]] = item;
+    if (tb1 > tb0) pairs[tile_off[tile_index(g, frame, axis, tl, tb0 + 1)] + sg.rank[1]] = item;
     for (int tb = tb0 + 2; tb <= tb1; tb++) {
         const int64_t tid = tile_index(g, frame, axis, tl, tb);
-        items[tile_off[tid] + cnt1[tid] + atomicAdd(&cur2[tid], 1)] = (int)i;
+        pairs[tile_off[tid] + cnt1[tid] + atomicAdd(&cur2[tid], 1)] = item;
     }
 }
 
@@ -407,9 +409,15 @@ __device__ __forceinline__ void fill_column(const LineSetup<T> &L, const ColRec<
     }
 }
 
+// LDS accumulator of the tile kernel, by accumulation mode OV:
+//   0  sums in double (on gfx950 ds_add_f64 sustains ~20x the lane rate of
+//      ds_add_f32; the tile is rounded to the image type once, at the flush)
+//   1  max overlap, as order-preserving unsigned codes (ds_max)
+//   2  float sums in 32-bit fixed point (ds_add_u32, the fastest LDS atomic,
+//      at half the LDS of double; see k_tiles)
 template <typename T, int OV>
 struct AccType {
-    using type = T;
+    using type = double;
 };
 template <>
 struct AccType<float, 1> {
@@ -419,16 +427,37 @@ template <>
 struct AccType<double, 1> {
     using type = unsigned long long;
 };
+template <>
+struct AccType<float, 2> {
+    using type = uint32_t;
+};
+
+// Adds one pixel value to the LDS tile.  For OV = 2, v is the profile value
+// already scaled by the round's 2^F; exact zeros only mark the pixel touched.
+template <int OV, typename Acc, typename V>
+__device__ __forceinline__ void tile_put(Acc *img, uint32_t *zb, int idx, V v)
+{
+    if constexpr (OV == 0) {
+        atomicAdd(&img[idx], (Acc)v);
+    } else if constexpr (OV == 1) {
+        if (v == v) atomicMax(&img[idx], ord_enc(v));
+    } else {
+        if (v > 0.0f) atomicAdd(&img[idx], max(1u, __float2uint_rn((float)v)));
+        else atomicOr(&zb[idx >> 5], 1u << (idx & 31));
+    }
+}
 
 // ---------------------------------------------------------------------------
 // 4. tile kernel (image mode)
 // ---------------------------------------------------------------------------
 // The fills of one column half for the float fast path: constants held in
 // registers, the tile window turned into a trip count, no division guard
-// (only for lines with L.safe).  Accumulates into the LDS tile.
+// (only for lines with L.safe).  Accumulates into the LDS tile; vscale is
+// max_val (OV 0, 1) or the fixed-point scale 2^F (OV 2).
 template <int K, int OV, typename Acc>
-__device__ __forceinline__ void fill_half_fast(Acc *img, const LineSetup<float> &L, int cs, float Te, float Ne,
-                                               int half, int cbase, int sstride, int s_lo, int s_hi, float max_val)
+__device__ __forceinline__ void fill_half_fast(Acc *img, uint32_t *zb, const LineSetup<float> &L, int cs, float Te,
+                                               float Ne, int half, int cbase, int sstride, int s_lo, int s_hi,
+                                               float vscale)
 {
     const float len = L.len, rlen = L.rlen, LL = L.LL, w2 = L.w2, rw2f = L.rw2f;
     const int b = 1 - L.axis;
@@ -443,15 +472,7 @@ __device__ __forceinline__ void fill_half_fast(Acc *img, const LineSetup<float> 
         const float qt = at * rlen;
         float e = __builtin_fmaf(qt, qt, nsq) * rw2f;
         if (__builtin_fabsf(e - 1.0f) <= 4e-6f) e = err_exact(L, at, nsq);
-        if (e <= 1.0f) {
-            const float v = max_val * profile_fast<K>(e);
-            Acc *px = &img[cbase + (s - s_lo) * sstride];
-            if constexpr (OV == 0) {
-                atomicAdd(px, v);
-            } else {
-                if (v == v) atomicMax(px, ord_enc(v));
-            }
-        }
+        if (e <= 1.0f) tile_put<OV>(img, zb, cbase + (s - s_lo) * sstride, vscale * profile_fast<K>(e));
     };
     if (half == 0 && cs >= s_lo && cs < s_hi) {
         const float q = div_rn_nz(Ne, len, rlen);
@@ -471,18 +492,42 @@ __device__ __forceinline__ void fill_half_fast(Acc *img, const LineSetup<float> 
     }
 }
 
-// Tile kernel.  The workgroup takes its tile's items in chunks of CH <= NT:
-// phase 1 gives each of CH threads one item (the walk replay, column records
-// into LDS); phase 2 gives each thread one (item, column, half) unit of the
-// fills at a time.
+// Value a fixed-point tile pixel adds to the image: acc * max_val * 2^-F, or
+// max_val * 0 (the sign rules of adding exact zeros) if only zeros touched
+// it.  has = false: the pixel was not touched.
+__device__ __forceinline__ double fix_value(uint32_t acc, const uint32_t *zb, int i, double scale, float max_val,
+                                            bool &has)
+{
+    has = acc != 0u || ((zb[i >> 5] >> (i & 31)) & 1u);
+    return acc ? (double)acc * scale : (double)(max_val * 0.0f);
+}
+
+// Tile kernel.  The workgroup takes its tile's items ((segment, line) pairs)
+// in chunks of CH <= NT: phase 1 gives each of CH threads one item (the walk
+// replay, column records into LDS); phase 2 gives each thread one (item,
+// column, half) unit of the fills at a time.  The first chunk's loads and
+// (256-thread float sum tiles) the tile's current values in `out` are
+// requested before the LDS tile is cleared, so their latency overlaps.
+//
+// Fixed-point sums (OV = 2): an item (one segment in one tile) adds at most
+// one value per pixel, so a round of n items adds at most n values of at most
+// 2^F with F = clz(n): the sum stays below 2^32.  Each value p > 0 adds
+// max(1, rn(p 2^F)) (absolute error <= 2^-F max_val per term, so a touched
+// pixel stays non-zero); exact zeros set a bit in zb.  Tiles of more than
+// FIX_ROUND items take several rounds, folded into the running pixel values.
+constexpr int FIX_ROUND = 4095;
+
+// (at least 5 waves per SIMD: the fastest register budget measured)
 template <typename T, int K, int OV, int TILE, int SEG, int NT, int CH, bool FAST>
-__global__ void __launch_bounds__(NT)
-k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const int *__restrict__ items,
+__global__ void __launch_bounds__(NT, 5)
+k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const int2 *__restrict__ pairs,
         const Segment<T> *__restrict__ segs, const LineSetup<T> *__restrict__ setup, T max_val,
         int *__restrict__ flags)
 {
     using Acc = typename AccType<T, OV>::type;
+    static_assert(OV != 2 || (FAST && sizeof(T) == 4), "fixed-point sums are the float fast path");
     __shared__ Acc img[TILE * TILE];
+    __shared__ uint32_t zb[OV == 2 ? TILE * TILE / 32 : 1];
     static_assert(CH <= NT, "one phase-1 thread per item");
     __shared__ int32_t rs[CH * SEG];
     __shared__ T rT[CH * SEG];
@@ -500,122 +545,220 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     const int X0 = tx * TILE, Y0 = ty * TILE;
     const int X1 = min(X0 + TILE, g.W), Y1 = min(Y0 + TILE, g.H);
     const int tid = threadIdx.x;
+    T *fo = out + (f * g.H) * (int64_t)g.W;
+    constexpr int V = 4;
+    const bool full = (X1 - X0 == TILE) && ((g.W & (V - 1)) == 0) && (((uintptr_t)out & 15) == 0);
 
-    for (int i = tid; i < TILE * TILE; i += NT) {
-        if constexpr (OV == 0) img[i] = T(-0.0); // additive identity, keeps -0.0 pixels intact
-        else img[i] = Acc(0);                     // below every encoded value
+    // first chunk's item loads
+    Segment<T> sg0;
+    LineSetup<T> L0;
+    if (tid < min(CH, end - beg)) {
+        const int2 it = pairs[beg + tid];
+        sg0 = segs[it.x];
+        L0 = setup[it.y];
+    }
+    // the tile's current values, for the flush (float sums, 256 threads:
+    // 4 float4 per thread)
+    constexpr bool PREF = (OV != 1) && sizeof(T) == 4 && NT == 256 && TILE == 64;
+    constexpr int PV = PREF ? (TILE * TILE) / (NT * V) : 1;
+    float4 pre[PV];
+    if constexpr (PREF) {
+        if (full) {
+#pragma unroll
+            for (int j = 0; j < PV; j++) {
+                const int i = (tid + j * NT) * V;
+                const int y = Y0 + i / TILE, x = X0 + (i % TILE);
+                if (y < Y1) pre[j] = *reinterpret_cast<const float4 *>(fo + (int64_t)y * g.W + x);
+            }
+        }
     }
 
-    for (int first = beg; first < end; first += CH) {
-        const int n = min(CH, end - first);
-        __syncthreads();
-        // phase 1: replay the walk of one item from its checkpoint and record
-        // the centre and errors of every emitted column
-        if (tid < n) {
-            const Segment<T> sg = segs[items[first + tid]];
-            const LineSetup<T> L = setup[sg.line];
-            const int a = L.axis, sa = pick(L.st, a);
-            hline[tid] = sg.line;
-            const int c0 = pick(sg.c, a);
-            hc0[tid] = c0;
-            int32_t *ps = &rs[tid * SEG];
-            T *pT = &rT[tid * SEG];
-            T *pN = &rN[tid * SEG];
-#pragma unroll
-            for (int k = 0; k < SEG; k++) ps[k] = INT_MIN;
-            WalkState<T> s;
-            s.c[0] = sg.c[0];
-            s.c[1] = sg.c[1];
-            s.Te = sg.Te;
-            s.Ne = sg.Ne;
-            int it = 0;
-            for (; it < 4 * SEG + 16; it++) {
-                if (walk_done(L, s)) break;
-                const int k = (pick(s.c, a) - c0) * sa;
-                if (k >= sg.ncols) break;
-                bool nx, ny;
-                walk_next(L, s.Ne, nx, ny);
-                if ((a == 0) ? nx : ny) {
-                    ps[k] = pick(s.c, 1 - a);
-                    pT[k] = s.Te;
-                    pN[k] = s.Ne;
-                }
-                walk_step(L, s, nx, ny);
-            }
-            if (it == 4 * SEG + 16) atomicOr(flags, FLAG_STUCK);
+    auto clear = [&]() {
+        for (int i = tid; i < TILE * TILE; i += NT) {
+            if constexpr (OV == 0) img[i] = Acc(-0.0); // additive identity, keeps -0.0 pixels intact
+            else img[i] = Acc(0);                      // below every max code / empty fixed-point sum
         }
-        __syncthreads();
-        // phase 2: one thread per (item, column, half); the centre goes with
-        // the half that steps in +st[b] (bresenham.hpp:364-380)
-        for (int u = tid; u < n * SEG * 2; u += NT) {
-            const int i = u / (2 * SEG), rem = u - i * (2 * SEG);
-            const int k = rem >> 1, half = rem & 1;
-            const int cs = rs[i * SEG + k];
-            if (cs == INT_MIN) continue;
-            const LineSetup<T> L = setup[hline[i]];
-            const int a = L.axis, b = 1 - a;
-            const int c = hc0[i] + k * pick(L.st, a);
-            const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
-            // LDS stride of the short axis and offset of the long-axis column
-            const int sstride = (a == 0) ? TILE : 1;
-            const int cbase = (a == 0) ? (c - X0) : (c - Y0) * TILE;
-            T Te = rT[i * SEG + k], Ne = rN[i * SEG + k];
-            if constexpr (FAST && sizeof(T) == 4) {
-                if (L.safe) {
-                    fill_half_fast<K, OV>(img, L, cs, Te, Ne, half, cbase, sstride, s_lo, s_hi, max_val);
-                    continue;
-                }
-            }
-            auto put = [&](int s, T Te_, T nsq) {
-                T v;
-                if (!pixel_eval<FAST, K>(L, Te_, nsq, max_val, v)) return;
-                Acc *px = &img[cbase + (s - s_lo) * sstride];
-                if constexpr (OV == 0) {
-                    atomicAdd(px, v);
+        if constexpr (OV == 2)
+            for (int i = tid; i < TILE * TILE / 32; i += NT) zb[i] = 0u;
+    };
+    clear();
+
+    // rounds [rb, re): one for OV 0 / 1, FIX_ROUND items each for OV 2
+    for (int rb = beg, re; rb < end; rb = re) {
+        re = (OV == 2) ? rb + min(FIX_ROUND, end - rb) : end;
+        const int F = (OV == 2) ? __clz(re - rb) : 0;
+        // value scale: max_val, or 2^F for fixed-point sums
+        const T vscale = (OV == 2) ? (T)__int_as_float((127 + F) << 23) : max_val;
+        if (rb != beg) {
+            __syncthreads();
+            clear();
+        }
+        for (int first = rb; first < re; first += CH) {
+            const int n = min(CH, re - first);
+            __syncthreads();
+            // phase 1: replay the walk of one item from its checkpoint and
+            // record the centre and errors of every emitted column
+            if (tid < n) {
+                Segment<T> sg;
+                LineSetup<T> L;
+                if (first == beg) {
+                    sg = sg0;
+                    L = L0;
                 } else {
-                    if (v == v) atomicMax(px, ord_enc(v));
+                    const int2 it = pairs[first + tid];
+                    sg = segs[it.x];
+                    L = setup[it.y];
                 }
-            };
-            if (half == 0 && cs >= s_lo && cs < s_hi) put(cs, Te, nsq_of(L, Ne));
-            const int sb = pick(L.st, b);
-            const int dir = half ? -sb : sb;
-            const T tb = pick(L.t, b), nb = pick(L.nv, b);
-            const T dT = (dir > 0) ? tb : -tb, dN = (dir > 0) ? nb : -nb;
-            int s = cs;
-            for (;;) {
-                s += dir;
-                Te += dT;
-                Ne += dN;
-                const T nsq = nsq_of(L, Ne);
-                if (!(nsq < L.w2)) break;
-                if (s >= s_lo && s < s_hi) put(s, Te, nsq);
-                else if ((dir > 0) ? (s >= s_hi) : (s < s_lo)) break;
+                const int a = L.axis, sa = pick(L.st, a);
+                hline[tid] = sg.line;
+                const int c0 = pick(sg.c, a);
+                hc0[tid] = c0;
+                int32_t *ps = &rs[tid * SEG];
+                T *pT = &rT[tid * SEG];
+                T *pN = &rN[tid * SEG];
+#pragma unroll
+                for (int k = 0; k < SEG; k++) ps[k] = INT_MIN;
+                WalkState<T> s;
+                s.c[0] = sg.c[0];
+                s.c[1] = sg.c[1];
+                s.Te = sg.Te;
+                s.Ne = sg.Ne;
+                int it = 0;
+                for (; it < 4 * SEG + 16; it++) {
+                    if (walk_done(L, s)) break;
+                    const int k = (pick(s.c, a) - c0) * sa;
+                    if (k >= sg.ncols) break;
+                    bool nx, ny;
+                    walk_next(L, s.Ne, nx, ny);
+                    if ((a == 0) ? nx : ny) {
+                        ps[k] = pick(s.c, 1 - a);
+                        pT[k] = s.Te;
+                        pN[k] = s.Ne;
+                    }
+                    walk_step(L, s, nx, ny);
+                }
+                if (it == 4 * SEG + 16) atomicOr(flags, FLAG_STUCK);
+            }
+            __syncthreads();
+            // phase 2: one thread per (item, column, half); the centre goes
+            // with the half that steps in +st[b] (bresenham.hpp:364-380)
+            for (int u = tid; u < n * SEG * 2; u += NT) {
+                const int i = u / (2 * SEG), rem = u - i * (2 * SEG);
+                const int k = rem >> 1, half = rem & 1;
+                const int cs = rs[i * SEG + k];
+                if (cs == INT_MIN) continue;
+                const LineSetup<T> L = setup[hline[i]];
+                const int a = L.axis, b = 1 - a;
+                const int c = hc0[i] + k * pick(L.st, a);
+                const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
+                // LDS stride of the short axis and offset of the long-axis column
+                const int sstride = (a == 0) ? TILE : 1;
+                const int cbase = (a == 0) ? (c - X0) : (c - Y0) * TILE;
+                T Te = rT[i * SEG + k], Ne = rN[i * SEG + k];
+                if constexpr (FAST && sizeof(T) == 4) {
+                    if (L.safe) {
+                        fill_half_fast<K, OV>(img, zb, L, cs, Te, Ne, half, cbase, sstride, s_lo, s_hi, (float)vscale);
+                        continue;
+                    }
+                }
+                auto put = [&](int s, T Te_, T nsq) {
+                    T v;
+                    if (!pixel_eval<FAST, K>(L, Te_, nsq, vscale, v)) return;
+                    tile_put<OV>(img, zb, cbase + (s - s_lo) * sstride, v);
+                };
+                if (half == 0 && cs >= s_lo && cs < s_hi) put(cs, Te, nsq_of(L, Ne));
+                const int sb = pick(L.st, b);
+                const int dir = half ? -sb : sb;
+                const T tb = pick(L.t, b), nb = pick(L.nv, b);
+                const T dT = (dir > 0) ? tb : -tb, dN = (dir > 0) ? nb : -nb;
+                int s = cs;
+                for (;;) {
+                    s += dir;
+                    Te += dT;
+                    Ne += dN;
+                    const T nsq = nsq_of(L, Ne);
+                    if (!(nsq < L.w2)) break;
+                    if (s >= s_lo && s < s_hi) put(s, Te, nsq);
+                    else if ((dir > 0) ? (s >= s_hi) : (s < s_lo)) break;
+                }
             }
         }
+        if constexpr (OV == 2) {
+            // fold the round into the pixel values (registers or `out`)
+            __syncthreads();
+            const double scale = ldexp((double)max_val, -F);
+            bool has;
+            if (PREF && full) {
+#pragma unroll
+                for (int j = 0; j < PV; j++) {
+                    const int i = (tid + j * NT) * V;
+                    const float mv = (float)max_val;
+                    double add = fix_value(img[i], zb, i, scale, mv, has);
+                    if (has) pre[j].x = (float)((double)pre[j].x + add);
+                    add = fix_value(img[i + 1], zb, i + 1, scale, mv, has);
+                    if (has) pre[j].y = (float)((double)pre[j].y + add);
+                    add = fix_value(img[i + 2], zb, i + 2, scale, mv, has);
+                    if (has) pre[j].z = (float)((double)pre[j].z + add);
+                    add = fix_value(img[i + 3], zb, i + 3, scale, mv, has);
+                    if (has) pre[j].w = (float)((double)pre[j].w + add);
+                }
+            } else {
+                for (int i = tid; i < TILE * TILE; i += NT) {
+                    const int y = Y0 + i / TILE, x = X0 + (i % TILE);
+                    if (y >= Y1 || x >= X1) continue;
+                    const double add = fix_value(img[i], zb, i, scale, (float)max_val, has);
+                    T *o = fo + (int64_t)y * g.W + x;
+                    if (has) *o = (T)((double)*o + add);
+                }
+            }
+        }
+    }
+    if constexpr (OV == 2) {
+        if (PREF && full) {
+#pragma unroll
+            for (int j = 0; j < PV; j++) {
+                const int i = (tid + j * NT) * V;
+                const int y = Y0 + i / TILE, x = X0 + (i % TILE);
+                if (y < Y1) *reinterpret_cast<float4 *>(fo + (int64_t)y * g.W + x) = pre[j];
+            }
+        }
+        return;
     }
     __syncthreads();
 
     // flush: one coalesced read-modify-write of the tile, 4 pixels per access
     // where the tile row lies inside the frame
-    T *fo = out + (f * g.H) * (int64_t)g.W;
-    constexpr int V = 4;
-    const bool full = (X1 - X0 == TILE) && ((g.W & (V - 1)) == 0) && (((uintptr_t)out & 15) == 0);
+    if constexpr (PREF) {
+        if (full) {
+#pragma unroll
+            for (int j = 0; j < PV; j++) {
+                const int i = (tid + j * NT) * V;
+                const int y = Y0 + i / TILE, x = X0 + (i % TILE);
+                if (y >= Y1) continue;
+                float4 ov = pre[j];
+                ov.x = (float)((double)ov.x + img[i]);
+                ov.y = (float)((double)ov.y + img[i + 1]);
+                ov.z = (float)((double)ov.z + img[i + 2]);
+                ov.w = (float)((double)ov.w + img[i + 3]);
+                *reinterpret_cast<float4 *>(fo + (int64_t)y * g.W + x) = ov;
+            }
+            return;
+        }
+    }
     for (int i = tid * V; i < TILE * TILE; i += NT * V) {
         const int y = Y0 + i / TILE, x = X0 + (i % TILE);
         if (y >= Y1) continue;
         T *o = fo + (int64_t)y * g.W + x;
         if (full) {
             if constexpr (OV == 0) {
-                using Vec = typename std::conditional<sizeof(T) == 4, float4, double2>::type;
                 if constexpr (sizeof(T) == 4) {
                     float4 ov = *reinterpret_cast<float4 *>(o);
-                    ov.x = ov.x + img[i];
-                    ov.y = ov.y + img[i + 1];
-                    ov.z = ov.z + img[i + 2];
-                    ov.w = ov.w + img[i + 3];
+                    ov.x = (float)((double)ov.x + img[i]);
+                    ov.y = (float)((double)ov.y + img[i + 1]);
+                    ov.z = (float)((double)ov.z + img[i + 2]);
+                    ov.w = (float)((double)ov.w + img[i + 3]);
                     *reinterpret_cast<float4 *>(o) = ov;
                 } else {
-                    (void)sizeof(Vec);
                     for (int j = 0; j < V; j++) o[j] = o[j] + img[i + j];
                 }
             } else {
@@ -632,7 +775,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
             for (int j = 0; j < V; j++) {
                 if (x + j >= X1) break;
                 if constexpr (OV == 0) {
-                    o[j] = o[j] + img[i + j];
+                    o[j] = (T)((double)o[j] + img[i + j]);
                 } else {
                     const Acc e = img[i + j];
                     if (e != Acc(0)) {
@@ -1078,6 +1221,7 @@ struct SegmentPlan {
     int64_t n_tiles = 0;
     DevBuf<LineSetup<T>> setup;
     DevBuf<int> seg_cnt, seg_off, tile_cnt, tile_off, tile_cur, items, flags;
+    DevBuf<int2> pairs; // unordered plan: per-tile (segment, line) lists
     DevBuf<Segment<T>> segs;
 };
 
@@ -1241,11 +1385,11 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     if ((rc = flags_to_status(fl))) return rc;
     if (n_items < 0) return fail(CBX_ERUNTIME, "tile item count overflow");
     P.n_items = n_items;
-    CBX_HIP(P.items.alloc(P.n_items, st));
+    CBX_HIP(P.pairs.alloc(P.n_items, st));
     CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
     CBX_HIP(hipMemsetAsync(P.tile_cur.ptr, 0, sizeof(int) * P.n_tiles, st));
     k_bin_slots<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr, P.tile_off.ptr,
-                                                            P.tile_cnt.ptr, P.tile_cur.ptr, P.items.ptr);
+                                                            P.tile_cnt.ptr, P.tile_cur.ptr, P.pairs.ptr);
     CBX_HIP(hipGetLastError());
     return CBX_OK;
 }
@@ -1271,8 +1415,13 @@ static void launch_tiles(T *out, const SegmentPlan<T> &P, T max_val, int threads
 {
 #define CBX_TILES(NT, CH, FST)                                                                                  \
     k_tiles<T, K, OV, TILE, SEG, NT, CH, FST><<<(unsigned)P.n_tiles, NT, 0, st>>>(                             \
-        out, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr)
-    if (fast) {
+        out, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr)
+    if constexpr (OV == 2) {
+        if (threads == 64) CBX_TILES(64, 64, true);
+        else if (threads == 128) CBX_TILES(128, 64, true);
+        else if (chunk == 64) CBX_TILES(256, 64, true);
+        else CBX_TILES(256, 256, true);
+    } else if (fast) {
         if (threads == 64) CBX_TILES(64, 64, true);
         else if (threads == 128) CBX_TILES(128, 64, true);
         else if (chunk == 64) CBX_TILES(256, 64, true);
@@ -1482,6 +1631,15 @@ static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const
         const int threads = (opts && opts->threads) ? opts->threads : 256;
         const bool fast = !(opts && opts->exact_values);
         const int chunk = (opts && opts->chunk) ? opts->chunk : 64;
+        // float sums on the fast path accumulate in fixed point, unless
+        // max_val is not finite (inf * 0 must stay NaN, as in the reference)
+        if constexpr (sizeof(T) == 4) {
+            if (overlap == 0 && fast && std::isfinite(max_val)) {
+                launch_tiles_ov<T, 2>(kernel, out, P, max_val, threads, chunk, fast, st);
+                CBX_HIP(hipGetLastError());
+                return CBX_OK;
+            }
+        }
         if (overlap == 0) launch_tiles_ov<T, 0>(kernel, out, P, max_val, threads, chunk, fast, st);
         else launch_tiles_ov<T, 1>(kernel, out, P, max_val, threads, chunk, fast, st);
     }
