@@ -37,6 +37,17 @@ enum { K_BIWEIGHT = 0, K_GAUSSIAN = 1, K_PARABOLIC = 2, K_RECTANGULAR = 3, K_TRI
  * runs for as long as the segment is. */
 #define CBXO_MAX_WALK (1L << 26)
 
+/* Planted mistakes, for the tests that show each pin catches a plausible
+ * error in this file (0 = none, the default; tests only):
+ *   1 drop the tangent term of the pixel error
+ *   2 wrong sign of one normal component
+ *   3 transposed operands (x and y of the end points swapped)
+ *   4 label ignores neighbour shifts of L1 norm > 1 (the connectivity norm
+ *     confused)
+ *   5 profile evaluated at the error instead of its square root */
+static int g_plant = 0;
+void cbxo_plant(int mistake) { g_plant = mistake; }
+
 /* ------------------------------------------------------------------------ */
 /* Profiles: include.hpp:446-468 (kernel_traits).  Float versions reproduce  */
 /* the promotions of the C++ expressions (std::pow(float, int) -> double,    */
@@ -131,7 +142,7 @@ static float err_f32(float T, float N, float len, float w)
     float a = -T, b = T - len * len;
     float m = (a < b) ? b : a;          /* std::max(a, b) */
     float at = (m < 0.0f) ? 0.0f : m;   /* std::max(., T()) */
-    float qt = at / len;
+    float qt = (g_plant == 1) ? 0.0f : at / len;
     float qn = N / len;
     float nsq = (float)((double)qn * (double)qn);
     double e = (double)qt * (double)qt + (double)nsq;
@@ -150,7 +161,7 @@ static double err_f64(double T, double N, double len, double w)
     double a = -T, b = T - len * len;
     double m = (a < b) ? b : a;
     double at = (m < 0.0) ? 0.0 : m;
-    double qt = at / len;
+    double qt = (g_plant == 1) ? 0.0 : at / len;
     double qn = N / len;
     double e = qt * qt + qn * qn;
     return e / (w * w);
@@ -164,10 +175,11 @@ static double nerr_f64(double N, double len)
 
 #define WALK_BODY(REAL, ROUND, CEIL, FABS, SQRT, ERRF, NERRF, EMIT_T)                           \
     REAL x0 = ln[0], y0 = ln[1], x1 = ln[2], y1 = ln[3], w = ln[4];                             \
+    if (g_plant == 3) { REAL s_ = x0; x0 = y0; y0 = s_; s_ = x1; x1 = y1; y1 = s_; }            \
     REAL p0[2] = {x0, y0}, p1[2] = {x1, y1};                                                    \
     /* tangent (geometry.hpp:316) and normal (bresenham.hpp:310-316) */                          \
     REAL t[2] = {x1 - x0, y1 - y0};                                                             \
-    REAL nv[2] = {y1 - y0, x0 - x1};                                                            \
+    REAL nv[2] = {y1 - y0, (g_plant == 2) ? (x1 - x0) : (x0 - x1)};                             \
     if (!(w > (REAL)0)) return CBXO_OK; /* error() == inf for w <= 0; NaN never emits */        \
     REAL mag = (REAL)0; mag += t[0] * t[0]; mag += t[1] * t[1];                                 \
     REAL len = SQRT(mag);                                                                       \
@@ -281,7 +293,7 @@ static void draw_px_f32(void *vctx, long x, long y, float err)
 {
     draw_ctx *c = (draw_ctx *)vctx;
     if (!(err <= 1.0f) || y < 0 || y >= c->H || x < 0 || x >= c->W) return;
-    float v = (float)c->max_val * prof_f32(c->kernel, sqrtf(err));
+    float v = (float)c->max_val * prof_f32(c->kernel, (g_plant == 5) ? err : sqrtf(err));
     float *o = (float *)c->out + (c->frame * c->H + y) * c->W + x;
     if (c->overlap) *o = (*o < v) ? v : *o;   /* std::max(out, value) */
     else *o = *o + v;
@@ -291,7 +303,7 @@ static void draw_px_f64(void *vctx, long x, long y, double err)
 {
     draw_ctx *c = (draw_ctx *)vctx;
     if (!(err <= 1.0) || y < 0 || y >= c->H || x < 0 || x >= c->W) return;
-    double v = c->max_val * prof_f64(c->kernel, sqrt(err));
+    double v = c->max_val * prof_f64(c->kernel, (g_plant == 5) ? err : sqrt(err));
     double *o = (double *)c->out + (c->frame * c->H + y) * c->W + x;
     if (c->overlap) *o = (*o < v) ? v : *o;
     else *o = *o + v;
@@ -342,7 +354,7 @@ static void list_px_f32(void *vctx, long x, long y, float err)
     if (k >= c->cap) return;
     c->pl_line[k] = (int32_t)c->line; c->pl_frame[k] = (int32_t)c->frame;
     c->pl_y[k] = (int32_t)y; c->pl_x[k] = (int32_t)x;
-    ((float *)c->pl_val)[k] = (float)c->max_val * prof_f32(c->kernel, sqrtf(err));
+    ((float *)c->pl_val)[k] = (float)c->max_val * prof_f32(c->kernel, (g_plant == 5) ? err : sqrtf(err));
     if (c->pl_err) ((float *)c->pl_err)[k] = err;
 }
 
@@ -354,7 +366,7 @@ static void list_px_f64(void *vctx, long x, long y, double err)
     if (k >= c->cap) return;
     c->pl_line[k] = (int32_t)c->line; c->pl_frame[k] = (int32_t)c->frame;
     c->pl_y[k] = (int32_t)y; c->pl_x[k] = (int32_t)x;
-    ((double *)c->pl_val)[k] = c->max_val * prof_f64(c->kernel, sqrt(err));
+    ((double *)c->pl_val)[k] = c->max_val * prof_f64(c->kernel, (g_plant == 5) ? err : sqrt(err));
     if (c->pl_err) ((double *)c->pl_err)[k] = err;
 }
 
@@ -565,6 +577,11 @@ long cbxo_label(const void *inp, int dtype, int ndim, const int64_t *shape, cons
         long vi = val_at(inp, dtype, i);
         for (long s = 0; s < n_shifts; s++) {
             const int64_t *sh = shifts + s * ndim;
+            if (g_plant == 4) {
+                long l1 = 0;
+                for (int d = 0; d < ndim; d++) l1 += labs((long)sh[d]);
+                if (l1 > 1) continue;
+            }
             long k = 0, ok = 1;
             for (int d = 0; d < ndim; d++) {
                 long c = coord[d] + (long)sh[d];

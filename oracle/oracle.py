@@ -58,6 +58,8 @@ def lib() -> ctypes.CDLL:
         _lib.cbxo_adjoint_f64.argtypes = [P, L, L, L, P, P, L, P, D, I, P]
         _lib.cbxo_label.argtypes = [P, I, I, P, P, L, L, P]
         _lib.cbxo_label.restype = L
+        _lib.cbxo_plant.argtypes = [I]
+        _lib.cbxo_plant.restype = None
         _lib.cbxo_profile_f64.argtypes = [I, D]
         _lib.cbxo_profile_f64.restype = D
         _lib.cbxo_profile_f32.argtypes = [I, F]
@@ -183,6 +185,22 @@ def label(inp: np.ndarray, shifts: np.ndarray, npts: int = 1) -> Tuple[np.ndarra
     if n < 0:
         _raise(-n, "oracle label failed")
     return out, int(n)
+
+
+class planted:
+    """Context manager that plants mistake ``m`` in the oracle (tests only; see
+    cbxo_plant in cbx_oracle.c for the list)."""
+
+    def __init__(self, m: int):
+        self.m = int(m)
+
+    def __enter__(self):
+        lib().cbxo_plant(self.m)
+        return self
+
+    def __exit__(self, *exc):
+        lib().cbxo_plant(0)
+        return False
 
 
 def profile(kernel: str, x: float, dtype=np.float64) -> float:

@@ -1,0 +1,99 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures of this directory from the reference's own CPU
+code: ``bresenham.draw_lines`` (cbclib_v2/_src/src/bresenham.cpp:142-149,
+329-330) and ``label.label`` + ``LabelResult.to_array`` (label.cpp:109-175,
+459-493, 530-532), compiled from the reference sources by ``oracle/Makefile``
+into ``oracle/_ref``.  The reference ships no golden vectors of its own, so
+these fixtures are what pins the oracle where the compiled reference is not
+available (the oracle's CPU tests compare against them bit for bit).
+
+Inputs are small and seeded; outputs are stored as computed (single thread,
+so sums are in ascending line order).  Run from the repository root:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from oracle import oracle as O  # noqa: E402
+
+KERNELS = ['biweight', 'gaussian', 'parabolic', 'rectangular', 'triangular']
+
+
+def draw_case(rng, dtype):
+    """Two 40x56 frames; segments of every kind the walk distinguishes:
+    random, half-integer end points (ties of the rounding), axis-aligned,
+    leaving the frame, zero-length, non-positive and sub-pixel widths."""
+    F, H, W = 2, 40, 56
+    n = 36
+    x0 = rng.uniform(-8, W + 8, n)
+    y0 = rng.uniform(-8, H + 8, n)
+    th = rng.uniform(0, 2 * np.pi, n)
+    L = rng.uniform(0, 45, n)
+    lines = np.stack([x0, y0, x0 + L * np.cos(th), y0 + L * np.sin(th), rng.uniform(0.3, 6.0, n)], 1)
+    lines[:8, :4] = np.round(lines[:8, :4] * 2) / 2          # half-integer end points
+    lines[8:11, 3] = lines[8:11, 1]                            # horizontal
+    lines[11:14, 2] = lines[11:14, 0]                          # vertical
+    lines[14, 2:4] = lines[14, 0:2]                            # zero length
+    lines[15, 4] = 0.0
+    lines[16, 4] = -1.0
+    idxs = rng.integers(0, F, n)
+    it = np.int64 if dtype == np.float64 else np.int32
+    return (F, H, W), lines.astype(dtype), idxs.astype(it)
+
+
+def main():
+    bres = O.ref_module('bresenham')
+    lab = O.ref_module('label')
+    if bres is None or lab is None:
+        sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources exist")
+    rng = np.random.default_rng(20240601)
+    draw = {}
+    i = 0
+    for dtype in (np.float32, np.float64):
+        for kernel in KERNELS:
+            for overlap in ('sum', 'max'):
+                shape, lines, idxs = draw_case(rng, dtype)
+                init = np.where(rng.random(shape) < 0.05, rng.uniform(-1, 1, shape), 0.0).astype(dtype)
+                out = init.copy()
+                bres.draw_lines(out, lines, idxs, 1.7, kernel, overlap, 1)
+                draw[f'c{i}_lines'] = lines
+                draw[f'c{i}_idxs'] = idxs
+                draw[f'c{i}_init'] = init
+                draw[f'c{i}_out'] = out
+                draw[f'c{i}_meta'] = np.array([KERNELS.index(kernel), 0 if overlap == 'sum' else 1], np.int64)
+                i += 1
+    np.savez_compressed(os.path.join(HERE, 'draw_cases.npz'), **draw)
+
+    label = {}
+    j = 0
+    for shape, radii, conn, dens in [((30, 41), [1, 1], 1, 0.45), ((30, 41), [1, 1], 2, 0.45),
+                                     ((4, 20, 23), [0, 1, 1], 2, 0.5), ((6, 9, 11), [1, 1, 1], 3, 0.35),
+                                     ((25, 25), [2, 2], 2, 0.3), ((64,), [1], 1, 0.5)]:
+        for kind in ('bool', 'int32'):
+            if kind == 'bool':
+                inp = rng.random(shape) < dens
+            else:
+                inp = (rng.random(shape) < dens) * rng.integers(1, 3, shape).astype(np.int32)
+            for npts in (1, 3):
+                st = lab.Structure(radii, conn)
+                res = lab.label(inp, st, npts, 1)
+                n = len(res.regions)
+                out = res.to_array(np.arange(1, n + 1)).astype(np.int32) if n else np.zeros(shape, np.int32)
+                label[f'c{j}_inp'] = inp
+                label[f'c{j}_radii'] = np.array(radii, np.int64)
+                label[f'c{j}_meta'] = np.array([conn, npts, n], np.int64)
+                label[f'c{j}_out'] = out
+                j += 1
+    np.savez_compressed(os.path.join(HERE, 'label_cases.npz'), **label)
+    print(f"wrote {i} draw cases and {j} label cases")
+
+
+if __name__ == '__main__':
+    main()

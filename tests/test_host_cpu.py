@@ -1,0 +1,118 @@
+"""Host logic on the CPU (no GPU calls): device selection, the reference's
+argument checks and exception types, the C ABI's own validation (which runs
+before any device work), the label Structure API, and that the product path
+fails loudly instead of falling back to the CPU."""
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from cbclib_v2_amd import _native, device, label, ndimage
+from cbclib_v2_amd._arrays import shape_of
+
+
+def test_device_selection_and_aliases():
+    assert device.get_device() == 'rocm'
+    with device.context('cuda') as d:
+        assert d == 'rocm'
+    with device.context('cpu'):
+        assert device.get_device() == 'cpu'
+    assert device.get_device() == 'rocm'
+    with pytest.raises(ValueError):
+        device.set_device('tpu')
+
+
+def test_cpu_device_raises_instead_of_falling_back():
+    with device.context('cpu'):
+        with pytest.raises(RuntimeError, match='only on ROCm'):
+            ndimage.draw_lines(np.zeros((8, 8), np.float32), np.zeros((1, 5), np.float32))
+        with pytest.raises(RuntimeError, match='only on ROCm'):
+            label.label(np.zeros((8, 8), bool), label.Structure([1, 1], 1))
+
+
+@pytest.mark.skipif(torch.cuda.is_available(), reason="checks the behaviour without a GPU")
+def test_no_gpu_fails_loudly():
+    """Without a GPU the HIP library reports an error: no silent CPU path."""
+    with pytest.raises(RuntimeError):
+        ndimage.draw_lines(np.zeros((8, 8), np.float32), np.array([[1, 1, 6, 6, 2.0]], np.float32))
+
+
+def test_reference_argument_errors():
+    with pytest.raises(ValueError, match='kernel is missing'):
+        _native.kernel_id('cosine')
+    with pytest.raises(ValueError, match='Invalid overlap keyword'):
+        _native.overlap_id('min')
+    with pytest.raises(RuntimeError, match='Invalid lines size'):
+        ndimage._line_count(np.zeros((3, 6), np.float32))
+    with pytest.raises(NotImplementedError):
+        ndimage._line_count(np.zeros((3, 7), np.float32))
+    assert ndimage._line_count(np.zeros((2, 3, 5), np.float32)) == 6
+    with pytest.raises(ValueError):
+        ndimage._check_idxs(np.zeros(2, np.int32), 3, 3)
+    with pytest.raises(ValueError):
+        ndimage._check_idxs(None, 3, 2)
+    ndimage._check_idxs(None, 3, 1)
+    ndimage._check_idxs(None, 3, 3)
+
+
+def test_status_codes_map_to_reference_exceptions():
+    L = _native.lib()
+    opts = _native.DrawOpts(0, 0, 0, 0, 0, 0)
+    # kernel id out of range -> ValueError, before any device work
+    with pytest.raises(ValueError, match='kernel is missing'):
+        _native.check(L.cbx_draw_lines_f32(None, 1, 16, 16, None, 3, None, 1.0, 9, 0, ctypes.byref(opts), None))
+    with pytest.raises(ValueError, match='overlap'):
+        _native.check(L.cbx_draw_lines_f32(None, 1, 16, 16, None, 3, None, 1.0, 1, 5, ctypes.byref(opts), None))
+    with pytest.raises(ValueError, match='incompatible size'):
+        _native.check(L.cbx_draw_lines_f32(None, 2, 16, 16, None, 3, None, 1.0, 1, 0, None, None))
+    bad = _native.DrawOpts(48, 0, 0, 0, 0, 0)
+    with pytest.raises(ValueError, match='tile must be'):
+        _native.check(L.cbx_draw_lines_f32(None, 1, 16, 16, None, 3, None, 1.0, 1, 0, ctypes.byref(bad), None))
+    n = ctypes.c_int64(0)
+    shp = np.array([4, 4], np.int64)
+    sh = np.zeros((1, 2), np.int64)
+    with pytest.raises(ValueError, match='dtype'):
+        _native.check(L.cbx_label(None, 7, 2, shp.ctypes.data, sh.ctypes.data, 1, 1, None, ctypes.byref(n), None))
+    with pytest.raises(ValueError):
+        _native.check(L.cbx_label(None, 0, 9, shp.ctypes.data, sh.ctypes.data, 1, 1, None, ctypes.byref(n), None))
+    # empty inputs are no-ops
+    _native.check(L.cbx_draw_lines_f32(None, 1, 16, 16, None, 0, None, 1.0, 1, 0, None, None))
+    n.value = 5
+    zero = np.array([0, 4], np.int64)
+    _native.check(L.cbx_label(None, 0, 2, zero.ctypes.data, sh.ctypes.data, 1, 1, None, ctypes.byref(n), None))
+    assert n.value == 0
+
+
+def test_structure_api():
+    st = label.Structure([0, 1, 1], 2)
+    assert st.rank == 3 and st.shape == [1, 3, 3] and len(st) == 9
+    assert st.squeeze().rank == 2 and len(st.squeeze()) == 9
+    ex = label.Structure([1, 1], 1).expand_dims(0)
+    assert ex.shape == [1, 3, 3] and len(ex) == 5
+    with pytest.raises(ValueError):
+        label.Structure([1, 1], 1).to_array(np.zeros((3,), bool))
+    arr = label.Structure([1, 1], 1).to_array()
+    assert arr.tolist() == [[False, True, False], [True, True, True], [False, True, False]]
+    with pytest.raises(ValueError, match='structure rank'):
+        label._shifts_for(label.Structure([1, 1], 1), 3)
+
+
+def test_label_result_regions():
+    lab = np.array([[0, 1, 1], [2, 0, 1], [2, 2, 0]], np.int32)
+    res = label.LabelResult.from_array(lab)
+    assert len(res) == 2
+    assert [list(r) for r in res.regions] == [[1, 2, 5], [3, 6, 7]]
+    assert np.array_equal(res.to_array(np.array([1, 2])), lab)
+
+
+def test_array_helpers():
+    assert shape_of(np.zeros((2, 3))) == (2, 3)
+    assert shape_of(torch.zeros(4, 5)) == (4, 5)
+
+
+def test_workspace_calls_without_gpu():
+    assert _native.workspace_bytes() >= 0
+    assert _native.workspace_release() >= 0
+    ms, n = _native.timing_read('draw_tiles')
+    assert n >= 0 and ms >= 0.0
