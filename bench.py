@@ -26,6 +26,23 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (profiles/traffic.txt: FETCH_SIZE and WRITE_SIZE per dispatch, scaled by
+    the same counters on a kernel of known traffic), or None."""
+    path = os.path.join(ROOT, 'profiles', 'traffic.txt')
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = line.split()
+        if len(parts) >= 4 and parts[0] == kernel:
+            try:
+                return float(parts[1]) + float(parts[2])
+            except ValueError:
+                return None
+    return None
+
+
 def fail_multi(n: int) -> None:
     print(json.dumps({"error": f"--gpus {n}: this build measures one GPU only; the path shards by frame "
                       "(see the sharding paragraph of the design notes) and no multi-GPU run is made"}))
@@ -112,6 +129,9 @@ def main() -> None:
     ap.add_argument('--sweep', action='store_true', help='time the tile kernel over launch shapes and exit')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    ap.add_argument('--calibrate', action='store_true',
+                    help='also run an elementwise kernel of known traffic (256 MiB read + 256 MiB written), the '
+                         'reference for scaling FETCH_SIZE / WRITE_SIZE counters')
     args = ap.parse_args()
     world = int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
@@ -205,7 +225,7 @@ def main() -> None:
                                "Structure([0,1,1],2)); step = zero + draw_lines + label",
                    "frames": F, "segments": N, "tile": args.tile or 64, "seg": args.seg or 8},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic('k_tiles'),
                      "kernel": "k_tiles (draw_lines tile accumulation)",
                      "bytes_per_launch": fwd_bytes, "avg_launch_ms": round(tile_avg, 4), "launches": tiles_n},
         "fwd_Mlines_per_s": round(N / (acc['fwd'] / 2 * 1e-3) / 1e6, 4),
@@ -216,6 +236,12 @@ def main() -> None:
         "label": {"n_labels": n_labels, "mask_density": round(mask_density, 6),
                   "algorithmic_GBs": round(8.0 * F * H * W / (label_ms / max(label_n, 1) * 1e-3) / 1e9, 2)},
     }
+    if args.calibrate:
+        src = torch.ones(64 * 1024 * 1024, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            torch.mul(src, 2.0, out=dst)
+        torch.cuda.synchronize()
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(lines_np, idxs_np, F, H, W, args.cpu_seconds)
     else:
