@@ -117,6 +117,114 @@ def sweep(out, lines, idxs, F, H, W, N):
                     print(json.dumps(rows[-1]), flush=True)
 
 
+def timed(fn, steps, warmup):
+    """Average milliseconds of fn() over `steps` calls after `warmup`, with the
+    device synchronised on both sides of the timed region."""
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        fn()
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def workload_c4(args):
+    """Forward + adjoint: 32 frames of 1024^2, 5000 segments each (C2
+    distributions, seed 3), biweight; loss = sum((render - target)^2) with the
+    target rendered from end points perturbed by N(0, 0.5)."""
+    import numpy as np
+    import torch
+    from cbclib_v2_amd import _native, autograd
+    from tests.lines_gen import random_segments
+    dev = torch.device('cuda', 0)
+    F, H, W, per = 32, 1024, 1024, 5000
+    rng = np.random.default_rng(3)
+    truth = random_segments(rng, F * per, H, W, 20.0, 400.0, 2.0, 6.0)
+    pert = truth.copy()
+    pert[:, :4] += rng.normal(0.0, 0.5, (F * per, 4))
+    idxs = torch.from_numpy(np.repeat(np.arange(F), per).astype(np.int32)).to(dev)
+    lines = torch.tensor(truth, dtype=torch.float32, device=dev, requires_grad=True)
+    target = autograd.render(torch.tensor(pert, dtype=torch.float32, device=dev), (F, H, W), idxs,
+                             kernel='biweight').detach()
+
+    def step():
+        lines.grad = None
+        img = autograd.render(lines, (F, H, W), idxs, kernel='biweight')
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+
+    _native.timing_reset()
+    _native.timing_enable(True)
+    ms = timed(step, args.steps, args.warmup)
+    _native.timing_enable(False)
+    adj_ms, adj_n = _native.timing_read('adjoint_tiles')
+    N = F * per
+    step_bytes = 12.0 * F * H * W + 68.0 * N
+    return {"metric": "fwd+adjoint frames/s (C4)", "value": round(F / (ms * 1e-3), 2), "unit": "frames/s",
+            "ms_per_step": round(ms, 4), "Mlines_per_s": round(N / (ms * 1e-3) / 1e6, 4),
+            "config": {"workload": f"forward + adjoint: {F} frames of {H}x{W} f32, {N} segments ({per}/frame, "
+                                   "C2 distributions, seed 3), biweight, loss sum((render - target)^2)"},
+            "step_GBs": round(step_bytes / (ms * 1e-3) / 1e9, 2), "step_bytes": step_bytes,
+            "adjoint_tiles_ms": round(adj_ms / max(adj_n, 1), 4)}
+
+
+def workload_c5(args):
+    """Refinement: 16 frames of 2048^2, 10000 segments each, biweight; start =
+    truth + N(0, 1) end points, widths x U[0.8, 1.2]; target = Poisson(50 x
+    render) / 50; Adam (lr 0.05) for `--adam-steps` steps."""
+    import numpy as np
+    import torch
+    from cbclib_v2_amd import autograd
+    from tests.lines_gen import random_segments
+    dev = torch.device('cuda', 0)
+    F, H, W, per = 16, 2048, 2048, 10000
+    rng = np.random.default_rng(4)
+    truth = random_segments(rng, F * per, H, W, 20.0, 400.0, 2.0, 6.0)
+    init = truth.copy()
+    init[:, :4] += rng.normal(0.0, 1.0, (F * per, 4))
+    init[:, 4] *= rng.uniform(0.8, 1.2, F * per)
+    idxs = torch.from_numpy(np.repeat(np.arange(F), per).astype(np.int32)).to(dev)
+    t_truth = torch.tensor(truth, dtype=torch.float32, device=dev)
+    clean = autograd.render(t_truth, (F, H, W), idxs, kernel='biweight').detach()
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(4)
+    target = torch.poisson(50.0 * clean, generator=gen) / 50.0
+    lines = torch.nn.Parameter(torch.tensor(init, dtype=torch.float32, device=dev))
+    opt = torch.optim.Adam([lines], lr=0.05, betas=(0.9, 0.999), eps=1e-8)
+
+    def step():
+        opt.zero_grad(set_to_none=True)
+        img = autograd.render(lines, (F, H, W), idxs, kernel='biweight')
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+        opt.step()
+
+    def rmse(ln):
+        a = ln[:, :4].detach().cpu().numpy().astype(np.float64)
+        d1 = ((a - truth[:, :4]) ** 2).sum(1)
+        d2 = ((a[:, [2, 3, 0, 1]] - truth[:, :4]) ** 2).sum(1)
+        return float(np.sqrt(np.minimum(d1, d2).mean() / 2.0))
+
+    r0 = rmse(lines)
+    n = args.adam_steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(n):
+        step()
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t0
+    N = F * per
+    return {"metric": "Adam refinement steps/s (C5)", "value": round(n / sec, 3), "unit": "steps/s",
+            "ms_per_step": round(sec * 1e3 / n, 4), "adam_steps": n,
+            "endpoint_rmse_px": {"start": round(r0, 4), "end": round(rmse(lines), 4)},
+            "config": {"workload": f"refinement: {F} frames of {H}x{W} f32, {N} segments ({per}/frame, seed 4), "
+                                   "biweight, target Poisson(50 render)/50, Adam lr 0.05"},
+            "step_GBs": round((12.0 * F * H * W + 68.0 * N + 140.0 * N) / (sec / n) / 1e9, 2)}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -129,6 +237,10 @@ def main() -> None:
     ap.add_argument('--sweep', action='store_true', help='time the tile kernel over launch shapes and exit')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
+    ap.add_argument('--workload', choices=['c2', 'c3', 'c4', 'c5'], default='c2',
+                    help='c2 (default, the headline line); c3 batched uneven forward; c4 forward + adjoint; '
+                         'c5 Adam refinement')
+    ap.add_argument('--adam-steps', type=int, default=200)
     ap.add_argument('--calibrate', action='store_true',
                     help='also run an elementwise kernel of known traffic (256 MiB read + 256 MiB written), the '
                          'reference for scaling FETCH_SIZE / WRITE_SIZE counters')
@@ -136,6 +248,12 @@ def main() -> None:
     world = int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
         fail_multi(max(args.gpus, world))
+
+    if args.workload in ('c4', 'c5'):
+        res = workload_c4(args) if args.workload == 'c4' else workload_c5(args)
+        res.update({"n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic"})
+        print(json.dumps(res))
+        return
 
     import numpy as np
     import torch

@@ -21,7 +21,7 @@ __all__ = ['device', 'ndimage', 'label', 'autograd']
 
 def __getattr__(name):
     # lazy submodules: importing the package must not require torch's GPU
-    if name in ('ndimage', 'label', 'autograd', 'batched'):
+    if name in ('ndimage', 'label', 'autograd', 'batch'):
         import importlib
         return importlib.import_module(f'{__name__}.{name}')
     raise AttributeError(name)

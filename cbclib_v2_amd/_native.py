@@ -14,7 +14,7 @@ import threading
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, 'libcbx.so')
 
-CBX_OK, CBX_EVALUE, CBX_EINDEX, CBX_ERUNTIME = 0, 1, 2, 3
+CBX_OK, CBX_EVALUE, CBX_EINDEX, CBX_ERUNTIME, CBX_ECAPACITY = 0, 1, 2, 3, 4
 KERNELS = {'biweight': 0, 'gaussian': 1, 'parabolic': 2, 'rectangular': 3, 'triangular': 4}
 OVERLAPS = {'sum': 0, 'max': 1}
 DTYPES = {'bool': 0, 'int32': 1, 'int64': 2}
@@ -23,14 +23,21 @@ EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32
            'cbx_draw_pixels_f32', 'cbx_draw_pixels_f64', 'cbx_draw_lines_adjoint_f32',
            'cbx_draw_lines_adjoint_f64', 'cbx_label', 'cbx_label_threshold_f32', 'cbx_timing_enable',
            'cbx_timing_reset', 'cbx_timing_read', 'cbx_workspace_release', 'cbx_workspace_bytes',
-           'cbx_label_force_path')
+           'cbx_label_force_path', 'cbx_plan_create_f32', 'cbx_plan_draw_f32', 'cbx_plan_check', 'cbx_plan_bytes',
+           'cbx_plan_destroy')
 
 
 class DrawOpts(ctypes.Structure):
     """``cbx_draw_opts``: tile side, columns per walk checkpoint, deterministic
-    flag, tile-kernel workgroup size, exact-value flag."""
+    flag, tile-kernel workgroup size, exact-value flag, items per phase, and
+    per-line profile ids (device pointer or None)."""
     _fields_ = [('tile', ctypes.c_int32), ('seg', ctypes.c_int32), ('deterministic', ctypes.c_int32),
-                ('threads', ctypes.c_int32), ('exact_values', ctypes.c_int32), ('chunk', ctypes.c_int32)]
+                ('threads', ctypes.c_int32), ('exact_values', ctypes.c_int32), ('chunk', ctypes.c_int32),
+                ('kernels', ctypes.c_void_p)]
+
+
+class CapacityError(RuntimeError):
+    """A plan's segment or tile-item capacity was exceeded (create a larger plan)."""
 
 
 _lib = None
@@ -68,6 +75,16 @@ def _declare(L: ctypes.CDLL) -> None:
     L.cbx_timing_reset.argtypes = []
     L.cbx_timing_read.restype = I32
     L.cbx_timing_read.argtypes = [C, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+    L.cbx_plan_create_f32.restype = I32
+    L.cbx_plan_create_f32.argtypes = [ctypes.POINTER(ctypes.c_void_p), I64, I64, I64, P, I64, P, P, I32, OP, D, P]
+    L.cbx_plan_draw_f32.restype = I32
+    L.cbx_plan_draw_f32.argtypes = [P, P, P, I64, P, P, F, I32, I32, P]
+    L.cbx_plan_check.restype = I32
+    L.cbx_plan_check.argtypes = [P, P]
+    L.cbx_plan_bytes.restype = I64
+    L.cbx_plan_bytes.argtypes = [P]
+    L.cbx_plan_destroy.restype = None
+    L.cbx_plan_destroy.argtypes = [P]
     L.cbx_label_force_path.restype = None
     L.cbx_label_force_path.argtypes = [I32]
     L.cbx_workspace_release.restype = I64
@@ -101,6 +118,8 @@ def check(status: int) -> None:
         raise ValueError(msg)
     if status == CBX_EINDEX:
         raise IndexError(msg)
+    if status == CBX_ECAPACITY:
+        raise CapacityError(msg)
     raise RuntimeError(msg)
 
 
@@ -109,6 +128,19 @@ def kernel_id(kernel: str) -> int:
         # kernels::get_type (include.hpp:495-503)
         raise ValueError(f"kernel is missing for {kernel}")
     return KERNELS[kernel]
+
+
+def kernel_ids(kernels) -> 'object':
+    """Per-line profiles as uint8 ids: names, ids, or an integer array."""
+    import numpy as np
+    if hasattr(kernels, 'dtype') and not isinstance(kernels, np.ndarray):
+        return kernels                      # a device tensor of ids
+    arr = np.asarray(kernels)
+    if arr.dtype.kind in 'US':
+        arr = np.array([kernel_id(str(k)) for k in arr.ravel()], np.uint8).reshape(arr.shape)
+    if arr.size and (arr.min() < 0 or arr.max() > 4):
+        raise ValueError("kernel is missing for a per-line profile id")
+    return arr.astype(np.uint8)
 
 
 def overlap_id(overlap: str) -> int:

@@ -19,6 +19,8 @@
 namespace cbx {
 
 enum Profile { K_BIWEIGHT = 0, K_GAUSSIAN = 1, K_PARABOLIC = 2, K_RECTANGULAR = 3, K_TRIANGULAR = 4 };
+// template value for "each line carries its own profile" (LineSetup::kid)
+constexpr int K_MIXED = 5;
 
 // Longest walk accepted (steps along the long axis) and the largest |pixel
 // coordinate| a walk end point may have.
@@ -29,6 +31,8 @@ constexpr double MAX_COORD = 1e9;
 constexpr int FLAG_INDEX = 1;     // idxs outside [0, n_frames)
 constexpr int FLAG_LONG = 2;      // walk longer than MAX_WALK or end point beyond MAX_COORD
 constexpr int FLAG_STUCK = 4;     // a walk did not leave its segment within its step budget
+constexpr int FLAG_KERNEL = 16;   // a per-line profile id outside K_BIWEIGHT .. K_TRIANGULAR
+constexpr int FLAG_OVERFLOW = 32; // a plan's segment or item capacity was exceeded
 
 // ---------------------------------------------------------------------------
 // type-exact math (float stays float: roundf/ceilf/sqrtf of the reference)
@@ -100,6 +104,7 @@ struct alignas(16) LineSetup {
     int8_t st[2];  // step per axis (+1 / -1)
     int8_t valid;  // 0: the line emits nothing
     int8_t safe;   // 1: quotients of its errors by len cannot be subnormal (see line_setup)
+    int8_t kid;    // profile of the line (K_BIWEIGHT .. K_TRIANGULAR)
     T e0[2];       // tangent and normal errors at m0
     T p0[2];       // first end point (for the adjoint's closed-form derivatives)
 };
@@ -363,6 +368,45 @@ __device__ __forceinline__ float profile_fast(float u)
     }
 }
 
+// Profile chosen at run time (per-line profiles).
+__device__ __forceinline__ float profile_fast_dyn(int k, float u)
+{
+    switch (k) {
+    case K_BIWEIGHT: return profile_fast<K_BIWEIGHT>(u);
+    case K_GAUSSIAN: return profile_fast<K_GAUSSIAN>(u);
+    case K_PARABOLIC: return profile_fast<K_PARABOLIC>(u);
+    case K_RECTANGULAR: return profile_fast<K_RECTANGULAR>(u);
+    default: return profile_fast<K_TRIANGULAR>(u);
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T pixel_value_dyn(int k, T err, T max_val)
+{
+    switch (k) {
+    case K_BIWEIGHT: return pixel_value<K_BIWEIGHT>(err, max_val);
+    case K_GAUSSIAN: return pixel_value<K_GAUSSIAN>(err, max_val);
+    case K_PARABOLIC: return pixel_value<K_PARABOLIC>(err, max_val);
+    case K_RECTANGULAR: return pixel_value<K_RECTANGULAR>(err, max_val);
+    default: return pixel_value<K_TRIANGULAR>(err, max_val);
+    }
+}
+
+// profile value of u = err for profile K, or the line's own profile
+template <int K>
+__device__ __forceinline__ float profile_fast_of(int kid, float u)
+{
+    if constexpr (K == K_MIXED) return profile_fast_dyn(kid, u);
+    else return profile_fast<K>(u);
+}
+
+template <int K, typename T>
+__device__ __forceinline__ T pixel_value_of(int kid, T err, T max_val)
+{
+    if constexpr (K == K_MIXED) return pixel_value_dyn(kid, err, max_val);
+    else return pixel_value<K>(err, max_val);
+}
+
 // Decides whether a pixel is emitted (exactly as the reference: error <= 1)
 // and computes its value.  FAST: the error is estimated with reciprocal
 // products and recomputed exactly only within 4e-6 of the threshold (the
@@ -376,11 +420,11 @@ __device__ __forceinline__ bool pixel_eval(const LineSetup<float> &L, float Te, 
         float e = __builtin_fmaf(qt, qt, nsq) * L.rw2f;
         if (__builtin_fabsf(e - 1.0f) <= 4e-6f) e = err_exact(L, at, nsq);
         if (!(e <= 1.0f)) return false;
-        v = max_val * profile_fast<K>(e);
+        v = max_val * profile_fast_of<K>(L.kid, e);
     } else {
         const float e = err_exact(L, at, nsq);
         if (!(e <= 1.0f)) return false;
-        v = pixel_value<K>(e, max_val);
+        v = pixel_value_of<K>(L.kid, e, max_val);
     }
     return true;
 }
@@ -390,7 +434,7 @@ __device__ __forceinline__ bool pixel_eval(const LineSetup<double> &L, double Te
 {
     const double e = err_exact(L, tangent_at(L, Te), nsq);
     if (!(e <= 1.0)) return false;
-    v = pixel_value<K>(e, max_val);
+    v = pixel_value_of<K>(L.kid, e, max_val);
     return true;
 }
 

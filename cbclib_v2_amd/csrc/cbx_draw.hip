@@ -63,11 +63,19 @@ struct Geometry {
 // segment pre-passes
 // ---------------------------------------------------------------------------
 // Line constants and frame check of line l; false if the line draws nothing.
+// kid: the line's profile, kernels[l] when per-line profiles are given.
 template <typename T, typename I>
-__device__ __forceinline__ bool setup_line(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t l,
+__device__ __forceinline__ bool setup_line(const T *__restrict__ lines, const I *__restrict__ idxs,
+                                           const uint8_t *__restrict__ kernels, int kernel, int64_t l,
                                            const Geometry &g, LineSetup<T> &L, int *__restrict__ flags)
 {
     int rc = line_setup(lines + 5 * l, L);
+    const int kid = kernels ? (int)kernels[l] : kernel;
+    L.kid = (int8_t)kid;
+    if (kid < K_BIWEIGHT || kid > K_TRIANGULAR) {
+        atomicOr(flags, FLAG_KERNEL);
+        L.valid = 0;
+    }
     int64_t f = idxs ? (int64_t)idxs[l] : ((g.n_frames == 1) ? 0 : l);
     if (f < 0 || f >= g.n_frames) {
         atomicOr(flags, FLAG_INDEX);
@@ -187,7 +195,7 @@ __device__ __forceinline__ int64_t tile_index(const Geometry &g, int frame, int 
 // scanned offsets.
 template <typename T, typename I, bool WRITE, bool BIN>
 __global__ void __launch_bounds__(256)
-k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_lines, Geometry g,
+k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int kernel, int64_t n_lines, Geometry g,
            LineSetup<T> *__restrict__ setup, int *__restrict__ seg_cnt, int *__restrict__ tile_cnt,
            const int *__restrict__ seg_off, Segment<T> *__restrict__ segs, const int *__restrict__ tile_off,
            int *__restrict__ tile_cur, int *__restrict__ items, int *__restrict__ flags)
@@ -196,7 +204,7 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
     if (l >= n_lines) return;
     LineSetup<T> L;
     if (!WRITE) {
-        const bool ok = setup_line(lines, idxs, l, g, L, flags);
+        const bool ok = setup_line(lines, idxs, (const uint8_t *)nullptr, kernel, l, g, L, flags);
         setup[l] = L;
         if (!ok) {
             seg_cnt[l] = 0;
@@ -234,14 +242,15 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_li
 //   k_bin_slots     one lane per slot: writes the per-tile item lists.
 template <typename T, typename I>
 __global__ void __launch_bounds__(256)
-k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n_lines, Geometry g,
-              LineSetup<T> *__restrict__ setup, unsigned long long *__restrict__ counters, int *__restrict__ flags)
+k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, const uint8_t *__restrict__ kernels,
+              int kernel, int64_t n_lines, Geometry g, LineSetup<T> *__restrict__ setup,
+              unsigned long long *__restrict__ counters, int *__restrict__ flags)
 {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int ub = 0;
     if (l < n_lines) {
         LineSetup<T> L;
-        setup_line(lines, idxs, l, g, L, flags);
+        setup_line(lines, idxs, kernels, kernel, l, g, L, flags);
         setup[l] = L;
         ub = segment_bound(L, g);
     }
@@ -253,8 +262,8 @@ k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, int64_t n
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
-             unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt, int *__restrict__ tile_cnt2,
-             int *__restrict__ flags)
+             int64_t slot_cap, unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt,
+             int *__restrict__ tile_cnt2, int *__restrict__ flags)
 {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -274,9 +283,12 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
     const int total = __shfl(pre, 63, 64);
     pre -= ub;
     unsigned long long wbase = 0;
-    if (lane == 0 && total) wbase = atomicAdd(&counters[1], (unsigned long long)total);
+    if (lane == 0 && total) {
+        wbase = atomicAdd(&counters[1], (unsigned long long)total);
+        if ((int64_t)(wbase + total) > slot_cap) atomicOr(flags, FLAG_OVERFLOW);
+    }
     wbase = __shfl(wbase, 0, 64);
-    if (ub == 0) return;
+    if (ub == 0 || (int64_t)(wbase + total) > slot_cap) return;
     const int64_t base = (int64_t)wbase + pre;
     int nseg = 0;
     // a segment is stored when the next one closes (or after the walk), so
@@ -307,15 +319,42 @@ __global__ void k_tile_total(int64_t n, const int *__restrict__ a, const int *__
     if (i < n) sum[i] = a[i] + b[i];
 }
 
+// Clears a plan's counters before a draw (a kernel, not a memset: a draw may
+// be captured in a HIP graph).
+__global__ void k_plan_reset(int64_t n_tiles, unsigned long long *__restrict__ counters, int *__restrict__ flags,
+                             int *__restrict__ cnt1, int *__restrict__ cnt2, int *__restrict__ cur)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        counters[0] = 0ull;
+        counters[1] = 0ull;
+        *flags = 0;
+    }
+    if (i <= n_tiles) {
+        cnt1[i] = 0;
+        cnt2[i] = 0;
+    }
+    if (i < n_tiles) cur[i] = 0;
+}
+
 // Tile t's list: class-1 pieces at [0, cnt1[t]) by their ranks, then class 2.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup, const Segment<T> *__restrict__ segs,
             const int *__restrict__ tile_off, const int *__restrict__ cnt1, int *__restrict__ cur2,
-            int2 *__restrict__ pairs)
+            int2 *__restrict__ pairs, const unsigned long long *__restrict__ counters, int64_t n_tiles,
+            int64_t item_cap, int *__restrict__ flags)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_slots) return;
+    // after a slot overflow some reserved slots were never written (stale
+    // segments): bin nothing; otherwise the slots taken by the walk, and the
+    // item total against the list capacity
+    if (*flags & FLAG_OVERFLOW) return;
+    if (i >= n_slots || i >= (int64_t)counters[1]) return;
+    if ((int64_t)tile_off[n_tiles] > item_cap) {
+        if (i == 0) atomicOr(flags, FLAG_OVERFLOW);
+        return;
+    }
     const Segment<T> sg = segs[i];
     if (sg.ncols <= 0) return;
     const int axis = setup[sg.line].axis, frame = setup[sg.line].frame;
@@ -460,7 +499,7 @@ __device__ __forceinline__ void fill_half_fast(Acc *img, uint32_t *zb, const Lin
                                                float vscale)
 {
     const float len = L.len, rlen = L.rlen, LL = L.LL, w2 = L.w2, rw2f = L.rw2f;
-    const int b = 1 - L.axis;
+    const int b = 1 - L.axis, kid = L.kid;
     const int sb = pick(L.st, b);
     const int dir = half ? -sb : sb;
     const float tb = pick(L.t, b), nb = pick(L.nv, b);
@@ -472,7 +511,7 @@ __device__ __forceinline__ void fill_half_fast(Acc *img, uint32_t *zb, const Lin
         const float qt = at * rlen;
         float e = __builtin_fmaf(qt, qt, nsq) * rw2f;
         if (__builtin_fabsf(e - 1.0f) <= 4e-6f) e = err_exact(L, at, nsq);
-        if (e <= 1.0f) tile_put<OV>(img, zb, cbase + (s - s_lo) * sstride, vscale * profile_fast<K>(e));
+        if (e <= 1.0f) tile_put<OV>(img, zb, cbase + (s - s_lo) * sstride, vscale * profile_fast_of<K>(kid, e));
     };
     if (half == 0 && cs >= s_lo && cs < s_hi) {
         const float q = div_rn_nz(Ne, len, rlen);
@@ -536,6 +575,8 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     __shared__ int32_t hc0[CH];
 
     const int tile = blockIdx.x;
+    // a plan's draw reports errors afterwards: leave `out` untouched then
+    if (*flags) return;
     const int beg = tile_off[tile], end = tile_off[tile + 1];
     if (beg == end) return;
     const int per_frame = g.ntx * g.nty;
@@ -1245,14 +1286,16 @@ static int flags_to_status(int flags)
     if (flags & FLAG_INDEX) return fail(CBX_EINDEX, "idxs is outside of (0, n_frames)");
     if (flags & FLAG_LONG)
         return fail(CBX_EVALUE, "a line's walk exceeds the supported length (2^26 px) or coordinate range (1e9)");
+    if (flags & FLAG_KERNEL) return fail(CBX_EVALUE, "kernel is missing for a per-line profile id");
     if (flags & FLAG_STUCK) return fail(CBX_ERUNTIME, "a line walk did not terminate within its step budget");
+    if (flags & FLAG_OVERFLOW) return fail(CBX_ECAPACITY, "the plan's segment or tile-item capacity was exceeded");
     return CBX_OK;
 }
 
 // Builds segments (and, with bin, the per-tile item lists).  One host sync.
 template <typename T, typename I>
-static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t n_lines, int64_t n_frames,
-                      int64_t H, int64_t W, int tile, int seg, bool bin, hipStream_t st)
+static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int kernel, int64_t n_lines,
+                      int64_t n_frames, int64_t H, int64_t W, int tile, int seg, bool bin, hipStream_t st)
 {
     Geometry &g = P.g;
     g.n_frames = n_frames;
@@ -1282,11 +1325,11 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t 
         KernelTimer timer1("segments_pass1", st);
     if (bin)
         k_segments<T, I, false, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
-            lines, idxs, n_lines, g, P.setup.ptr, P.seg_cnt.ptr, P.tile_cnt.ptr, nullptr, nullptr, nullptr, nullptr,
+            lines, idxs, kernel, n_lines, g, P.setup.ptr, P.seg_cnt.ptr, P.tile_cnt.ptr, nullptr, nullptr, nullptr, nullptr,
             nullptr, P.flags.ptr);
     else
         k_segments<T, I, false, false><<<grid_for(n_lines, blk), blk, 0, st>>>(
-            lines, idxs, n_lines, g, P.setup.ptr, P.seg_cnt.ptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+            lines, idxs, kernel, n_lines, g, P.setup.ptr, P.seg_cnt.ptr, nullptr, nullptr, nullptr, nullptr, nullptr,
             nullptr, P.flags.ptr);
     }
     CBX_HIP(hipGetLastError());
@@ -1311,11 +1354,11 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t 
         CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
         CBX_HIP(hipMemsetAsync(P.tile_cur.ptr, 0, sizeof(int) * P.n_tiles, st));
         k_segments<T, I, true, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
-            lines, idxs, n_lines, g, P.setup.ptr, nullptr, nullptr, P.seg_off.ptr, P.segs.ptr, P.tile_off.ptr,
+            lines, idxs, kernel, n_lines, g, P.setup.ptr, nullptr, nullptr, P.seg_off.ptr, P.segs.ptr, P.tile_off.ptr,
             P.tile_cur.ptr, P.items.ptr, P.flags.ptr);
     } else {
         k_segments<T, I, true, false><<<grid_for(n_lines, blk), blk, 0, st>>>(
-            lines, idxs, n_lines, g, P.setup.ptr, nullptr, nullptr, P.seg_off.ptr, P.segs.ptr, nullptr, nullptr,
+            lines, idxs, kernel, n_lines, g, P.setup.ptr, nullptr, nullptr, P.seg_off.ptr, P.segs.ptr, nullptr, nullptr,
             nullptr, P.flags.ptr);
     }
     CBX_HIP(hipGetLastError());
@@ -1327,8 +1370,9 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t 
 // slots have ncols = 0 and are in no item list).  Two host syncs: the slot
 // total, then the item total.
 template <typename T, typename I>
-static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs, int64_t n_lines, int64_t n_frames,
-                                int64_t H, int64_t W, int tile, int seg, hipStream_t st)
+static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs, const uint8_t *kernels,
+                                int kernel, int64_t n_lines, int64_t n_frames, int64_t H, int64_t W, int tile,
+                                int seg, hipStream_t st)
 {
     Geometry &g = P.g;
     g.n_frames = n_frames;
@@ -1352,8 +1396,8 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     CBX_HIP(hipMemsetAsync(counters.ptr, 0, 2 * sizeof(unsigned long long), st));
     CBX_HIP(hipMemsetAsync(P.tile_cnt.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
     const int blk = 256;
-    k_setup_lines<T, I><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, n_lines, g, P.setup.ptr, counters.ptr,
-                                                               P.flags.ptr);
+    k_setup_lines<T, I><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
+                                                               P.setup.ptr, counters.ptr, P.flags.ptr);
     CBX_HIP(hipGetLastError());
     unsigned long long slots = 0;
     int fl = 0;
@@ -1372,8 +1416,8 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     CBX_HIP(hipMemsetAsync(cnt2.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
     {
         KernelTimer timer("segments_walk", st);
-        k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, counters.ptr,
-                                                                P.tile_cnt.ptr, cnt2.ptr, P.flags.ptr);
+        k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                counters.ptr, P.tile_cnt.ptr, cnt2.ptr, P.flags.ptr);
     }
     CBX_HIP(hipGetLastError());
     k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, cnt2.ptr, total.ptr);
@@ -1389,7 +1433,8 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
     CBX_HIP(hipMemsetAsync(P.tile_cur.ptr, 0, sizeof(int) * P.n_tiles, st));
     k_bin_slots<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr, P.tile_off.ptr,
-                                                            P.tile_cnt.ptr, P.tile_cur.ptr, P.pairs.ptr);
+                                                            P.tile_cnt.ptr, P.tile_cur.ptr, P.pairs.ptr, counters.ptr,
+                                                            P.n_tiles, P.n_items, P.flags.ptr);
     CBX_HIP(hipGetLastError());
     return CBX_OK;
 }
@@ -1454,7 +1499,8 @@ static void launch_tiles_ov(int kernel, T *out, const SegmentPlan<T> &P, T max_v
     case K_GAUSSIAN: launch_tiles_k<T, K_GAUSSIAN, OV>(out, P, max_val, threads, chunk, fast, st); break;
     case K_PARABOLIC: launch_tiles_k<T, K_PARABOLIC, OV>(out, P, max_val, threads, chunk, fast, st); break;
     case K_RECTANGULAR: launch_tiles_k<T, K_RECTANGULAR, OV>(out, P, max_val, threads, chunk, fast, st); break;
-    default: launch_tiles_k<T, K_TRIANGULAR, OV>(out, P, max_val, threads, chunk, fast, st); break;
+    case K_TRIANGULAR: launch_tiles_k<T, K_TRIANGULAR, OV>(out, P, max_val, threads, chunk, fast, st); break;
+    default: launch_tiles_k<T, K_MIXED, OV>(out, P, max_val, threads, chunk, fast, st); break;
     }
 }
 
@@ -1599,6 +1645,26 @@ static int ordered_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_va
     return CBX_OK;
 }
 
+// The tile kernel over a built unordered plan; kernel may be K_MIXED.
+template <typename T>
+static void run_tiles(T *out, const SegmentPlan<T> &P, int kernel, int overlap, T max_val, const cbx_draw_opts *opts,
+                      hipStream_t st)
+{
+    const int threads = (opts && opts->threads) ? opts->threads : 256;
+    const bool fast = !(opts && opts->exact_values);
+    const int chunk = (opts && opts->chunk) ? opts->chunk : 64;
+    // float sums on the fast path accumulate in fixed point, unless max_val
+    // is not finite (inf * 0 must stay NaN, as in the reference)
+    if constexpr (sizeof(T) == 4) {
+        if (overlap == 0 && fast && std::isfinite(max_val)) {
+            launch_tiles_ov<T, 2>(kernel, out, P, max_val, threads, chunk, fast, st);
+            return;
+        }
+    }
+    if (overlap == 0) launch_tiles_ov<T, 0>(kernel, out, P, max_val, threads, chunk, fast, st);
+    else launch_tiles_ov<T, 1>(kernel, out, P, max_val, threads, chunk, fast, st);
+}
+
 template <typename T, typename I>
 static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const T *lines, int64_t n_lines,
                            const I *idxs, T max_val, int kernel, int overlap, const cbx_draw_opts *opts, void *stream)
@@ -1613,35 +1679,25 @@ static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const
     if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
     hipStream_t st = (hipStream_t)stream;
     const bool det = opts && opts->deterministic;
+    const uint8_t *kernels = opts ? opts->kernels : nullptr;
+    if (kernels && det) return fail(CBX_EVALUE, "per-line profiles are drawn by the tile path only (deterministic = 0)");
     if (det) {
         tile = 32;
         seg = 16;
     }
     SegmentPlan<T> P;
     if (det) {
-        if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
+        if ((rc = build_plan<T, I>(P, lines, idxs, kernel, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
         if (P.n_segs == 0) return CBX_OK;
         return ordered_pipeline<T>(P, kernel, overlap, max_val, out, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
                                    nullptr, st);
     }
-    if ((rc = build_plan_unordered<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, st))) return rc;
+    if ((rc = build_plan_unordered<T, I>(P, lines, idxs, kernels, kernel, n_lines, n_frames, H, W, tile, seg, st)))
+        return rc;
     if (P.n_segs == 0) return CBX_OK;
     {
         KernelTimer timer("draw_tiles", st);
-        const int threads = (opts && opts->threads) ? opts->threads : 256;
-        const bool fast = !(opts && opts->exact_values);
-        const int chunk = (opts && opts->chunk) ? opts->chunk : 64;
-        // float sums on the fast path accumulate in fixed point, unless
-        // max_val is not finite (inf * 0 must stay NaN, as in the reference)
-        if constexpr (sizeof(T) == 4) {
-            if (overlap == 0 && fast && std::isfinite(max_val)) {
-                launch_tiles_ov<T, 2>(kernel, out, P, max_val, threads, chunk, fast, st);
-                CBX_HIP(hipGetLastError());
-                return CBX_OK;
-            }
-        }
-        if (overlap == 0) launch_tiles_ov<T, 0>(kernel, out, P, max_val, threads, chunk, fast, st);
-        else launch_tiles_ov<T, 1>(kernel, out, P, max_val, threads, chunk, fast, st);
+        run_tiles<T>(out, P, kernels ? K_MIXED : kernel, overlap, max_val, opts, st);
     }
     CBX_HIP(hipGetLastError());
     return CBX_OK;
@@ -1659,7 +1715,7 @@ static int draw_pixels_impl(int64_t n_frames, int64_t H, int64_t W, const T *lin
     if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
     hipStream_t st = (hipStream_t)stream;
     SegmentPlan<T> P;
-    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, 32, 16, merge != 0, st))) return rc;
+    if ((rc = build_plan<T, I>(P, lines, idxs, kernel, n_lines, n_frames, H, W, 32, 16, merge != 0, st))) return rc;
     if (merge) return ordered_pipeline<T>(P, kernel, overlap, max_val, nullptr, of, oy, ox, ov, ol, capacity, count, st);
     return raw_list_pipeline<T>(P, kernel, max_val, of, oy, ox, ov, ol, capacity, count, st);
 }
@@ -1697,7 +1753,7 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     CBX_HIP(hipMemsetAsync(grad_lines, 0, sizeof(T) * 5 * n_lines, st));
     if (n_frames * H * W == 0 || kernel == K_RECTANGULAR) return CBX_OK;
     SegmentPlan<T> P;
-    if ((rc = build_plan<T, I>(P, lines, idxs, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
+    if ((rc = build_plan<T, I>(P, lines, idxs, kernel, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
     if (P.n_segs == 0) return CBX_OK;
     const int blk = 256;
     DevBuf<int64_t> scnt, sbase;
@@ -1727,6 +1783,181 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
 }
 
 } // namespace cbx
+
+// ---------------------------------------------------------------------------
+// plans: the unordered tile path sized once, then drawn without host syncs,
+// allocations or memsets
+// ---------------------------------------------------------------------------
+struct cbx_plan {
+    cbx::SegmentPlan<float> P; // capacities in P.n_segs (slots) and P.n_items
+    cbx::DevBuf<unsigned long long> counters;
+    cbx::DevBuf<int> cnt2, total;
+    cbx::DevBuf<uint8_t> scan_tmp;
+    size_t scan_bytes = 0;
+    int64_t n_lines_max = 0;
+    int64_t bytes = 0;
+    cbx_draw_opts opts{};
+};
+
+namespace cbx {
+
+static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t W, const float *lines,
+                       int64_t n_lines, const int32_t *idxs, const uint8_t *kernels, int kernel,
+                       const cbx_draw_opts *opts, double headroom, hipStream_t st)
+{
+    if (!out_plan) return fail(CBX_EVALUE, "plan must not be NULL");
+    *out_plan = nullptr;
+    int rc = check_common(n_frames, H, W, n_lines, idxs != nullptr, kernel, 0);
+    if (rc) return rc;
+    if (!(headroom >= 1.0) || headroom > 64.0) return fail(CBX_EVALUE, "headroom must be in [1, 64]");
+    if (n_lines <= 0 || n_frames * H * W == 0) return fail(CBX_EVALUE, "a plan needs lines and a non-empty frame");
+    int tile, seg;
+    resolve_opts(opts, tile, seg);
+    if ((rc = check_opts(tile, seg))) return rc;
+    if (opts && !(opts->threads == 0 || opts->threads == 64 || opts->threads == 128 || opts->threads == 256))
+        return fail(CBX_EVALUE, "threads must be 64, 128 or 256");
+    if (opts && opts->deterministic) return fail(CBX_EVALUE, "plans draw with the tile path (deterministic = 0)");
+    // sizing pass on the given lines (intermediates from the workspace cache)
+    int64_t slots = 0, items = 0;
+    {
+        Workspace ws(st);
+        SegmentPlan<float> S;
+        if ((rc = build_plan_unordered<float, int32_t>(S, lines, idxs, kernels, kernel, n_lines, n_frames, H, W, tile,
+                                                        seg, st)))
+            return rc;
+        slots = S.n_segs;
+        items = S.n_items;
+    }
+    const int64_t slot_cap = std::max<int64_t>((int64_t)std::ceil((double)slots * headroom), slots + 1024);
+    const int64_t item_cap = std::max<int64_t>((int64_t)std::ceil((double)items * headroom), items + 1024);
+    if (slot_cap >= (int64_t(1) << 31) || item_cap >= (int64_t(1) << 31))
+        return fail(CBX_EVALUE, "plan capacity exceeds 2^31 entries");
+    cbx_plan *pl = new cbx_plan();
+    SegmentPlan<float> &P = pl->P;
+    Geometry &g = P.g;
+    g.n_frames = n_frames;
+    g.H = (int32_t)H;
+    g.W = (int32_t)W;
+    g.tile = tile;
+    g.seg = seg;
+    g.ntx = (int32_t)((W + tile - 1) / tile);
+    g.nty = (int32_t)((H + tile - 1) / tile);
+    P.n_lines = n_lines;
+    P.n_tiles = n_frames * g.ntx * g.nty;
+    P.n_segs = slot_cap;
+    P.n_items = item_cap;
+    pl->n_lines_max = n_lines;
+    if (opts) pl->opts = *opts;
+    pl->opts.kernels = nullptr;
+    pl->opts.tile = tile;
+    pl->opts.seg = seg;
+    hipError_t e = hipSuccess;
+    auto take = [&](hipError_t r) {
+        if (e == hipSuccess) e = r;
+    };
+    // owned allocations (no workspace is open here)
+    take(P.setup.alloc(n_lines, st));
+    take(P.flags.alloc(1, st));
+    take(P.segs.alloc(slot_cap, st));
+    take(P.tile_cnt.alloc(P.n_tiles + 1, st));
+    take(P.tile_off.alloc(P.n_tiles + 1, st));
+    take(P.tile_cur.alloc(P.n_tiles, st));
+    take(P.pairs.alloc(item_cap, st));
+    take(pl->counters.alloc(2, st));
+    take(pl->cnt2.alloc(P.n_tiles + 1, st));
+    take(pl->total.alloc(P.n_tiles + 1, st));
+    if (e == hipSuccess)
+        e = hipcub::DeviceScan::ExclusiveSum(nullptr, pl->scan_bytes, pl->total.ptr, P.tile_off.ptr,
+                                             (int)(P.n_tiles + 1), st);
+    take(pl->scan_tmp.alloc(pl->scan_bytes ? pl->scan_bytes : 1, st));
+    if (e != hipSuccess) {
+        delete pl;
+        return fail(CBX_ERUNTIME, std::string("plan allocation: ") + hipGetErrorString(e));
+    }
+    pl->bytes = (int64_t)(sizeof(LineSetup<float>) * n_lines + sizeof(Segment<float>) * slot_cap +
+                          sizeof(int2) * item_cap + 4 * sizeof(int) * (P.n_tiles + 1) + pl->scan_bytes);
+    *out_plan = pl;
+    return CBX_OK;
+}
+
+static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lines, const int32_t *idxs,
+                     const uint8_t *kernels, float max_val, int kernel, int overlap, hipStream_t st)
+{
+    if (!pl) return fail(CBX_EVALUE, "plan must not be NULL");
+    SegmentPlan<float> &P = pl->P;
+    int rc = check_common(P.g.n_frames, P.g.H, P.g.W, n_lines, idxs != nullptr, kernel, overlap);
+    if (rc) return rc;
+    if (n_lines > pl->n_lines_max) return fail(CBX_EVALUE, "more lines than the plan was created for");
+    if (n_lines == 0) return CBX_OK;
+    const int blk = 256;
+    const Geometry &g = P.g;
+    k_plan_reset<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles, pl->counters.ptr, P.flags.ptr,
+                                                               P.tile_cnt.ptr, pl->cnt2.ptr, P.tile_cur.ptr);
+    k_setup_lines<float, int32_t><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
+                                                                         P.setup.ptr, pl->counters.ptr, P.flags.ptr);
+    k_walk_lines<float><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
+                                                                P.flags.ptr);
+    k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, pl->cnt2.ptr,
+                                                              pl->total.ptr);
+    size_t bytes = pl->scan_bytes;
+    CBX_HIP(hipcub::DeviceScan::ExclusiveSum(pl->scan_tmp.ptr, bytes, pl->total.ptr, P.tile_off.ptr,
+                                             (int)(P.n_tiles + 1), st));
+    k_bin_slots<float><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr,
+                                                                P.tile_off.ptr, P.tile_cnt.ptr, P.tile_cur.ptr,
+                                                                P.pairs.ptr, pl->counters.ptr, P.n_tiles, P.n_items,
+                                                                P.flags.ptr);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(st, &cap);
+    if (cap == hipStreamCaptureStatusNone) {
+        KernelTimer timer("draw_tiles", st);
+        run_tiles<float>(out, P, kernels ? K_MIXED : kernel, overlap, max_val, &pl->opts, st);
+    } else {
+        run_tiles<float>(out, P, kernels ? K_MIXED : kernel, overlap, max_val, &pl->opts, st);
+    }
+    CBX_HIP(hipGetLastError());
+    return CBX_OK;
+}
+
+static int plan_check(cbx_plan *pl, hipStream_t st)
+{
+    if (!pl) return fail(CBX_EVALUE, "plan must not be NULL");
+    int fl = 0;
+    CBX_HIP(hipMemcpyAsync(&fl, pl->P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
+    CBX_HIP(hipStreamSynchronize(st));
+    return flags_to_status(fl);
+}
+
+} // namespace cbx
+
+extern "C" {
+
+int cbx_plan_create_f32(cbx_plan **plan, int64_t n_frames, int64_t height, int64_t width, const float *lines,
+                        int64_t n_lines, const int32_t *idxs, const uint8_t *kernels, int kernel,
+                        const cbx_draw_opts *opts, double headroom, void *stream)
+{
+    return cbx::plan_create(plan, n_frames, height, width, lines, n_lines, idxs, kernels, kernel, opts, headroom,
+                            (hipStream_t)stream);
+}
+
+int cbx_plan_draw_f32(cbx_plan *plan, float *out, const float *lines, int64_t n_lines, const int32_t *idxs,
+                      const uint8_t *kernels, float max_val, int kernel, int overlap, void *stream)
+{
+    return cbx::plan_draw(plan, out, lines, n_lines, idxs, kernels, max_val, kernel, overlap, (hipStream_t)stream);
+}
+
+int cbx_plan_check(cbx_plan *plan, void *stream) { return cbx::plan_check(plan, (hipStream_t)stream); }
+
+int64_t cbx_plan_bytes(const cbx_plan *plan) { return plan ? plan->bytes : 0; }
+
+void cbx_plan_destroy(cbx_plan *plan)
+{
+    if (!plan) return;
+    (void)hipDeviceSynchronize();
+    delete plan;
+}
+
+} // extern "C"
 
 // ---------------------------------------------------------------------------
 // C ABI

@@ -57,22 +57,23 @@ def _check_idxs(idxs: Any, n_lines: int, n_frames: int) -> None:
 
 
 def _opts(tile: Optional[int], seg: Optional[int], deterministic: bool, threads: Optional[int] = None,
-          exact_values: bool = False, chunk: Optional[int] = None):
+          exact_values: bool = False, chunk: Optional[int] = None, kernels_ptr: Optional[int] = None):
     return _native.DrawOpts(int(tile or 0), int(seg or 0), 1 if deterministic else 0, int(threads or 0),
-                            1 if exact_values else 0, int(chunk or 0))
+                            1 if exact_values else 0, int(chunk or 0), kernels_ptr)
 
 
 def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, kernel: str = 'rectangular',
                overlap: str = 'sum', *, tile: Optional[int] = None, seg: Optional[int] = None,
                deterministic: bool = False, threads: Optional[int] = None, exact_values: bool = False,
-               chunk: Optional[int] = None):
+               chunk: Optional[int] = None, kernels: Any = None):
     """Draw thick lines with a profile into ``out`` (in place) and return ``out``.
 
     Extra keyword-only arguments (no reference counterpart): ``tile`` (32 or
     64, pixel tile per workgroup), ``seg`` (8 or 16, walk checkpoint spacing),
     ``threads`` (64, 128 or 256, tile workgroup size), ``deterministic``
-    (reference summation order: bit-exact images) and ``exact_values``
-    (reference value arithmetic in the tile path).
+    (reference summation order: bit-exact images), ``exact_values``
+    (reference value arithmetic in the tile path) and ``kernels`` (one profile
+    per line, names or ids; ``kernel`` is then ignored; tile path only).
     """
     device.require_rocm('draw_lines')
     kid = _native.kernel_id(kernel)
@@ -94,7 +95,12 @@ def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, ker
     t_idxs = None if idxs is None else to_device(idxs, torch.int64 if f64 else torch.int32)
     L = _native.lib()
     fn = L.cbx_draw_lines_f64 if f64 else L.cbx_draw_lines_f32
-    opts = _opts(tile, seg, deterministic, threads, exact_values, chunk)
+    t_kern = None
+    if kernels is not None:
+        t_kern = to_device(_native.kernel_ids(kernels), torch.uint8).reshape(-1)
+        if t_kern.numel() != n_lines:
+            raise ValueError(f"kernels has an invalid size ({t_kern.numel()} != {n_lines})")
+    opts = _opts(tile, seg, deterministic, threads, exact_values, chunk, ptr(t_kern))
     _native.check(fn(t_out.data_ptr(), n_frames, H, W, t_lines.data_ptr(), n_lines, ptr(t_idxs), float(max_val),
                      kid, oid, ctypes.byref(opts), stream_ptr()))
     return finish()

@@ -32,6 +32,7 @@ extern "C" {
 #define CBX_EVALUE 1
 #define CBX_EINDEX 2
 #define CBX_ERUNTIME 3
+#define CBX_ECAPACITY 4 /* a plan's capacity was exceeded: create a larger plan (RuntimeError) */
 
 /* Profiles, numbered as kernels::type (cbclib_v2/_src/src/include.hpp:473-480). */
 #define CBX_BIWEIGHT 0
@@ -78,6 +79,10 @@ typedef struct cbx_draw_opts {
     int32_t threads;      /* workgroup size of the tile kernel: 64, 128 or 256 (0 = 256) */
     int32_t exact_values; /* 1: the tile path computes values with the reference's arithmetic */
     int32_t chunk;        /* items per phase of the tile kernel: 64 or the workgroup size (0 = 64) */
+    const uint8_t *kernels; /* NULL, or a DEVICE array of n_lines profile ids (CBX_BIWEIGHT ..): each line
+                               drawn with its own profile (tile path only; the `kernel` argument is then
+                               ignored).  No reference counterpart: the reference takes one profile per
+                               call, so mixed-profile frames need one call per profile there. */
 } cbx_draw_opts;
 
 /*
@@ -151,6 +156,33 @@ CBX_API int cbx_label(const void *inp, int dtype, int ndim, const int64_t *shape
  */
 CBX_API int cbx_label_threshold_f32(const float *image, float vmin, int ndim, const int64_t *shape, const int64_t *shifts,
                             int64_t n_shifts, int64_t npts, int32_t *out, int64_t *n_labels, void *stream);
+
+/*
+ * Plans (no reference counterpart): the tile path of cbx_draw_lines_f32 split
+ * into sizing and drawing, for repeated draws of similar line sets (batched
+ * frames, refinement loops) and for HIP-graph capture.
+ *
+ * cbx_plan_create_f32 sizes the intermediate buffers (segment checkpoints and
+ * per-tile item lists) on the given lines, times `headroom` (>= 1), and
+ * allocates them once; the frame shape, the maximum line count and the launch
+ * options (tile, seg, threads, chunk, exact_values) are fixed by the plan.
+ * cbx_plan_draw_f32 draws `n_lines` <= the plan's line count into `out` with
+ * no host synchronisation, no allocation and no memset (the whole call can be
+ * captured in a HIP graph and replayed); `kernels` (DEVICE, n_lines profile
+ * ids, or NULL for `kernel`) gives per-line profiles.  Errors of a draw (bad
+ * idxs or profile ids, capacity exceeded) leave `out` untouched and are
+ * reported by the next cbx_plan_check, which waits for the stream;
+ * CBX_ECAPACITY means: create a plan with more headroom and draw again.
+ */
+typedef struct cbx_plan cbx_plan;
+CBX_API int cbx_plan_create_f32(cbx_plan **plan, int64_t n_frames, int64_t height, int64_t width, const float *lines,
+                                int64_t n_lines, const int32_t *idxs, const uint8_t *kernels, int kernel,
+                                const cbx_draw_opts *opts, double headroom, void *stream);
+CBX_API int cbx_plan_draw_f32(cbx_plan *plan, float *out, const float *lines, int64_t n_lines, const int32_t *idxs,
+                              const uint8_t *kernels, float max_val, int kernel, int overlap, void *stream);
+CBX_API int cbx_plan_check(cbx_plan *plan, void *stream);
+CBX_API int64_t cbx_plan_bytes(const cbx_plan *plan);
+CBX_API void cbx_plan_destroy(cbx_plan *plan);
 
 /*
  * Path selection for tests and measurements (no reference counterpart):
