@@ -263,7 +263,7 @@ template <typename T>
 __global__ void __launch_bounds__(256)
 k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
              int64_t slot_cap, unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt,
-             int *__restrict__ tile_cnt2, int *__restrict__ flags)
+             int *__restrict__ tile_cnt2, int *__restrict__ flags, int64_t *__restrict__ line_base)
 {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -288,6 +288,8 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
         if ((int64_t)(wbase + total) > slot_cap) atomicOr(flags, FLAG_OVERFLOW);
     }
     wbase = __shfl(wbase, 0, 64);
+    // first slot of each line (the adjoint's per-line reduction)
+    if (line_base && l < n_lines) line_base[l] = (int64_t)wbase + pre;
     if (ub == 0 || (int64_t)(wbase + total) > slot_cap) return;
     const int64_t base = (int64_t)wbase + pre;
     int nseg = 0;
@@ -1100,42 +1102,54 @@ __global__ void k_row_write(const uint8_t *__restrict__ touched, const T *__rest
 // adjoint (sum overlap): per (segment, tile) partial gradients, then an
 // ordered per-line reduction
 // ---------------------------------------------------------------------------
-// Per-pixel gradient terms, in double: the triangular profile's derivative
-// carries the unit vector (p - q) / dist, which float loses near the line.
-template <typename T, int K>
-__device__ __forceinline__ void pixel_grad(const LineSetup<T> &L, int x, int y, T gv, T max_val, double *acc)
+// Per-line constants of the per-pixel gradient terms (double).
+struct GradConst {
+    double x0, y0, tx, ty, inv_mag, w, inv_w, inv_w2, sg;
+};
+
+template <typename T>
+__device__ __forceinline__ GradConst grad_const(const LineSetup<T> &L, T max_val)
 {
-    const double tx = (double)L.t[0], ty = (double)L.t[1];
-    const double rx = (double)x - (double)L.p0[0], ry = (double)y - (double)L.p0[1];
-    const double mag = tx * tx + ty * ty;
-    double ts = (rx * tx + ry * ty) / mag;
+    GradConst c;
+    c.x0 = (double)L.p0[0];
+    c.y0 = (double)L.p0[1];
+    c.tx = (double)L.t[0];
+    c.ty = (double)L.t[1];
+    c.inv_mag = 1.0 / (c.tx * c.tx + c.ty * c.ty);
+    c.w = (double)L.w;
+    c.inv_w = 1.0 / c.w;
+    c.inv_w2 = c.inv_w * c.inv_w;
+    c.sg = (double)max_val;
+    return c;
+}
+
+// Per-pixel gradient terms of max_val K(dist / w) w.r.t. (x0, y0, x1, y1, w),
+// in double: the triangular profile's derivative carries the unit vector
+// (p - q) / dist, which float loses near the line.
+template <int K>
+__device__ __forceinline__ void pixel_grad(const GradConst &C, int x, int y, double gv, double *acc)
+{
+    const double rx = (double)x - C.x0, ry = (double)y - C.y0;
+    double ts = (rx * C.tx + ry * C.ty) * C.inv_mag;
     ts = (ts < 0.0) ? 0.0 : ((ts > 1.0) ? 1.0 : ts);
-    const double dx = rx - ts * tx, dy = ry - ts * ty;
+    const double dx = rx - ts * C.tx, dy = ry - ts * C.ty;
     const double d2 = dx * dx + dy * dy;
-    const double w = (double)L.w, w2 = w * w;
-    const double u = d2 / w2;
+    const double u = d2 * C.inv_w2;
     double dk_dd2, dk_dw;
     if constexpr (K == K_TRIANGULAR) {
         const double dist = sqrt(d2);
-        dk_dd2 = (dist > 0.0) ? -1.0 / (2.0 * w * dist) : 0.0;
-        dk_dw = dist / w2;
-    } else if constexpr (K == K_GAUSSIAN) {
-        const double k1 = -4.5 * 0.3989422804014327 * ::exp(-4.5 * u);
-        dk_dd2 = k1 / w2;
-        dk_dw = k1 * (-2.0 * u / w);
-    } else if constexpr (K == K_BIWEIGHT) {
-        const double k1 = -1.875 * (1.0 - u);
-        dk_dd2 = k1 / w2;
-        dk_dw = k1 * (-2.0 * u / w);
-    } else if constexpr (K == K_PARABOLIC) {
-        const double k1 = -0.75;
-        dk_dd2 = k1 / w2;
-        dk_dw = k1 * (-2.0 * u / w);
+        dk_dd2 = (dist > 0.0) ? -0.5 * C.inv_w / dist : 0.0;
+        dk_dw = dist * C.inv_w2;
     } else {
-        dk_dd2 = 0.0;
-        dk_dw = 0.0;
+        double k1;
+        if constexpr (K == K_GAUSSIAN) k1 = -4.5 * 0.3989422804014327 * ::exp(-4.5 * u);
+        else if constexpr (K == K_BIWEIGHT) k1 = -1.875 * (1.0 - u);
+        else if constexpr (K == K_PARABOLIC) k1 = -0.75;
+        else k1 = 0.0;
+        dk_dd2 = k1 * C.inv_w2;
+        dk_dw = -2.0 * k1 * u * C.inv_w;
     }
-    const double sg = (double)gv * (double)max_val;
+    const double sg = gv * C.sg;
     const double c = sg * dk_dd2 * -2.0;
     acc[0] += c * (1.0 - ts) * dx;
     acc[1] += c * (1.0 - ts) * dy;
@@ -1144,10 +1158,55 @@ __device__ __forceinline__ void pixel_grad(const LineSetup<T> &L, int x, int y, 
     acc[4] += sg * dk_dw;
 }
 
+// The fills of one column for the float fast path (both halves and the
+// centre), with the forward's inclusion decisions (fast error estimate, exact
+// recompute within 4e-6 of the edge); lines with L.safe only.
+template <int K, int TILE>
+__device__ __forceinline__ void adj_column_fast(const LineSetup<float> &L, const GradConst &C, const float *gt, int cs,
+                                                float Te0, float Ne0, int c, int X0, int Y0, int s_lo, int s_hi,
+                                                double *acc)
+{
+    const float len = L.len, rlen = L.rlen, LL = L.LL, w2 = L.w2, rw2f = L.rw2f;
+    const int a = L.axis, b = 1 - a;
+    auto visit = [&](int s, float T, float nsq) {
+        const float a0 = -T, a1 = T - LL;
+        float at = (a0 < a1) ? a1 : a0;
+        at = (at < 0.0f) ? 0.0f : at;
+        const float qt = at * rlen;
+        float e = __builtin_fmaf(qt, qt, nsq) * rw2f;
+        if (__builtin_fabsf(e - 1.0f) <= 4e-6f) e = err_exact(L, at, nsq);
+        if (!(e <= 1.0f)) return;
+        const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
+        const float gv = gt[(y - Y0) * TILE + (x - X0)];
+        if (gv != 0.0f) pixel_grad<K>(C, x, y, (double)gv, acc);
+    };
+    if (cs >= s_lo && cs < s_hi) {
+        const float q = div_rn_nz(Ne0, len, rlen);
+        visit(cs, Te0, q * q);
+    }
+    const int sb = pick(L.st, b);
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        const int dir = half ? -sb : sb;
+        const float dT = (dir > 0) ? pick(L.t, b) : -pick(L.t, b), dN = (dir > 0) ? pick(L.nv, b) : -pick(L.nv, b);
+        const int nsteps = (dir > 0) ? (s_hi - 1 - cs) : (cs - s_lo);
+        const int jin = (dir > 0) ? (s_lo - cs) : (cs - s_hi + 1);
+        float Te = Te0, Ne = Ne0;
+        for (int j = 1; j <= nsteps; j++) {
+            Te += dT;
+            Ne += dN;
+            const float q = div_rn_nz(Ne, len, rlen);
+            const float nsq = q * q;
+            if (!(nsq < w2)) break;
+            if (j >= jin) visit(cs + j * dir, Te, nsq);
+        }
+    }
+}
+
 template <typename T, int K, int TILE, int SEG, int NT, int CH>
 __global__ void __launch_bounds__(NT)
 k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ tile_off,
-                const int *__restrict__ items, const Segment<T> *__restrict__ segs,
+                const int2 *__restrict__ pairs, const Segment<T> *__restrict__ segs,
                 const LineSetup<T> *__restrict__ setup, const int64_t *__restrict__ slot_base, T max_val,
                 double *__restrict__ partial, int *__restrict__ flags)
 {
@@ -1179,11 +1238,11 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
     for (int first = beg + wave * CH; first < end; first += NW * CH) {
         const int n = min(CH, end - first);
         if (lane < n) {
-            const int sid = items[first + lane];
-            const Segment<T> sg = segs[sid];
-            const LineSetup<T> L = setup[sg.line];
-            hdr[wave][lane].seg = sid;
-            hdr[wave][lane].line = sg.line;
+            const int2 it = pairs[first + lane];
+            const Segment<T> sg = segs[it.x];
+            const LineSetup<T> L = setup[it.y];
+            hdr[wave][lane].seg = it.x;
+            hdr[wave][lane].line = it.y;
             walk_columns<T, SEG>(L, sg, R + lane * SEG, flags);
         }
         wave_sync();
@@ -1200,13 +1259,24 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
                     const int a = L.axis;
                     const int c = pick(sg.c, a) + k * pick(L.st, a);
                     const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
-                    fill_column(L, cr, s_lo, s_hi, [&](int s, T Te, T Ne) {
-                        const T err = err_of(L, Te, Ne);
-                        if (!(err <= T(1))) return;
-                        const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
-                        const T gv = gt[(y - Y0) * TILE + (x - X0)];
-                        if (gv != T(0)) pixel_grad<T, K>(L, x, y, gv, max_val, acc);
-                    });
+                    const GradConst C = grad_const(L, max_val);
+                    bool done = false;
+                    if constexpr (sizeof(T) == 4) {
+                        if (L.safe) {
+                            adj_column_fast<K, TILE>(L, C, (const float *)gt, cr.s, cr.Te, cr.Ne, c, X0, Y0, s_lo,
+                                                     s_hi, acc);
+                            done = true;
+                        }
+                    }
+                    if (!done) {
+                        fill_column(L, cr, s_lo, s_hi, [&](int s, T Te, T Ne) {
+                            const T err = err_of(L, Te, Ne);
+                            if (!(err <= T(1))) return;
+                            const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
+                            const T gv = gt[(y - Y0) * TILE + (x - X0)];
+                            if (gv != T(0)) pixel_grad<K>(C, x, y, (double)gv, acc);
+                        });
+                    }
                 }
             }
             // the SEG lanes of one item hold its columns: reduce them in a fixed tree
@@ -1228,20 +1298,25 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
     }
 }
 
+// tiles each segment slot touches (0 for unused slots)
 template <typename T>
 __global__ void k_slot_count(const Segment<T> *__restrict__ segs, int64_t n_segs, int tile, int64_t *__restrict__ cnt)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_segs) cnt[i] = segs[i].hi / tile - segs[i].lo / tile + 1;
+    if (i < n_segs) cnt[i] = (segs[i].ncols > 0) ? segs[i].hi / tile - segs[i].lo / tile + 1 : 0;
 }
 
+// Per-line sums of the partials, in slot order (deterministic).  A line's
+// slots are [line_base[l], line_base[l] + segment_bound).
 template <typename T>
-__global__ void k_line_grad(const int *__restrict__ seg_off, const int64_t *__restrict__ slot_base,
-                            const double *__restrict__ partial, int64_t n_lines, T *__restrict__ grad_lines)
+__global__ void k_line_grad(const int64_t *__restrict__ line_base, const LineSetup<T> *__restrict__ setup, Geometry g,
+                            const int64_t *__restrict__ slot_base, const double *__restrict__ partial,
+                            int64_t n_lines, T *__restrict__ grad_lines)
 {
     int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= n_lines) return;
-    const int64_t s0 = slot_base[seg_off[l]], s1 = slot_base[seg_off[l + 1]];
+    const int ub = segment_bound(setup[l], g);
+    const int64_t s0 = slot_base[line_base[l]], s1 = slot_base[line_base[l] + ub];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int64_t s = s0; s < s1; s++)
 #pragma unroll
@@ -1372,7 +1447,7 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int kern
 template <typename T, typename I>
 static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs, const uint8_t *kernels,
                                 int kernel, int64_t n_lines, int64_t n_frames, int64_t H, int64_t W, int tile,
-                                int seg, hipStream_t st)
+                                int seg, hipStream_t st, int64_t *line_base = nullptr)
 {
     Geometry &g = P.g;
     g.n_frames = n_frames;
@@ -1417,7 +1492,8 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     {
         KernelTimer timer("segments_walk", st);
         k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                counters.ptr, P.tile_cnt.ptr, cnt2.ptr, P.flags.ptr);
+                                                                counters.ptr, P.tile_cnt.ptr, cnt2.ptr, P.flags.ptr,
+                                                                line_base);
     }
     CBX_HIP(hipGetLastError());
     k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, cnt2.ptr, total.ptr);
@@ -1727,16 +1803,16 @@ static void launch_adjoint(const T *grad, const SegmentPlan<T> &P, const int64_t
     constexpr int NT = 256;
     if (P.g.tile == 64 && P.g.seg == 16)
         k_adjoint_tiles<T, K, 64, 16, NT, 16><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-            grad, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+            grad, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
     else if (P.g.tile == 64)
         k_adjoint_tiles<T, K, 64, 8, NT, 32><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-            grad, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+            grad, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
     else if (P.g.seg == 16)
         k_adjoint_tiles<T, K, 32, 16, NT, 16><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-            grad, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+            grad, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
     else
         k_adjoint_tiles<T, K, 32, 8, NT, 32><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-            grad, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+            grad, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
 }
 
 template <typename T, typename I>
@@ -1753,7 +1829,11 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     CBX_HIP(hipMemsetAsync(grad_lines, 0, sizeof(T) * 5 * n_lines, st));
     if (n_frames * H * W == 0 || kernel == K_RECTANGULAR) return CBX_OK;
     SegmentPlan<T> P;
-    if ((rc = build_plan<T, I>(P, lines, idxs, kernel, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
+    DevBuf<int64_t> line_base;
+    CBX_HIP(line_base.alloc(n_lines, st));
+    if ((rc = build_plan_unordered<T, I>(P, lines, idxs, nullptr, kernel, n_lines, n_frames, H, W, tile, seg, st,
+                                         line_base.ptr)))
+        return rc;
     if (P.n_segs == 0) return CBX_OK;
     const int blk = 256;
     DevBuf<int64_t> scnt, sbase;
@@ -1773,8 +1853,8 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     default: launch_adjoint<T, K_TRIANGULAR>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
     }
     CBX_HIP(hipGetLastError());
-    k_line_grad<T><<<grid_for(n_lines, blk), blk, 0, st>>>(P.seg_off.ptr, sbase.ptr, partial.ptr, n_lines,
-                                                          grad_lines);
+    k_line_grad<T><<<grid_for(n_lines, blk), blk, 0, st>>>(line_base.ptr, P.setup.ptr, P.g, sbase.ptr, partial.ptr,
+                                                          n_lines, grad_lines);
     CBX_HIP(hipGetLastError());
     int fl = 0;
     CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -1897,7 +1977,7 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
                                                                          P.setup.ptr, pl->counters.ptr, P.flags.ptr);
     k_walk_lines<float><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
                                                                 pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
-                                                                P.flags.ptr);
+                                                                P.flags.ptr, nullptr);
     k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, pl->cnt2.ptr,
                                                               pl->total.ptr);
     size_t bytes = pl->scan_bytes;
