@@ -122,7 +122,6 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
     s.Te = L.e0[0];
     s.Ne = L.e0[1];
 
-    bool open = false;
     int b_lo = 0, b_hi = 0, tl = 0, nem = 0, smin = 0, smax = 0;
     double maxN = 0.0;
     Segment<T> cur;
@@ -141,35 +140,69 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
     // every step moves the walk; a walk takes at most one short-axis-only
     // step per long-axis step, so this budget is never reached by a valid line
     const int64_t budget = 4 * ((int64_t)abs(pick(L.m1, a) - pick(L.m0, a)) + abs(pick(L.m1, b) - pick(L.m0, b))) + 64;
+    // Loop structure: an outer loop over segments and an inner loop over the
+    // steps of one SEG-column block, so that the lanes of a wave close their
+    // segments together (a close inside the step loop would run, with a few
+    // lanes active, on almost every step).  The steps and decisions are those
+    // of one plain loop over the walk.
     wait_vm_loads();
-    for (int64_t it = 0;; it++) {
+    int64_t it = 0;
+    bool stop = false;
+    // columns before the frame (the walk is monotone along the long axis)
+    for (;; it++) {
         if (it == budget) {
             atomicOr(flags, FLAG_STUCK);
+            stop = true;
             break;
         }
-        if (walk_done(L, s)) break;
+        if (walk_done(L, s)) {
+            stop = true;
+            break;
+        }
         const int c = pick(s.c, a);
-        if ((sa > 0) ? (c >= da) : (c < 0)) break; // no in-frame column is left
+        if ((sa > 0) ? (c >= da) : (c < 0)) { // no in-frame column is left
+            stop = true;
+            break;
+        }
+        if (c >= 0 && c < da) break; // the first segment starts here
         bool nx, ny;
         walk_next(L, s.Ne, nx, ny);
-        if (c >= 0 && c < da) {
-            if (!open || c < b_lo || c >= b_hi) {
-                if (open) close();
-                open = true;
-                b_lo = (c / g.seg) * g.seg;
-                b_hi = min(b_lo + g.seg, da);
-                tl = c / g.tile;
-                nem = 0;
-                maxN = 0.0;
-                smin = INT_MAX;
-                smax = INT_MIN;
-                cur.line = (int32_t)l;
-                cur.c[0] = s.c[0];
-                cur.c[1] = s.c[1];
-                cur.Te = s.Te;
-                cur.Ne = s.Ne;
-                cur.ncols = (sa > 0) ? (b_hi - c) : (c - b_lo + 1);
+        walk_step(L, s, nx, ny);
+    }
+    while (!stop) {
+        // open a segment at the current (in-frame) column
+        const int c0 = pick(s.c, a);
+        b_lo = (c0 / g.seg) * g.seg;
+        b_hi = min(b_lo + g.seg, da);
+        tl = c0 / g.tile;
+        nem = 0;
+        maxN = 0.0;
+        smin = INT_MAX;
+        smax = INT_MIN;
+        cur.line = (int32_t)l;
+        cur.c[0] = s.c[0];
+        cur.c[1] = s.c[1];
+        cur.Te = s.Te;
+        cur.Ne = s.Ne;
+        cur.ncols = (sa > 0) ? (b_hi - c0) : (c0 - b_lo + 1);
+        for (;; it++) {
+            if (it == budget) {
+                atomicOr(flags, FLAG_STUCK);
+                stop = true;
+                break;
             }
+            if (walk_done(L, s)) {
+                stop = true;
+                break;
+            }
+            const int c = pick(s.c, a);
+            if ((sa > 0) ? (c >= da) : (c < 0)) {
+                stop = true;
+                break;
+            }
+            if (c < b_lo || c >= b_hi) break; // the next block: close, then open there
+            bool nx, ny;
+            walk_next(L, s.Ne, nx, ny);
             if ((a == 0) ? nx : ny) {
                 nem++;
                 const int sb = pick(s.c, b);
@@ -177,10 +210,10 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
                 smax = max(smax, sb);
                 maxN = fmax(maxN, fabs((double)s.Ne));
             }
+            walk_step(L, s, nx, ny);
         }
-        walk_step(L, s, nx, ny);
+        close();
     }
-    if (open) close();
 }
 
 __device__ __forceinline__ int64_t tile_index(const Geometry &g, int frame, int axis, int tl, int tb)
