@@ -504,8 +504,8 @@ k_band_final(In in, BandGeom g, const int32_t *__restrict__ lab, int32_t *__rest
 //                     length to its root, k_run_count counts kept roots.)
 //   R6 scan           first label of every row.
 //   R7 k_run_rank     each root gets its label, stored as ~label in par.
-//   R8 k_run_label    each run gets its root's ~label.
-//   R9 k_run_final    one wave per row: every pixel's label (0 if off).
+//   R8 k_run_final    one wave per row: every pixel's label (0 if off):
+//                     ~par[run] for a root, ~par[par[run]] for any other run.
 // HBM traffic: the input once, the labels once; the bit rows (W/8 bytes per
 // row) and the run arrays are small enough to stay in the caches.
 // ---------------------------------------------------------------------------
@@ -517,6 +517,7 @@ struct RunGeom {
 };
 
 constexpr int RUN_NT = 256;
+typedef int v4i __attribute__((ext_vector_type(4)));
 constexpr int RUN_ROWS = RUN_NT / 64; // rows per workgroup, one per wave
 
 __device__ __forceinline__ uint64_t le_mask(int b) { return (b >= 63) ? ~0ull : ((2ull << b) - 1ull); }
@@ -688,7 +689,7 @@ __device__ __forceinline__ void run_contacts(const RunGeom &g, const uint64_t *_
 // are unioned in global memory.  Contacts across band borders are left to
 // k_run_border.
 constexpr int RUN_BAND = 32;
-constexpr int RUN_LCAP = 8192;
+constexpr int RUN_LCAP = 4096;
 
 __global__ void __launch_bounds__(RUN_NT)
 k_run_band(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff, int32_t *par)
@@ -831,19 +832,6 @@ k_run_rank(RunGeom g, const int32_t *__restrict__ roff, const int32_t *__restric
 }
 
 __global__ void __launch_bounds__(RUN_NT)
-k_run_label(RunGeom g, const int32_t *__restrict__ roff, int32_t *par)
-{
-    const int lane = threadIdx.x & 63;
-    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
-    if (row >= g.rows) return;
-    const int32_t off = roff[row], n = roff[row + 1] - off;
-    for (int k = lane; k < n; k += 64) {
-        const int32_t v = par[off + k];
-        if (v >= 0) par[off + k] = par[v]; // a root's entry already holds ~label
-    }
-}
-
-__global__ void __launch_bounds__(RUN_NT)
 k_run_final(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff,
             const int32_t *__restrict__ par, int32_t *__restrict__ out, int vec)
 {
@@ -877,10 +865,16 @@ k_run_final(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restr
             if (nib) {
 #pragma unroll
                 for (int i = 0; i < 4; i++)
-                    if ((nib >> i) & 1u) v[i] = ~par[cbase + pj + __popcll(sj & le_mask(b0 + i)) - 1];
+                    if ((nib >> i) & 1u) {
+                        // a root holds ~label, any other run its root
+                        const int32_t p = par[cbase + pj + __popcll(sj & le_mask(b0 + i)) - 1];
+                        v[i] = (p < 0) ? ~p : ~par[p];
+                    }
             }
             if (vec && x0 + 3 < g.W) {
-                *reinterpret_cast<int4 *>(orow + x0) = make_int4(v[0], v[1], v[2], v[3]);
+                // write-once output: non-temporal, not kept in the caches
+                const v4i q = {v[0], v[1], v[2], v[3]};
+                __builtin_nontemporal_store(q, reinterpret_cast<v4i *>(orow + x0));
             } else {
 #pragma unroll
                 for (int i = 0; i < 4; i++)
@@ -942,7 +936,6 @@ static int label_runs(In in, int ndim, const int64_t *shape, int prev, int64_t n
     }
     CBX_HIP(exclusive_scan_i32(kcnt.ptr, koff.ptr, R + 1, st));
     k_run_rank<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, koff.ptr, par.ptr, sizes ? size.ptr : nullptr, npts);
-    k_run_label<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr);
     const int vec = (g.W % 4 == 0 && ((uintptr_t)out & 15) == 0) ? 1 : 0;
     k_run_final<<<grid, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr, out, vec);
     CBX_HIP(hipGetLastError());
