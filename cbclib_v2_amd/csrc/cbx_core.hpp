@@ -95,8 +95,10 @@ struct alignas(16) LineSetup {
     T len, rlen;   // |t| (float sqrt of the float magnitude) and RN(1/len)
     T LL;          // RN(len * len) (TangentError::error_at)
     T w, w2;       // width and RN(w * w)
+    float nlim;    // float lines: |Ne| < nlim exactly when RN(RN(Ne / len)^2) < w2 (band_limit)
     double rw2;    // RN64(1 / (double)w2)
     float rw2f;    // RN32(1 / w2) for the fast error estimate
+    float ea;      // float lines: RN32(1 / (len^2 w2)), the fast path's error scale (err_fast)
     int32_t m1[2]; // walk end (exclusive, "any coordinate equal" test)
     int32_t m0[2]; // walk start
     int32_t frame;
@@ -108,6 +110,26 @@ struct alignas(16) LineSetup {
     T e0[2];       // tangent and normal errors at m0
     T p0[2];       // first end point (for the adjoint's closed-form derivatives)
 };
+
+// Smallest float x >= 0 with !(RN(RN(x / len)^2) < w2) (the normal-error test
+// of nsq_of).  That square is non-decreasing in |x| and RN is symmetric, so
+// |Ne| < nlim decides the fill loops' band test exactly.  Bisection over the
+// bit patterns of the non-negative floats, which are ordered like their values.
+__device__ __forceinline__ float band_limit(const LineSetup<float> &L)
+{
+    auto out = [&](uint32_t bits) {
+        const float q = div_rn(__uint_as_float(bits), L.len, L.rlen);
+        return !(q * q < L.w2);
+    };
+    if (out(0u)) return 0.0f;
+    uint32_t lo = 0u, hi = 0x7f800000u;   // out(lo) false; out(+inf) true (a NaN or +inf square)
+    while (hi - lo > 1u) {
+        const uint32_t mid = lo + ((hi - lo) >> 1);
+        if (out(mid)) hi = mid;
+        else lo = mid;
+    }
+    return __uint_as_float(hi);
+}
 
 // Builds the setup exactly as BresenhamPlotter's constructor does
 // (bresenham.hpp:243-253, 303-347).  Returns a FLAG_* on a rejected line.
@@ -168,9 +190,18 @@ __device__ __forceinline__ int line_setup(const T *ln, LineSetup<T> &L)
     // With every end point coordinate 0 or at least 2^-20 in magnitude (and
     // w >= 2^-20), each walk error is 0 or far above the subnormal range, so
     // the reciprocal-based division needs no tiny-quotient guard.
+    // Widths up to 2^20 also keep every square in err_fast finite and normal
+    // near the threshold.
     const T lim = T(1.0 / 1048576.0);
     auto ok = [&](T v) { return v == T(0) || m_abs(v) >= lim; };
-    L.safe = (int8_t)(ok(x0) && ok(y0) && ok(x1) && ok(y1) && w >= lim);
+    L.safe = (int8_t)(ok(x0) && ok(y0) && ok(x1) && ok(y1) && w >= lim && w <= T(1048576.0));
+    if constexpr (sizeof(T) == 4) {
+        L.ea = (float)(1.0 / ((double)len * (double)len * (double)L.w2));
+        L.nlim = band_limit(L);
+    } else {
+        L.ea = 0.0f;
+        L.nlim = 0.0f;
+    }
     return 0;
 }
 
@@ -184,6 +215,11 @@ __device__ __forceinline__ int line_setup(const T *ln, LineSetup<T> &L)
 // and, being a count, also waits on every iteration for memory operations the
 // loop itself issued (returning atomics, stores).
 __device__ __forceinline__ void wait_vm_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// s_waitcnt lgkmcnt(0) (vmcnt and expcnt left at their maxima): the same for
+// LDS and scalar loads, before a loop that issues LDS atomics, whose pending
+// count would otherwise make the loop wait for its own previous atomic.
+__device__ __forceinline__ void wait_lds_loads() { __builtin_amdgcn_s_waitcnt(0xC07F); }
 
 template <typename T>
 struct WalkState {
@@ -271,6 +307,21 @@ __device__ __forceinline__ double err_exact(const LineSetup<double> &L, double a
     const double qt2 = qt * qt;
     const double e = qt2 + nsq;
     return div_rn(e, L.w2, L.rw2);
+}
+
+// Error of a pixel in the float fast path (L.safe lines), for the decision
+// e <= 1 and the fast profiles.  Estimate: (at^2 + Ne^2) * ea, i.e. the exact
+// error's (qt^2 + nsq) / w2 with the divisions by len^2 and w2 folded into one
+// product: within ~10 ulp (< 1e-6 relative) of the exact error, whose
+// roundings it skips.  Within 4e-6 of the threshold the exact error is
+// recomputed, so e <= 1 decides as the reference does.  tangent_at as
+// max(-Te, Te - LL, 0): the errors are finite, and a -0 overshoot squares to +0.
+__device__ __forceinline__ float err_fast(const LineSetup<float> &L, float LL, float ea, float Te, float Ne)
+{
+    const float at = __builtin_fmaxf(__builtin_fmaxf(-Te, Te - LL), 0.0f);
+    const float e = __builtin_fmaf(at, at, Ne * Ne) * ea;
+    if (__builtin_fabsf(e - 1.0f) <= 4e-6f) return err_exact(L, at, nsq_of(L, Ne));
+    return e;
 }
 
 template <typename T>
@@ -367,6 +418,49 @@ __device__ __forceinline__ float profile_fast(float u)
         return (m < 0.0f) ? 0.0f : m;
     }
 }
+
+// The fast profile as c * shape(u) with c = s * (the profile's constant
+// factor) computed once per column (fast_scale) and shape evaluated per pixel
+// (fast_value): s * K(sqrt(u)) to a few ulp.  K_MIXED picks the profile kid.
+template <int K>
+__device__ __forceinline__ float fast_scale(int kid, float s)
+{
+    const int k = (K == K_MIXED) ? kid : K;
+    return (k == K_BIWEIGHT) ? s * 0.9375f
+         : (k == K_GAUSSIAN) ? s * 0.3989422804014327f
+         : (k == K_PARABOLIC) ? s * 0.75f
+                              : s;
+}
+
+template <int K>
+__device__ __forceinline__ float fast_value(int kid, float u, float c)
+{
+    if constexpr (K == K_MIXED) {
+        switch (kid) {
+        case K_BIWEIGHT: return fast_value<K_BIWEIGHT>(kid, u, c);
+        case K_GAUSSIAN: return fast_value<K_GAUSSIAN>(kid, u, c);
+        case K_PARABOLIC: return fast_value<K_PARABOLIC>(kid, u, c);
+        case K_RECTANGULAR: return c;
+        default: return fast_value<K_TRIANGULAR>(kid, u, c);
+        }
+    } else if constexpr (K == K_BIWEIGHT) {
+        const float m = 1.0f - u;
+        return c * (m * m);
+    } else if constexpr (K == K_GAUSSIAN) {
+        return c * __builtin_amdgcn_exp2f(u * -6.4921276840f);   // exp(-4.5 u) = 2^(-4.5 log2(e) u)
+    } else if constexpr (K == K_PARABOLIC) {
+        return c * (1.0f - u);
+    } else if constexpr (K == K_RECTANGULAR) {
+        return c;
+    } else {
+        const float m = 1.0f - __builtin_amdgcn_sqrtf(u);
+        return (m < 0.0f) ? 0.0f : c * m;
+    }
+}
+
+// Profiles whose fast value is positive on 0 <= u <= 1 for s > 0.
+template <int K>
+constexpr bool fast_positive() { return K == K_GAUSSIAN || K == K_RECTANGULAR; }
 
 // Profile chosen at run time (per-line profiles).
 __device__ __forceinline__ float profile_fast_dyn(int k, float u)

@@ -508,13 +508,16 @@ struct AccType<float, 2> {
 
 // Adds one pixel value to the LDS tile.  For OV = 2, v is the profile value
 // already scaled by the round's 2^F; exact zeros only mark the pixel touched.
-template <int OV, typename Acc, typename V>
+// POS: v is known to be positive (and, for OV = 2, at least 1).
+template <int OV, bool POS = false, typename Acc, typename V>
 __device__ __forceinline__ void tile_put(Acc *img, uint32_t *zb, int idx, V v)
 {
     if constexpr (OV == 0) {
         atomicAdd(&img[idx], (Acc)v);
     } else if constexpr (OV == 1) {
         if (v == v) atomicMax(&img[idx], ord_enc(v));
+    } else if constexpr (POS) {
+        atomicAdd(&img[idx], __float2uint_rn((float)v));
     } else {
         if (v > 0.0f) atomicAdd(&img[idx], max(1u, __float2uint_rn((float)v)));
         else atomicOr(&zb[idx >> 5], 1u << (idx & 31));
@@ -524,45 +527,80 @@ __device__ __forceinline__ void tile_put(Acc *img, uint32_t *zb, int idx, V v)
 // ---------------------------------------------------------------------------
 // 4. tile kernel (image mode)
 // ---------------------------------------------------------------------------
-// The fills of one column half for the float fast path: constants held in
-// registers, the tile window turned into a trip count, no division guard
-// (only for lines with L.safe).  Accumulates into the LDS tile; vscale is
-// max_val (OV 0, 1) or the fixed-point scale 2^F (OV 2).
-template <int K, int OV, typename Acc>
-__device__ __forceinline__ void fill_half_fast(Acc *img, uint32_t *zb, const LineSetup<float> &L, int cs, float Te,
-                                               float Ne, int half, int cbase, int sstride, int s_lo, int s_hi,
-                                               float vscale)
+// Per-item record in LDS, written by the phase-1 thread that replays the
+// item's walk (it holds the line's setup in registers) and read by the
+// phase-2 threads of its columns: the item's line and first long-axis
+// coordinate, and for float lines the fast path's constants, so the fills
+// need no global load (the exact error near the threshold reads the setup).
+struct alignas(16) FillRec {
+    float LL, ea, nlim;
+    float tb, nb;     // tangent and normal components along the short axis b
+    int32_t c0;       // long-axis coordinate of the item's first column
+    int32_t line;
+    int32_t meta;     // FR_* bits, profile id in bits 8-15
+};
+constexpr int FR_AXIS = 1, FR_SA = 2, FR_SB = 4, FR_SAFE = 8;
+
+template <typename T>
+__device__ __forceinline__ FillRec fill_rec(const LineSetup<T> &L, int c0, int line)
 {
-    const float len = L.len, rlen = L.rlen, LL = L.LL, w2 = L.w2, rw2f = L.rw2f;
-    const int b = 1 - L.axis, kid = L.kid;
-    const int sb = pick(L.st, b);
+    FillRec r;
+    const int a = L.axis, b = 1 - a;
+    r.c0 = c0;
+    r.line = line;
+    r.meta = a | (pick(L.st, a) > 0 ? FR_SA : 0) | (pick(L.st, b) > 0 ? FR_SB : 0) |
+             (sizeof(T) == 4 && L.safe ? FR_SAFE : 0) | ((int)L.kid << 8);
+    r.LL = (float)L.LL;
+    r.ea = L.ea;
+    r.nlim = L.nlim;
+    r.tb = (float)pick(L.t, b);
+    r.nb = (float)pick(L.nv, b);
+    return r;
+}
+
+// The fills of one column half for the float fast path (lines with L.safe):
+// the band test |Ne| < nlim, the one-product error estimate (err_fast), the
+// profile's constant folded into the column's value scale and the tile window
+// turned into a skip count and a trip count, with the LDS index stepped per
+// pixel.  The errors advance by the same FP32 additions as the reference's
+// loop.  Lg is the line's setup in global memory, read only for exact errors
+// near the threshold.  vscale is max_val (OV 0, 1) or the fixed-point scale
+// 2^F (OV 2: F >= 20, so positive profiles give values of at least 1).
+template <int K, int OV, typename Acc>
+__device__ __forceinline__ void fill_half_fast(Acc *img, uint32_t *zb, const FillRec &R,
+                                               const LineSetup<float> &Lg, int cs, float Te, float Ne, int half,
+                                               int cbase, int sstride, int s_lo, int s_hi, float vscale)
+{
+    constexpr bool POS = (OV == 2) && fast_positive<K>();
+    const float LL = R.LL, ea = R.ea, nlim = R.nlim;
+    const int kid = R.meta >> 8;
+    const int sb = (R.meta & FR_SB) ? 1 : -1;
     const int dir = half ? -sb : sb;
-    const float tb = pick(L.t, b), nb = pick(L.nv, b);
-    const float dT = (dir > 0) ? tb : -tb, dN = (dir > 0) ? nb : -nb;
-    auto put = [&](int s, float T, float nsq) {
-        const float a0 = -T, a1 = T - LL;
-        float at = (a0 < a1) ? a1 : a0;
-        at = (at < 0.0f) ? 0.0f : at;
-        const float qt = at * rlen;
-        float e = __builtin_fmaf(qt, qt, nsq) * rw2f;
-        if (__builtin_fabsf(e - 1.0f) <= 4e-6f) e = err_exact(L, at, nsq);
-        if (e <= 1.0f) tile_put<OV>(img, zb, cbase + (s - s_lo) * sstride, vscale * profile_fast_of<K>(kid, e));
+    const float dT = (dir > 0) ? R.tb : -R.tb, dN = (dir > 0) ? R.nb : -R.nb;
+    const float c = fast_scale<K>(kid, vscale);
+    auto put = [&](int idx, float T, float N) {
+        const float e = err_fast(Lg, LL, ea, T, N);
+        if (e <= 1.0f) tile_put<OV, POS>(img, zb, idx, fast_value<K>(kid, e, c));
     };
-    if (half == 0 && cs >= s_lo && cs < s_hi) {
-        const float q = div_rn_nz(Ne, len, rlen);
-        put(cs, Te, q * q);
-    }
+    wait_lds_loads();
+    if (half == 0 && cs >= s_lo && cs < s_hi) put(cbase + (cs - s_lo) * sstride, Te, Ne);
     // steps to the last in-window pixel in the direction of travel, and the
     // first step that lands inside the window
     const int nsteps = (dir > 0) ? (s_hi - 1 - cs) : (cs - s_lo);
-    const int jin = (dir > 0) ? (s_lo - cs) : (cs - s_hi + 1);
-    for (int j = 1; j <= nsteps; j++) {
+    if (nsteps < 1) return;
+    const int jin = max(1, (dir > 0) ? (s_lo - cs) : (cs - s_hi + 1));
+    for (int j = 1; j < jin; j++) {
         Te += dT;
         Ne += dN;
-        const float q = div_rn_nz(Ne, len, rlen);
-        const float nsq = q * q;
-        if (!(nsq < w2)) break;
-        if (j >= jin) put(cs + j * dir, Te, nsq);
+        if (!(__builtin_fabsf(Ne) < nlim)) return;
+    }
+    const int step = dir * sstride;
+    int idx = cbase + (cs + jin * dir - s_lo) * sstride;
+    for (int j = jin; j <= nsteps; j++, idx += step) {
+        Te += dT;
+        Ne += dN;
+        if (!(__builtin_fabsf(Ne) < nlim)) return;
+        put(idx, Te, Ne);
     }
 }
 
@@ -606,8 +644,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     __shared__ int32_t rs[CH * SEG];
     __shared__ T rT[CH * SEG];
     __shared__ T rN[CH * SEG];
-    __shared__ int32_t hline[CH];
-    __shared__ int32_t hc0[CH];
+    __shared__ FillRec frec[CH];
 
     const int tile = blockIdx.x;
     // a plan's draw reports errors afterwards: leave `out` untouched then
@@ -686,9 +723,8 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                     L = setup[it.y];
                 }
                 const int a = L.axis, sa = pick(L.st, a);
-                hline[tid] = sg.line;
                 const int c0 = pick(sg.c, a);
-                hc0[tid] = c0;
+                frec[tid] = fill_rec(L, c0, sg.line);
                 int32_t *ps = &rs[tid * SEG];
                 T *pT = &rT[tid * SEG];
                 T *pN = &rN[tid * SEG];
@@ -723,20 +759,22 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 const int k = rem >> 1, half = rem & 1;
                 const int cs = rs[i * SEG + k];
                 if (cs == INT_MIN) continue;
-                const LineSetup<T> L = setup[hline[i]];
-                const int a = L.axis, b = 1 - a;
-                const int c = hc0[i] + k * pick(L.st, a);
+                const FillRec R = frec[i];
+                const int a = R.meta & FR_AXIS, b = 1 - a;
+                const int c = R.c0 + ((R.meta & FR_SA) ? k : -k);
                 const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
                 // LDS stride of the short axis and offset of the long-axis column
                 const int sstride = (a == 0) ? TILE : 1;
                 const int cbase = (a == 0) ? (c - X0) : (c - Y0) * TILE;
                 T Te = rT[i * SEG + k], Ne = rN[i * SEG + k];
                 if constexpr (FAST && sizeof(T) == 4) {
-                    if (L.safe) {
-                        fill_half_fast<K, OV>(img, zb, L, cs, Te, Ne, half, cbase, sstride, s_lo, s_hi, (float)vscale);
+                    if (R.meta & FR_SAFE) {
+                        fill_half_fast<K, OV>(img, zb, R, setup[R.line], cs, Te, Ne, half, cbase, sstride, s_lo,
+                                              s_hi, (float)vscale);
                         continue;
                     }
                 }
+                const LineSetup<T> L = setup[R.line];
                 auto put = [&](int s, T Te_, T nsq) {
                     T v;
                     if (!pixel_eval<FAST, K>(L, Te_, nsq, vscale, v)) return;
@@ -1192,31 +1230,22 @@ __device__ __forceinline__ void pixel_grad(const GradConst &C, int x, int y, dou
 }
 
 // The fills of one column for the float fast path (both halves and the
-// centre), with the forward's inclusion decisions (fast error estimate, exact
-// recompute within 4e-6 of the edge); lines with L.safe only.
+// centre), with the forward's band test and inclusion decisions (err_fast);
+// lines with L.safe only.
 template <int K, int TILE>
 __device__ __forceinline__ void adj_column_fast(const LineSetup<float> &L, const GradConst &C, const float *gt, int cs,
                                                 float Te0, float Ne0, int c, int X0, int Y0, int s_lo, int s_hi,
                                                 double *acc)
 {
-    const float len = L.len, rlen = L.rlen, LL = L.LL, w2 = L.w2, rw2f = L.rw2f;
+    const float LL = L.LL, ea = L.ea, nlim = L.nlim;
     const int a = L.axis, b = 1 - a;
-    auto visit = [&](int s, float T, float nsq) {
-        const float a0 = -T, a1 = T - LL;
-        float at = (a0 < a1) ? a1 : a0;
-        at = (at < 0.0f) ? 0.0f : at;
-        const float qt = at * rlen;
-        float e = __builtin_fmaf(qt, qt, nsq) * rw2f;
-        if (__builtin_fabsf(e - 1.0f) <= 4e-6f) e = err_exact(L, at, nsq);
-        if (!(e <= 1.0f)) return;
+    auto visit = [&](int s, float T, float N) {
+        if (!(err_fast(L, LL, ea, T, N) <= 1.0f)) return;
         const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
         const float gv = gt[(y - Y0) * TILE + (x - X0)];
         if (gv != 0.0f) pixel_grad<K>(C, x, y, (double)gv, acc);
     };
-    if (cs >= s_lo && cs < s_hi) {
-        const float q = div_rn_nz(Ne0, len, rlen);
-        visit(cs, Te0, q * q);
-    }
+    if (cs >= s_lo && cs < s_hi) visit(cs, Te0, Ne0);
     const int sb = pick(L.st, b);
 #pragma unroll
     for (int half = 0; half < 2; half++) {
@@ -1228,10 +1257,8 @@ __device__ __forceinline__ void adj_column_fast(const LineSetup<float> &L, const
         for (int j = 1; j <= nsteps; j++) {
             Te += dT;
             Ne += dN;
-            const float q = div_rn_nz(Ne, len, rlen);
-            const float nsq = q * q;
-            if (!(nsq < w2)) break;
-            if (j >= jin) visit(cs + j * dir, Te, nsq);
+            if (!(__builtin_fabsf(Ne) < nlim)) break;
+            if (j >= jin) visit(cs + j * dir, Te, Ne);
         }
     }
 }
