@@ -614,6 +614,15 @@ __device__ __forceinline__ double fix_value(uint32_t acc, const uint32_t *zb, in
     return acc ? (double)acc * scale : (double)(max_val * 0.0f);
 }
 
+// fix_value for a pixel already in a register: o + acc * scale (or + zero,
+// the value of exact zeros, when only zeros touched it: zbit set), or o
+// untouched.
+__device__ __forceinline__ float fold_fix(float o, uint32_t acc, uint32_t zbit, double scale, double zero)
+{
+    const double add = acc ? (double)acc * scale : zero;
+    return (acc | zbit) ? (float)((double)o + add) : o;
+}
+
 // Tile kernel.  The workgroup takes its tile's items ((segment, line) pairs)
 // in chunks of CH <= NT: phase 1 gives each of CH threads one item (the walk
 // replay, column records into LDS); phase 2 gives each thread one (item,
@@ -638,7 +647,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
 {
     using Acc = typename AccType<T, OV>::type;
     static_assert(OV != 2 || (FAST && sizeof(T) == 4), "fixed-point sums are the float fast path");
-    __shared__ Acc img[TILE * TILE];
+    __shared__ alignas(16) Acc img[TILE * TILE];
     __shared__ uint32_t zb[OV == 2 ? TILE * TILE / 32 : 1];
     static_assert(CH <= NT, "one phase-1 thread per item");
     __shared__ int32_t rs[CH * SEG];
@@ -647,10 +656,11 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     __shared__ FillRec frec[CH];
 
     const int tile = blockIdx.x;
-    // a plan's draw reports errors afterwards: leave `out` untouched then
-    if (*flags) return;
+    // (one round trip for the item range and the error flags: a plan's draw
+    // reports errors afterwards and leaves `out` untouched then)
     const int beg = tile_off[tile], end = tile_off[tile + 1];
-    if (beg == end) return;
+    const int fl = *flags;
+    if (fl || beg == end) return;
     const int per_frame = g.ntx * g.nty;
     const int64_t f = tile / per_frame;
     const int r = tile - (int)f * per_frame;
@@ -662,16 +672,8 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     constexpr int V = 4;
     const bool full = (X1 - X0 == TILE) && ((g.W & (V - 1)) == 0) && (((uintptr_t)out & 15) == 0);
 
-    // first chunk's item loads
-    Segment<T> sg0;
-    LineSetup<T> L0;
-    if (tid < min(CH, end - beg)) {
-        const int2 it = pairs[beg + tid];
-        sg0 = segs[it.x];
-        L0 = setup[it.y];
-    }
     // the tile's current values, for the flush (float sums, 256 threads:
-    // 4 float4 per thread)
+    // 4 float4 per thread), requested first
     constexpr bool PREF = (OV != 1) && sizeof(T) == 4 && NT == 256 && TILE == 64;
     constexpr int PV = PREF ? (TILE * TILE) / (NT * V) : 1;
     float4 pre[PV];
@@ -684,6 +686,14 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 if (y < Y1) pre[j] = *reinterpret_cast<const float4 *>(fo + (int64_t)y * g.W + x);
             }
         }
+    }
+    // first chunk's item loads (after the tile's loads, which do not wait on them)
+    Segment<T> sg0;
+    LineSetup<T> L0;
+    if (tid < min(CH, end - beg)) {
+        const int2 it = pairs[beg + tid];
+        sg0 = segs[it.x];
+        L0 = setup[it.y];
     }
 
     auto clear = [&]() {
@@ -803,18 +813,23 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
             const double scale = ldexp((double)max_val, -F);
             bool has;
             if (PREF && full) {
+                // all LDS reads first (4 pixels and their zero bits per read),
+                // then branch-free updates
+                const double zero = (double)((float)max_val * 0.0f);
+                uint4 acc[PV];
+                uint32_t zw[PV];
 #pragma unroll
                 for (int j = 0; j < PV; j++) {
                     const int i = (tid + j * NT) * V;
-                    const float mv = (float)max_val;
-                    double add = fix_value(img[i], zb, i, scale, mv, has);
-                    if (has) pre[j].x = (float)((double)pre[j].x + add);
-                    add = fix_value(img[i + 1], zb, i + 1, scale, mv, has);
-                    if (has) pre[j].y = (float)((double)pre[j].y + add);
-                    add = fix_value(img[i + 2], zb, i + 2, scale, mv, has);
-                    if (has) pre[j].z = (float)((double)pre[j].z + add);
-                    add = fix_value(img[i + 3], zb, i + 3, scale, mv, has);
-                    if (has) pre[j].w = (float)((double)pre[j].w + add);
+                    acc[j] = *reinterpret_cast<const uint4 *>(&img[i]);
+                    zw[j] = zb[i >> 5] >> (i & 31);
+                }
+#pragma unroll
+                for (int j = 0; j < PV; j++) {
+                    pre[j].x = fold_fix(pre[j].x, acc[j].x, zw[j] & 1u, scale, zero);
+                    pre[j].y = fold_fix(pre[j].y, acc[j].y, zw[j] & 2u, scale, zero);
+                    pre[j].z = fold_fix(pre[j].z, acc[j].z, zw[j] & 4u, scale, zero);
+                    pre[j].w = fold_fix(pre[j].w, acc[j].w, zw[j] & 8u, scale, zero);
                 }
             } else {
                 for (int i = tid; i < TILE * TILE; i += NT) {
