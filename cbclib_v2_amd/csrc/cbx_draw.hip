@@ -650,9 +650,9 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     __shared__ alignas(16) Acc img[TILE * TILE];
     __shared__ uint32_t zb[OV == 2 ? TILE * TILE / 32 : 1];
     static_assert(CH <= NT, "one phase-1 thread per item");
-    __shared__ int32_t rs[CH * SEG];
-    __shared__ T rT[CH * SEG];
-    __shared__ T rN[CH * SEG];
+    __shared__ int32_t rs[CH * SEG + CH];   // + one sink slot per phase-1 lane
+    __shared__ T rT[CH * SEG + CH];
+    __shared__ T rN[CH * SEG + CH];
     __shared__ FillRec frec[CH];
 
     const int tile = blockIdx.x;
@@ -718,7 +718,9 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
         }
         for (int first = rb; first < re; first += CH) {
             const int n = min(CH, re - first);
-            __syncthreads();
+            // (phase 1 writes only the item records: the first chunk needs no
+            // barrier after the clear)
+            if (first != beg) __syncthreads();
             // phase 1: replay the walk of one item from its checkpoint and
             // record the centre and errors of every emitted column
             if (tid < n) {
@@ -735,31 +737,31 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 const int a = L.axis, sa = pick(L.st, a);
                 const int c0 = pick(sg.c, a);
                 frec[tid] = fill_rec(L, c0, sg.line);
-                int32_t *ps = &rs[tid * SEG];
-                T *pT = &rT[tid * SEG];
-                T *pN = &rN[tid * SEG];
 #pragma unroll
-                for (int k = 0; k < SEG; k++) ps[k] = INT_MIN;
+                for (int k = 0; k < SEG; k++) rs[tid * SEG + k] = INT_MIN;
                 WalkState<T> s;
                 s.c[0] = sg.c[0];
                 s.c[1] = sg.c[1];
                 s.Te = sg.Te;
                 s.Ne = sg.Ne;
-                int it = 0;
-                for (; it < 4 * SEG + 16; it++) {
-                    if (walk_done(L, s)) break;
+                // a loop uniform over the wave: lanes whose walk has ended are
+                // masked by predicates and store into their sink slot
+                bool act = true;
+                for (int it = 0; it < 4 * SEG + 16; it++) {
+                    act = act && !walk_done(L, s);
                     const int k = (pick(s.c, a) - c0) * sa;
-                    if (k >= sg.ncols) break;
+                    act = act && k < sg.ncols;
+                    if (__ballot(act) == 0) break;
                     bool nx, ny;
                     walk_next(L, s.Ne, nx, ny);
-                    if ((a == 0) ? nx : ny) {
-                        ps[k] = pick(s.c, 1 - a);
-                        pT[k] = s.Te;
-                        pN[k] = s.Ne;
-                    }
-                    walk_step(L, s, nx, ny);
+                    const bool rec = act && ((a == 0) ? nx : ny);
+                    const int slot = rec ? tid * SEG + k : CH * SEG + tid;
+                    rs[slot] = pick(s.c, 1 - a);
+                    rT[slot] = s.Te;
+                    rN[slot] = s.Ne;
+                    walk_step(L, s, act && nx, act && ny);
                 }
-                if (it == 4 * SEG + 16) atomicOr(flags, FLAG_STUCK);
+                if (act) atomicOr(flags, FLAG_STUCK);   // still walking after the step budget
             }
             __syncthreads();
             // phase 2: one thread per (item, column, half); the centre goes
