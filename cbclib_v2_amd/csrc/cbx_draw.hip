@@ -32,6 +32,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstddef>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -292,6 +293,8 @@ k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, const uin
     if ((threadIdx.x & 63) == 0 && ub) atomicAdd(&counters[0], (unsigned long long)ub);
 }
 
+constexpr int WALK_RING = 4;
+
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
@@ -326,24 +329,65 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
     if (ub == 0 || (int64_t)(wbase + total) > slot_cap) return;
     const int64_t base = (int64_t)wbase + pre;
     int nseg = 0;
-    // a segment is stored when the next one closes (or after the walk), so
-    // the walk runs on while its count atomics are in flight
-    Segment<T> pending;
+    // Segments go through a per-thread ring of WALK_RING records in LDS and
+    // are written to memory WALK_RING at a time, all loads before all
+    // stores: a store's data registers cannot be reused before the store
+    // completes, so writing each segment from registers at its close would
+    // stall every close on the previous store.  A record's places (the
+    // returns of its count atomics) are filled in at the next close, so the
+    // walk runs on while the atomics are in flight.
+    // (ring words [slot][word][thread]: a record moves as 32-bit words with
+    // constant indices, so it stays in registers)
+    constexpr int NW = (int)(sizeof(Segment<T>) / 4);
+    static_assert(sizeof(Segment<T>) % 8 == 0, "records move as 64-bit pairs");
+    __shared__ uint32_t ring[WALK_RING][NW][256];
+    const int t = threadIdx.x;
+    int r0 = 0, r1 = 0;
+    auto put_ranks = [&](int slot) {
+        constexpr int RW = (int)(offsetof(Segment<T>, rank) / 4);
+        ring[slot][RW][t] = (uint32_t)r0;
+        ring[slot][RW + 1][t] = (uint32_t)r1;
+    };
+    auto flush = [&](int first, int cnt) {   // records first .. first + cnt - 1, ring slots 0 .. cnt - 1
+        uint32_t w[WALK_RING][NW];
+#pragma unroll
+        for (int j = 0; j < WALK_RING; j++)
+#pragma unroll
+            for (int k = 0; k < NW; k++)
+                if (j < cnt) w[j][k] = ring[j][k][t];
+#pragma unroll
+        for (int j = 0; j < WALK_RING; j++) {
+            if (j < cnt && first + j < ub) {
+                uint2 *dst = reinterpret_cast<uint2 *>(segs + base + first + j);
+#pragma unroll
+                for (int k = 0; k < NW / 2; k++) dst[k] = make_uint2(w[j][2 * k], w[j][2 * k + 1]);
+            }
+        }
+    };
     walk_segments(L, l, g, flags, [&](const Segment<T> &cur, int tl) {
-        if (nseg > 0 && nseg <= ub) segs[base + nseg - 1] = pending;
+        if (nseg > 0) {
+            put_ranks((nseg - 1) % WALK_RING);
+            if (nseg % WALK_RING == 0) flush(nseg - WALK_RING, WALK_RING);
+        }
+        uint32_t cw[NW];
+        __builtin_memcpy(cw, &cur, sizeof(cw));
+        const int slot = nseg % WALK_RING;
+#pragma unroll
+        for (int k = 0; k < NW; k++) ring[slot][k][t] = cw[k];
         // the counting atomics hand out each piece's place in its tile's list:
         // the first two tiles of a segment in class 1 (kept in the segment),
         // further ones in class 2 (placed by k_bin_slots)
-        pending = cur;
         const int tb0 = cur.lo / g.tile, tb1 = cur.hi / g.tile;
-        // straight-line, fixed destinations: nothing reads the returned
-        // places until the segment is stored
-        pending.rank[0] = atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb0)], 1);
-        if (tb1 > tb0) pending.rank[1] = atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb0 + 1)], 1);
+        r0 = atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb0)], 1);
+        if (tb1 > tb0) r1 = atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb0 + 1)], 1);
         for (int tb = tb0 + 2; tb <= tb1; tb++) atomicAdd(&tile_cnt2[tile_index(g, L.frame, L.axis, tl, tb)], 1);
         nseg++;
     });
-    if (nseg > 0 && nseg <= ub) segs[base + nseg - 1] = pending;
+    if (nseg > 0) {
+        put_ranks((nseg - 1) % WALK_RING);
+        const int done = ((nseg - 1) / WALK_RING) * WALK_RING;   // records flushed at closes
+        flush(done, nseg - done);
+    }
     if (nseg > ub) atomicOr(flags, FLAG_STUCK); // cannot happen: the bound holds for every walk
     for (int k = nseg; k < ub; k++) segs[base + k].ncols = 0;
 }
