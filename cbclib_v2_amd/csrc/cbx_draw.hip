@@ -1348,10 +1348,18 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
     const int X1 = min(X0 + TILE, g.W), Y1 = min(Y0 + TILE, g.H);
 
     const T *fg = grad + (f * g.H) * (int64_t)g.W;
-    for (int i = threadIdx.x; i < TILE * TILE; i += NT) {
+    // the tile of dLoss/dImage: every load issued before the LDS writes
+    static_assert((TILE * TILE) % NT == 0, "whole tile rows per thread pass");
+    constexpr int PER = TILE * TILE / NT;
+    T gv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; j++) {
+        const int i = threadIdx.x + j * NT;
         const int y = Y0 + i / TILE, x = X0 + (i % TILE);
-        gt[i] = (y < Y1 && x < X1) ? fg[(int64_t)y * g.W + x] : T(0);
+        gv[j] = (y < Y1 && x < X1) ? fg[(int64_t)y * g.W + x] : T(0);
     }
+#pragma unroll
+    for (int j = 0; j < PER; j++) gt[threadIdx.x + j * NT] = gv[j];
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
