@@ -863,13 +863,17 @@ k_run_final(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restr
             const uint32_t nib = (uint32_t)(wj >> b0) & 0xFu;
             int v[4] = {0, 0, 0, 0};
             if (nib) {
+                // a root holds ~label, any other run its root: two gathers,
+                // each issued for the 4 pixels before any result is used
+                int32_t q[4];
 #pragma unroll
                 for (int i = 0; i < 4; i++)
-                    if ((nib >> i) & 1u) {
-                        // a root holds ~label, any other run its root
-                        const int32_t p = par[cbase + pj + __popcll(sj & le_mask(b0 + i)) - 1];
-                        v[i] = (p < 0) ? ~p : ~par[p];
-                    }
+                    q[i] = ((nib >> i) & 1u) ? par[cbase + pj + __popcll(sj & le_mask(b0 + i)) - 1] : -1;
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    if (q[i] >= 0) q[i] = par[q[i]];
+#pragma unroll
+                for (int i = 0; i < 4; i++) v[i] = ((nib >> i) & 1u) ? ~q[i] : 0;
             }
             if (vec && x0 + 3 < g.W) {
                 // write-once output: non-temporal, not kept in the caches
