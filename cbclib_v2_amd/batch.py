@@ -33,6 +33,7 @@ __all__ = ['Plan', 'GraphDraw', 'tune', 'CANDIDATES']
 # launch shapes the autotuner tries: (tile, seg, threads, chunk)
 CANDIDATES: Tuple[Tuple[int, int, int, int], ...] = (
     (64, 8, 256, 64), (64, 16, 256, 64), (64, 8, 128, 64), (32, 8, 256, 64), (32, 16, 256, 64), (32, 8, 128, 64),
+    (64, 8, 256, 256), (64, 16, 256, 256),
 )
 _tuned: Dict[tuple, Dict[str, Any]] = {}
 

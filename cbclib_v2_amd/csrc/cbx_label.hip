@@ -213,22 +213,24 @@ struct BandGeom {
     int32_t prev;            // bit 0: (0,-1), 1: (-1,-1), 2: (-1,0), 3: (-1,+1)
 };
 
-__device__ __forceinline__ int lds_find(volatile int32_t *par, int i)
+// (relaxed workgroup-scope atomic loads: every step re-reads the entry,
+// without the waits for all outstanding memory operations that volatile
+// accesses get)
+__device__ __forceinline__ int lds_find(int32_t *par, int i)
 {
-    int p = par[i];
+    int p = __hip_atomic_load(&par[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     while (p != i) {
         i = p;
-        p = par[i];
+        p = __hip_atomic_load(&par[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     return i;
 }
 
 __device__ __forceinline__ void lds_union(int32_t *par, int a, int b)
 {
-    volatile int32_t *vp = par;
     while (true) {
-        a = lds_find(vp, a);
-        b = lds_find(vp, b);
+        a = lds_find(par, a);
+        b = lds_find(par, b);
         if (a == b) return;
         if (a < b) {
             const int t = a;
@@ -631,6 +633,67 @@ k_run_init(RunGeom g, const int32_t *__restrict__ roff, int32_t *__restrict__ pa
 // Contacts between row y and row y-1 (same frame) for one wave: calls
 // join(a, b) with the run ids of each contact pair that the event rule
 // keeps (a run of row y, b of row y-1; b < a).
+// The words a contact pass reads for one 64-word chunk of row y: row y's
+// word w and w-1, row y-1's words w-1, w, w+1 (0 outside the row).
+struct RowChunk {
+    uint64_t cw, cwl, pw, pwl, pwr;
+};
+
+__device__ __forceinline__ RowChunk load_chunk(const uint64_t *__restrict__ cur, const uint64_t *__restrict__ up,
+                                               int nw, int w)
+{
+    const bool act = w < nw;
+    RowChunk r;
+    r.cw = act ? cur[w] : 0;
+    r.cwl = (act && w > 0) ? cur[w - 1] : 0;
+    r.pw = act ? up[w] : 0;
+    r.pwl = (act && w > 0) ? up[w - 1] : 0;
+    r.pwr = (act && w + 1 < nw) ? up[w + 1] : 0;
+    return r;
+}
+
+// Contacts of one chunk (see run_contacts); ccarry / pcarry count the runs of
+// rows y and y-1 in the chunks before it.
+template <typename Join>
+__device__ __forceinline__ void chunk_contacts(const RunGeom &g, const RowChunk &r, int32_t cbase, int32_t pbase,
+                                               int &ccarry, int &pcarry, int lane, Join join)
+{
+    const uint64_t cw = r.cw, pw = r.pw, pwl = r.pwl, pwr = r.pwr;
+    const uint64_t cst = run_starts(cw, r.cwl), pst = run_starts(pw, pwl);
+    const int cn = __popcll(cst), pn = __popcll(pst);
+    const int cpre = ccarry + wave_excl_scan(cn, lane), ppre = pcarry + wave_excl_scan(pn, lane);
+#pragma unroll
+    for (int d = -1; d <= 1; d++) {
+        if (!(g.prev & (d < 0 ? 2 : (d == 0 ? 4 : 8)))) continue;
+        // sp / sps: bit p = pixel / run start of row y-1 at p + d
+        uint64_t sp, sps;
+        if (d == 0) {
+            sp = pw;
+            sps = pst;
+        } else if (d > 0) {
+            sp = (pw >> 1) | (pwr << 63);
+            sps = (pst >> 1) | ((pwr & ~(pw >> 63) & 1ull) << 63);
+        } else {
+            sp = (pw << 1) | (pwl >> 63);
+            sps = (pst << 1) | ((pwl >> 63) & ~(pwl >> 62) & 1ull);
+        }
+        uint64_t ev = cw & sp & (cst | sps);
+        while (ev) {
+            const int p = __builtin_ctzll(ev);
+            ev &= ev - 1;
+            const uint64_t le = le_mask(p);
+            const int32_t a = cbase + cpre + __popcll(cst & le) - 1;
+            // runs of row y-1 starting at or before p + d
+            int nb = ppre + __popcll(pst & le);
+            if (d > 0) nb += (int)((sps >> p) & 1ull);
+            if (d < 0) nb -= (int)((pst >> p) & 1ull);
+            join(a, pbase + nb - 1);
+        }
+    }
+    ccarry += wave_sum(cn);
+    pcarry += wave_sum(pn);
+}
+
 template <typename Join>
 __device__ __forceinline__ void run_contacts(const RunGeom &g, const uint64_t *__restrict__ bits,
                                              const int32_t *__restrict__ roff, int64_t row, int lane, Join join)
@@ -639,46 +702,8 @@ __device__ __forceinline__ void run_contacts(const RunGeom &g, const uint64_t *_
     const uint64_t *cur = bits + row * nw, *up = cur - nw;
     const int32_t cbase = roff[row], pbase = roff[row - 1];
     int ccarry = 0, pcarry = 0;
-    for (int c0 = 0; c0 < nw; c0 += 64) {
-        const int w = c0 + lane;
-        const bool act = w < nw;
-        const uint64_t cw = act ? cur[w] : 0, cwl = (act && w > 0) ? cur[w - 1] : 0;
-        const uint64_t pw = act ? up[w] : 0, pwl = (act && w > 0) ? up[w - 1] : 0;
-        const uint64_t pwr = (act && w + 1 < nw) ? up[w + 1] : 0;
-        const uint64_t cst = run_starts(cw, cwl), pst = run_starts(pw, pwl);
-        const int cn = __popcll(cst), pn = __popcll(pst);
-        const int cpre = ccarry + wave_excl_scan(cn, lane), ppre = pcarry + wave_excl_scan(pn, lane);
-#pragma unroll
-        for (int d = -1; d <= 1; d++) {
-            if (!(g.prev & (d < 0 ? 2 : (d == 0 ? 4 : 8)))) continue;
-            // sp / sps: bit p = pixel / run start of row y-1 at p + d
-            uint64_t sp, sps;
-            if (d == 0) {
-                sp = pw;
-                sps = pst;
-            } else if (d > 0) {
-                sp = (pw >> 1) | (pwr << 63);
-                sps = (pst >> 1) | ((pwr & ~(pw >> 63) & 1ull) << 63);
-            } else {
-                sp = (pw << 1) | (pwl >> 63);
-                sps = (pst << 1) | ((pwl >> 63) & ~(pwl >> 62) & 1ull);
-            }
-            uint64_t ev = cw & sp & (cst | sps);
-            while (ev) {
-                const int p = __builtin_ctzll(ev);
-                ev &= ev - 1;
-                const uint64_t le = le_mask(p);
-                const int32_t a = cbase + cpre + __popcll(cst & le) - 1;
-                // runs of row y-1 starting at or before p + d
-                int nb = ppre + __popcll(pst & le);
-                if (d > 0) nb += (int)((sps >> p) & 1ull);
-                if (d < 0) nb -= (int)((pst >> p) & 1ull);
-                join(a, pbase + nb - 1);
-            }
-        }
-        ccarry += wave_sum(cn);
-        pcarry += wave_sum(pn);
-    }
+    for (int c0 = 0; c0 < nw; c0 += 64)
+        chunk_contacts(g, load_chunk(cur, up, nw, c0 + lane), cbase, pbase, ccarry, pcarry, lane, join);
 }
 
 // One workgroup per band of RUN_BAND rows of one frame.  The band's runs are

@@ -101,9 +101,10 @@ def test_image_deterministic_bit_exact(cuda, oracle, dtype, overlap, kernel):
 @pytest.mark.parametrize('dtype', [np.float32, np.float64])
 @pytest.mark.parametrize('overlap', ['sum', 'max'])
 @pytest.mark.parametrize('kernel', KERNELS)
-@pytest.mark.parametrize('tile,seg,threads', [(64, 16, 256), (32, 8, 64), (64, 8, 128), (32, 16, 256)])
+@pytest.mark.parametrize('tile,seg,threads,chunk', [(64, 16, 256, 64), (32, 8, 64, 64), (64, 8, 128, 64),
+                                                     (32, 16, 256, 64), (64, 8, 256, 256), (32, 16, 256, 256)])
 @pytest.mark.parametrize('exact', [False, True])
-def test_image_tiles(cuda, oracle, dtype, overlap, kernel, tile, seg, threads, exact):
+def test_image_tiles(cuda, oracle, dtype, overlap, kernel, tile, seg, threads, chunk, exact):
     """Tile path: same support as the reference always; values exact per term with
     exact_values (max overlap then bit-exact), within RTOL/ATOL otherwise."""
     for seed in range(3):
@@ -112,7 +113,7 @@ def test_image_tiles(cuda, oracle, dtype, overlap, kernel, tile, seg, threads, e
         ref = np.full(shape, init, dtype)
         oracle.draw_lines(ref, lines, idxs, 1.7, kernel, overlap)
         got = gpu_image(shape, lines, idxs, kernel, overlap, dtype, init, tile=tile, seg=seg, threads=threads,
-                        exact_values=exact)
+                        chunk=chunk, exact_values=exact)
         if overlap == 'max' and kernel != 'gaussian' and (exact or dtype == np.float64):
             assert np.array_equal(got, ref), f"seed {seed}"
         else:
