@@ -578,6 +578,8 @@ __device__ __forceinline__ void run_union(int32_t *par, int32_t a, int32_t b)
     }
 }
 
+constexpr int PACK_BATCH = 32;
+
 template <typename In>
 __global__ void __launch_bounds__(RUN_NT)
 k_run_pack(In in, RunGeom g, uint64_t *__restrict__ bits, int32_t *__restrict__ cnt)
@@ -592,15 +594,16 @@ k_run_pack(In in, RunGeom g, uint64_t *__restrict__ bits, int32_t *__restrict__ 
     for (int c0 = 0; c0 < g.nw; c0 += 64) {
         const int nwc = min(64, g.nw - c0);
         uint64_t mine = 0;
-        for (int k0 = 0; k0 < nwc; k0 += 16) {
-            typename In::Raw v[16];
+        // PACK_BATCH words per memory round trip (a 2048-pixel row in one)
+        for (int k0 = 0; k0 < nwc; k0 += PACK_BATCH) {
+            typename In::Raw v[PACK_BATCH];
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
+            for (int k = 0; k < PACK_BATCH; k++) {
                 const int64_t x = (int64_t)(c0 + k0 + k) * 64 + lane;
                 v[k] = in.load(base + min(x, (int64_t)g.W - 1));
             }
 #pragma unroll
-            for (int k = 0; k < 16; k++) {
+            for (int k = 0; k < PACK_BATCH; k++) {
                 const int64_t x = (int64_t)(c0 + k0 + k) * 64 + lane;
                 const uint64_t m = __ballot(x < g.W && in.test(v[k]));
                 if (lane == k0 + k) mine = m;
