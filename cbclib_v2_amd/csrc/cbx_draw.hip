@@ -1603,11 +1603,14 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     k_setup_lines<T, I><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                P.setup.ptr, counters.ptr, P.flags.ptr);
     CBX_HIP(hipGetLastError());
-    unsigned long long slots = 0;
-    int fl = 0;
-    CBX_HIP(hipMemcpyAsync(&slots, counters.ptr, sizeof(slots), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    uint64_t hv[2];
+    {
+        const void *src[2] = {counters.ptr, P.flags.ptr};
+        const int nb[2] = {8, 4};
+        CBX_HIP(read_device_values(st, 2, src, nb, hv));
+    }
+    const unsigned long long slots = hv[0];
+    int fl = (int)(uint32_t)hv[1];
     int rc = flags_to_status(fl);
     if (rc) return rc;
     if (slots >= (1ull << 31)) return fail(CBX_EVALUE, "too many segments (2^31)");
@@ -1627,10 +1630,13 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     CBX_HIP(hipGetLastError());
     k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, cnt2.ptr, total.ptr);
     CBX_HIP(exclusive_scan_i32(total.ptr, P.tile_off.ptr, P.n_tiles + 1, st));
-    int n_items = 0;
-    CBX_HIP(hipMemcpyAsync(&n_items, P.tile_off.ptr + P.n_tiles, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[2] = {P.tile_off.ptr + P.n_tiles, P.flags.ptr};
+        const int nb[2] = {4, 4};
+        CBX_HIP(read_device_values(st, 2, src, nb, hv));
+    }
+    const int n_items = (int)(uint32_t)hv[0];
+    fl = (int)(uint32_t)hv[1];
     if ((rc = flags_to_status(fl))) return rc;
     if (n_items < 0) return fail(CBX_ERUNTIME, "tile item count overflow");
     P.n_items = n_items;

@@ -108,6 +108,12 @@ struct KernelTimer {
     ~KernelTimer() { stop(); }
 };
 
+// Reads up to 4 device values (4 or 8 bytes each) after the stream's earlier
+// work: one small kernel writes them into pinned host memory, then one stream
+// synchronisation.  (Replaces one staged device-to-host copy per value.)
+// src[i]: device address, bytes[i]: 4 or 8; out[i]: the value, zero-extended.
+hipError_t read_device_values(hipStream_t s, int n, const void *const *src, const int *bytes, uint64_t *out);
+
 hipError_t exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s);
 hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
 

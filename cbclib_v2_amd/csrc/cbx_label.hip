@@ -973,8 +973,13 @@ static int label_runs(In in, int ndim, const int64_t *shape, int prev, int64_t n
     CBX_HIP(hipGetLastError());
     timer.stop();
     int32_t n = 0;
-    CBX_HIP(hipMemcpyAsync(&n, koff.ptr + R, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[1] = {koff.ptr + R};
+        const int nb[1] = {4};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        n = (int32_t)(uint32_t)hv;
+    }
     *n_labels = n;
     return CBX_OK;
 }
@@ -1067,8 +1072,13 @@ static int label_band(In in, int ndim, const int64_t *shape, int prev, int64_t n
     CBX_HIP(hipGetLastError());
     timer.stop();
     int32_t n = 0;
-    CBX_HIP(hipMemcpyAsync(&n, band_off.ptr + g.n_bands, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[1] = {band_off.ptr + g.n_bands};
+        const int nb[1] = {4};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        n = (int32_t)(uint32_t)hv;
+    }
     *n_labels = n;
     return CBX_OK;
 }
@@ -1173,8 +1183,13 @@ static int label_dispatch(In in, int ndim, const int64_t *shape, const int64_t *
     CBX_HIP(hipGetLastError());
     timer.stop();
     int32_t n = 0;
-    CBX_HIP(hipMemcpyAsync(&n, rank.ptr + P, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[1] = {rank.ptr + P};
+        const int nb[1] = {4};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        n = (int32_t)(uint32_t)hv;
+    }
     *n_labels = n;
     return CBX_OK;
 }

@@ -189,6 +189,69 @@ static size_t workspace_bytes()
     return n;
 }
 
+// ---------------------------------------------------------------------------
+// small device-to-host reads
+// ---------------------------------------------------------------------------
+struct MailSrc {
+    const void *p[4];
+    int bytes[4];
+    int n;
+};
+
+__global__ void k_mail(MailSrc s, uint64_t *dst)
+{
+    const int i = threadIdx.x;
+    if (i < s.n)
+        dst[i] = (s.bytes[i] == 8) ? *static_cast<const uint64_t *>(s.p[i])
+                                   : (uint64_t) * static_cast<const uint32_t *>(s.p[i]);
+}
+
+// One pinned, mapped, coherent mailbox of 4 values per host thread and device
+// (a host thread runs one call at a time; kept until the process ends).
+struct Mailbox {
+    uint64_t *host = nullptr;
+    uint64_t *dev = nullptr;
+};
+
+hipError_t read_device_values(hipStream_t s, int n, const void *const *src, const int *bytes, uint64_t *out)
+{
+    if (n < 1 || n > 4) return hipErrorInvalidValue;
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    thread_local std::vector<Mailbox> boxes;
+    if ((int)boxes.size() <= dev) boxes.resize(dev + 1);
+    Mailbox &mb = boxes[dev];
+    if (!mb.host) {
+        void *h = nullptr;
+        e = hipHostMalloc(&h, 4 * sizeof(uint64_t), hipHostMallocMapped | hipHostMallocCoherent);
+        if (e != hipSuccess) return e;
+        void *d = nullptr;
+        e = hipHostGetDevicePointer(&d, h, 0);
+        if (e != hipSuccess) {
+            (void)hipHostFree(h);
+            return e;
+        }
+        mb.host = static_cast<uint64_t *>(h);
+        mb.dev = static_cast<uint64_t *>(d);
+    }
+    MailSrc ms{};
+    for (int i = 0; i < n; i++) {
+        if (bytes[i] != 4 && bytes[i] != 8) return hipErrorInvalidValue;
+        ms.p[i] = src[i];
+        ms.bytes[i] = bytes[i];
+    }
+    ms.n = n;
+    k_mail<<<1, 64, 0, s>>>(ms, mb.dev);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    const volatile uint64_t *h = mb.host;
+    for (int i = 0; i < n; i++) out[i] = h[i];
+    return hipSuccess;
+}
+
 template <typename V>
 static hipError_t exclusive_scan(const V *in, V *out, int64_t n, hipStream_t s)
 {
