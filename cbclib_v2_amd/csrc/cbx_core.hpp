@@ -113,8 +113,9 @@ struct alignas(16) LineSetup {
 
 // Smallest float x >= 0 with !(RN(RN(x / len)^2) < w2) (the normal-error test
 // of nsq_of).  That square is non-decreasing in |x| and RN is symmetric, so
-// |Ne| < nlim decides the fill loops' band test exactly.  Bisection over the
-// bit patterns of the non-negative floats, which are ordered like their values.
+// |Ne| < nlim decides the fill loops' band test exactly.  Found by stepping
+// or bisecting over the bit patterns of the non-negative floats, which are
+// ordered like their values.
 __device__ __forceinline__ float band_limit(const LineSetup<float> &L)
 {
     auto out = [&](uint32_t bits) {
@@ -122,6 +123,17 @@ __device__ __forceinline__ float band_limit(const LineSetup<float> &L)
         return !(q * q < L.w2);
     };
     if (out(0u)) return 0.0f;
+    // the threshold lies within a few ulps of w len: step from there (x is
+    // the answer when out(x) and not out(x - 1)), and bisect only if that
+    // has not settled within 16 steps
+    uint32_t x = __float_as_uint(L.w * L.len);
+    if (x > 0u && x < 0x7f800000u) {
+        for (int i = 0; i < 16; i++) {
+            if (!out(x)) x++;
+            else if (out(x - 1u)) x--;
+            else return __uint_as_float(x);
+        }
+    }
     uint32_t lo = 0u, hi = 0x7f800000u;   // out(lo) false; out(+inf) true (a NaN or +inf square)
     while (hi - lo > 1u) {
         const uint32_t mid = lo + ((hi - lo) >> 1);

@@ -780,9 +780,25 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 }
                 const int a = L.axis, sa = pick(L.st, a);
                 const int c0 = pick(sg.c, a);
-                frec[tid] = fill_rec(L, c0, sg.line);
+                // per-line profiles: the wave's items take their record slots
+                // grouped by profile, so the fills of a wave mostly share one
+                // (fixed-point sums do not depend on the item order)
+                int dst = tid;
+                if constexpr (K == K_MIXED) {
+                    const int kc = (L.kid >= 0 && L.kid < K_TRIANGULAR) ? L.kid : K_TRIANGULAR;
+                    const uint64_t below = (1ull << (tid & 63)) - 1ull;
+                    int off = 0, rank = 0;
 #pragma unroll
-                for (int k = 0; k < SEG; k++) rs[tid * SEG + k] = INT_MIN;
+                    for (int k = 0; k <= K_TRIANGULAR; k++) {
+                        const uint64_t m = __ballot(kc == k);
+                        if (kc == k) rank = off + __popcll(m & below);
+                        off += __popcll(m);
+                    }
+                    dst = (tid & ~63) + rank;
+                }
+                frec[dst] = fill_rec(L, c0, sg.line);
+#pragma unroll
+                for (int k = 0; k < SEG; k++) rs[dst * SEG + k] = INT_MIN;
                 WalkState<T> s;
                 s.c[0] = sg.c[0];
                 s.c[1] = sg.c[1];
@@ -799,7 +815,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                     bool nx, ny;
                     walk_next(L, s.Ne, nx, ny);
                     const bool rec = act && ((a == 0) ? nx : ny);
-                    const int slot = rec ? tid * SEG + k : CH * SEG + tid;
+                    const int slot = rec ? dst * SEG + k : CH * SEG + tid;
                     rs[slot] = pick(s.c, 1 - a);
                     rT[slot] = s.Te;
                     rN[slot] = s.Ne;
@@ -1541,10 +1557,15 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int kern
     if (bin) CBX_HIP(exclusive_scan_i32(P.tile_cnt.ptr, P.tile_off.ptr, P.n_tiles + 1, st));
 
     int host[3] = {0, 0, 0};
-    CBX_HIP(hipMemcpyAsync(&host[0], P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipMemcpyAsync(&host[1], P.seg_off.ptr + n_lines, sizeof(int), hipMemcpyDeviceToHost, st));
-    if (bin) CBX_HIP(hipMemcpyAsync(&host[2], P.tile_off.ptr + P.n_tiles, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[3] = {P.flags.ptr, P.seg_off.ptr + n_lines, bin ? (const void *)(P.tile_off.ptr + P.n_tiles)
+                                                                         : (const void *)P.flags.ptr};
+        const int nb[3] = {4, 4, 4};
+        uint64_t hv[3];
+        CBX_HIP(read_device_values(st, 3, src, nb, hv));
+        for (int i = 0; i < 3; i++) host[i] = (int)(uint32_t)hv[i];
+        if (!bin) host[2] = 0;
+    }
     int rc = flags_to_status(host[0]);
     if (rc) return rc;
     if (host[1] < 0 || host[2] < 0) return fail(CBX_ERUNTIME, "segment count overflow");
@@ -1755,9 +1776,14 @@ static int raw_list_pipeline(SegmentPlan<T> &P, int kernel, T max_val, int32_t *
     CBX_HIP(exclusive_scan_i64(cnt64.ptr, off.ptr, P.n_segs + 1, st));
     int64_t E = 0;
     int fl = 0;
-    CBX_HIP(hipMemcpyAsync(&E, off.ptr + P.n_segs, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[2] = {off.ptr + P.n_segs, P.flags.ptr};
+        const int nb[2] = {8, 4};
+        uint64_t hv[2];
+        CBX_HIP(read_device_values(st, 2, src, nb, hv));
+        E = (int64_t)hv[0];
+        fl = (int)(uint32_t)hv[1];
+    }
     int rc = flags_to_status(fl);
     if (rc) return rc;
     *count = E;
@@ -1829,8 +1855,13 @@ static int ordered_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_va
     }
     CBX_HIP(hipGetLastError());
     int fl = 0;
-    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[1] = {P.flags.ptr};
+        const int nb[1] = {4};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        fl = (int)(uint32_t)hv;
+    }
     if (fl & FLAG_TILE_FULL)
         return fail(CBX_ERUNTIME, "a tile holds more than 8192 segments; the ordered mode cannot sort it");
     int rc = flags_to_status(fl);
@@ -1845,8 +1876,13 @@ static int ordered_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_va
     k_row_count<<<grid_for(rows, blk), blk, 0, st>>>(touched.ptr, rows, P.g.W, rcnt.ptr);
     CBX_HIP(exclusive_scan_i64(rcnt.ptr, roff.ptr, rows + 1, st));
     int64_t U = 0;
-    CBX_HIP(hipMemcpyAsync(&U, roff.ptr + rows, sizeof(int64_t), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[1] = {roff.ptr + rows};
+        const int nb[1] = {8};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        U = (int64_t)hv;
+    }
     *count = U;
     if (U == 0 || U > capacity) return CBX_OK;
     k_row_write<T><<<grid_for(rows, blk), blk, 0, st>>>(touched.ptr, img, first.ptr, rows, P.g.H, P.g.W, roff.ptr, of,
@@ -1992,8 +2028,13 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
                                                           n_lines, grad_lines);
     CBX_HIP(hipGetLastError());
     int fl = 0;
-    CBX_HIP(hipMemcpyAsync(&fl, P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[1] = {P.flags.ptr};
+        const int nb[1] = {4};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        fl = (int)(uint32_t)hv;
+    }
     return flags_to_status(fl);
 }
 
@@ -2138,8 +2179,13 @@ static int plan_check(cbx_plan *pl, hipStream_t st)
 {
     if (!pl) return fail(CBX_EVALUE, "plan must not be NULL");
     int fl = 0;
-    CBX_HIP(hipMemcpyAsync(&fl, pl->P.flags.ptr, sizeof(int), hipMemcpyDeviceToHost, st));
-    CBX_HIP(hipStreamSynchronize(st));
+    {
+        const void *src[1] = {pl->P.flags.ptr};
+        const int nb[1] = {4};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        fl = (int)(uint32_t)hv;
+    }
     return flags_to_status(fl);
 }
 
