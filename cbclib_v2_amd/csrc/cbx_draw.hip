@@ -293,9 +293,13 @@ k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, const uin
     if ((threadIdx.x & 63) == 0 && ub) atomicAdd(&counters[0], (unsigned long long)ub);
 }
 
-constexpr int WALK_RING = 4;
+// Records per thread in the walk's LDS ring: 4 (40 KB per 256 threads) for
+// launches of up to WALK_RING_LINES lines, whose waves are all resident at
+// once anyway; 2 (20 KB) above that, where more resident workgroups per CU
+// matter more than the stores saved (measured on 128k and 972k lines).
+constexpr int64_t WALK_RING_LINES = 262144;
 
-template <typename T>
+template <typename T, int WALK_RING>
 __global__ void __launch_bounds__(256)
 k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
              int64_t slot_cap, unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt,
@@ -1644,9 +1648,14 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     CBX_HIP(hipMemsetAsync(cnt2.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
     {
         KernelTimer timer("segments_walk", st);
-        k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                counters.ptr, P.tile_cnt.ptr, cnt2.ptr, P.flags.ptr,
-                                                                line_base);
+        if (n_lines <= WALK_RING_LINES)
+            k_walk_lines<T, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                       counters.ptr, P.tile_cnt.ptr, cnt2.ptr,
+                                                                       P.flags.ptr, line_base);
+        else
+            k_walk_lines<T, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                       counters.ptr, P.tile_cnt.ptr, cnt2.ptr,
+                                                                       P.flags.ptr, line_base);
     }
     CBX_HIP(hipGetLastError());
     k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, cnt2.ptr, total.ptr);
@@ -2151,9 +2160,14 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
                                                                P.tile_cnt.ptr, pl->cnt2.ptr, P.tile_cur.ptr);
     k_setup_lines<float, int32_t><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                          P.setup.ptr, pl->counters.ptr, P.flags.ptr);
-    k_walk_lines<float><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
-                                                                P.flags.ptr, nullptr);
+    if (n_lines <= WALK_RING_LINES)
+        k_walk_lines<float, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                       pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
+                                                                       P.flags.ptr, nullptr);
+    else
+        k_walk_lines<float, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                       pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
+                                                                       P.flags.ptr, nullptr);
     k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, pl->cnt2.ptr,
                                                               pl->total.ptr);
     size_t bytes = pl->scan_bytes;

@@ -122,6 +122,31 @@ def test_image_tiles(cuda, oracle, dtype, overlap, kernel, tile, seg, threads, c
         assert np.array_equal(got != init, ref != init)
 
 
+def test_many_lines_parity(cuda, oracle):
+    """More than 262144 lines: the segment walk then uses its smaller LDS ring
+    (two records per thread).  300000 short lines over 4 frames of 256^2,
+    gaussian sum and rectangular max, against the oracle."""
+    rng = np.random.default_rng(7)
+    n, F, H, W = 300_000, 4, 256, 256
+    p0 = rng.uniform(-4, 260, (n, 2))
+    ang = rng.uniform(0, 2 * np.pi, n)
+    ln = rng.uniform(0, 10, n)
+    lines = np.empty((n, 5), np.float32)
+    lines[:, :2] = p0
+    lines[:, 2] = p0[:, 0] + ln * np.cos(ang)
+    lines[:, 3] = p0[:, 1] + ln * np.sin(ang)
+    lines[:, 4] = rng.uniform(0.3, 2.5, n)
+    idxs = rng.integers(0, F, n).astype(np.int32)
+    for kernel, overlap in (('gaussian', 'sum'), ('rectangular', 'max')):
+        ref = oracle.draw_lines(np.zeros((F, H, W), np.float32), lines, idxs, 1.0, kernel, overlap)
+        got = gpu_image((F, H, W), lines, idxs, kernel, overlap, np.float32, max_val=1.0)
+        if overlap == 'max':
+            assert np.array_equal(got, ref)
+        else:
+            np.testing.assert_allclose(got, ref, rtol=RTOL, atol=ATOL)
+        assert np.array_equal(got != 0, ref != 0)
+
+
 def test_config1_parity(cuda, oracle):
     """Config 1: one 512x512 frame, 1000 segments, triangular, sum; plus its pixel list."""
     lines = config1()
