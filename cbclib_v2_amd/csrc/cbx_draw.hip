@@ -290,7 +290,13 @@ k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, const uin
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) ub += __shfl_xor(ub, o, 64);
-    if ((threadIdx.x & 63) == 0 && ub) atomicAdd(&counters[0], (unsigned long long)ub);
+    // one global atomic per workgroup (same-address atomics serialise)
+    __shared__ unsigned long long bsum;
+    if (threadIdx.x == 0) bsum = 0ull;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && ub) atomicAdd(&bsum, (unsigned long long)ub);
+    __syncthreads();
+    if (threadIdx.x == 0 && bsum) atomicAdd(&counters[0], bsum);
 }
 
 // Records per thread in the walk's LDS ring: 4 (40 KB per 256 threads) for
@@ -322,12 +328,25 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
     }
     const int total = __shfl(pre, 63, 64);
     pre -= ub;
-    unsigned long long wbase = 0;
-    if (lane == 0 && total) {
-        wbase = atomicAdd(&counters[1], (unsigned long long)total);
-        if ((int64_t)(wbase + total) > slot_cap) atomicOr(flags, FLAG_OVERFLOW);
+    // slots for the workgroup's waves with one global atomic (same-address
+    // atomics serialise), handed out in wave order
+    __shared__ unsigned long long wsum[4];
+    const int wave = threadIdx.x >> 6, n_waves = (blockDim.x + 63) >> 6;
+    if (lane == 0) wsum[wave] = (unsigned long long)total;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long bt = 0;
+        for (int w = 0; w < n_waves; w++) bt += wsum[w];
+        unsigned long long b = bt ? atomicAdd(&counters[1], bt) : 0ull;
+        if (bt && (int64_t)(b + bt) > slot_cap) atomicOr(flags, FLAG_OVERFLOW);
+        for (int w = 0; w < n_waves; w++) {
+            const unsigned long long t = wsum[w];
+            wsum[w] = b;
+            b += t;
+        }
     }
-    wbase = __shfl(wbase, 0, 64);
+    __syncthreads();
+    const unsigned long long wbase = wsum[wave];
     // first slot of each line (the adjoint's per-line reduction)
     if (line_base && l < n_lines) line_base[l] = (int64_t)wbase + pre;
     if (ub == 0 || (int64_t)(wbase + total) > slot_cap) return;
