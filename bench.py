@@ -44,7 +44,7 @@ def measured_traffic(kernel: str):
 
 
 def fail_multi(n: int) -> None:
-    print(json.dumps({"error": f"--gpus {n}: this build measures one GPU only; the path shards by frame "
+    print(json.dumps({"error": f"--gpus {n}: this benchmark measures one GPU only; the path shards by frame "
                       "(see the sharding paragraph of the design notes) and no multi-GPU run is made"}))
     sys.exit(2)
 
