@@ -681,6 +681,8 @@ __device__ __forceinline__ double fix_value(uint32_t acc, const uint32_t *zb, in
     return acc ? (double)acc * scale : (double)(max_val * 0.0f);
 }
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // fix_value for a pixel already in a register: o + acc * scale (or + zero,
 // the value of exact zeros, when only zeros touched it: zbit set), or o
 // untouched.
@@ -933,7 +935,11 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
             for (int j = 0; j < PV; j++) {
                 const int i = (tid + j * NT) * V;
                 const int y = Y0 + i / TILE, x = X0 + (i % TILE);
-                if (y < Y1) *reinterpret_cast<float4 *>(fo + (int64_t)y * g.W + x) = pre[j];
+                if (y < Y1) {
+                    // written once per draw: non-temporal
+                    const v4f q = {pre[j].x, pre[j].y, pre[j].z, pre[j].w};
+                    __builtin_nontemporal_store(q, reinterpret_cast<v4f *>(fo + (int64_t)y * g.W + x));
+                }
             }
         }
         return;

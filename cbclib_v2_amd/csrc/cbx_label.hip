@@ -63,7 +63,8 @@ struct ThresholdF32 {
     __device__ __forceinline__ bool on(int64_t i) const { return img[i] > vmin; }
     __device__ __forceinline__ bool same(int64_t, int64_t) const { return true; }
     using Raw = float;
-    __device__ __forceinline__ Raw load(int64_t i) const { return img[i]; }
+    // (streamed once: non-temporal)
+    __device__ __forceinline__ Raw load(int64_t i) const { return __builtin_nontemporal_load(&img[i]); }
     __device__ __forceinline__ bool test(Raw r) const { return r > vmin; }
 };
 
