@@ -752,7 +752,10 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
             for (int j = 0; j < PV; j++) {
                 const int i = (tid + j * NT) * V;
                 const int y = Y0 + i / TILE, x = X0 + (i % TILE);
-                if (y < Y1) pre[j] = *reinterpret_cast<const float4 *>(fo + (int64_t)y * g.W + x);
+                if (y < Y1) {
+                    const v4f q = __builtin_nontemporal_load(reinterpret_cast<const v4f *>(fo + (int64_t)y * g.W + x));
+                    pre[j] = make_float4(q.x, q.y, q.z, q.w);
+                }
             }
         }
     }
