@@ -1404,7 +1404,7 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
     for (int j = 0; j < PER; j++) {
         const int i = threadIdx.x + j * NT;
         const int y = Y0 + i / TILE, x = X0 + (i % TILE);
-        gv[j] = (y < Y1 && x < X1) ? fg[(int64_t)y * g.W + x] : T(0);
+        gv[j] = (y < Y1 && x < X1) ? __builtin_nontemporal_load(&fg[(int64_t)y * g.W + x]) : T(0);
     }
 #pragma unroll
     for (int j = 0; j < PER; j++) gt[threadIdx.x + j * NT] = gv[j];
