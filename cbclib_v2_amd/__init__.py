@@ -2,8 +2,9 @@
 
 Public surface (drop-ins for the reference's):
 
-* ``cbclib_v2_amd.ndimage.draw_lines`` (cbclib_v2.ndimage.draw_lines) and the
-  pixel-list mode ``draw_pixels``;
+* ``cbclib_v2_amd.ndimage.draw_lines`` (cbclib_v2.ndimage.draw_lines), the
+  pixel-list mode ``draw_pixels`` and ``accumulate_lines``
+  (cbclib_v2.ndimage.accumulate_lines);
 * ``cbclib_v2_amd.label.{label, Structure, LabelResult, CPLabelResult,
   labels, index}`` (cbclib_v2.label);
 * ``cbclib_v2_amd.device`` — device selection (``'rocm'``);

@@ -20,7 +20,8 @@ OVERLAPS = {'sum': 0, 'max': 1}
 DTYPES = {'bool': 0, 'int32': 1, 'int64': 2}
 
 EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32', 'cbx_draw_lines_f64',
-           'cbx_draw_pixels_f32', 'cbx_draw_pixels_f64', 'cbx_draw_lines_adjoint_f32',
+           'cbx_draw_pixels_f32', 'cbx_draw_pixels_f64', 'cbx_accumulate_lines_f32', 'cbx_accumulate_lines_f64',
+           'cbx_draw_lines_adjoint_f32',
            'cbx_draw_lines_adjoint_f64', 'cbx_label', 'cbx_label_threshold_f32', 'cbx_timing_enable',
            'cbx_timing_reset', 'cbx_timing_read', 'cbx_workspace_release', 'cbx_workspace_bytes',
            'cbx_label_force_path', 'cbx_plan_create_f32', 'cbx_plan_draw_f32', 'cbx_plan_check', 'cbx_plan_bytes',
@@ -62,6 +63,9 @@ def _declare(L: ctypes.CDLL) -> None:
         fn.restype = I32
         fn.argtypes = [I64, I64, I64, P, I64, P, R, I32, I32, I32, P, P, P, P, P, I64,
                        ctypes.POINTER(ctypes.c_int64), P]
+        fn = getattr(L, 'cbx_accumulate_lines_' + name)
+        fn.restype = I32
+        fn.argtypes = [P, I64, I64, I64, P, I64, P, P, I64, R, I32, I32, I32, OP, P]
         fn = getattr(L, 'cbx_draw_lines_adjoint_' + name)
         fn.restype = I32
         fn.argtypes = [P, I64, I64, I64, P, I64, P, R, I32, P, OP, P]

@@ -2004,6 +2004,157 @@ static int draw_pixels_impl(int64_t n_frames, int64_t H, int64_t W, const T *lin
     return raw_list_pipeline<T>(P, kernel, max_val, of, oy, ox, ov, ol, capacity, count, st);
 }
 
+// accumulate_lines, step 1: range checks of terms and frames (bad[0]: a term
+// outside [0, n_terms), bad[1]: a frame outside [0, n_frames)) and, when
+// comp != nullptr, the frame of every line, comp[i] = frames[terms[i]].
+template <typename I>
+__global__ void k_term_frames(const I *__restrict__ terms, const I *__restrict__ frames, int64_t n_lines,
+                              int64_t n_terms, int64_t n_frames, I *__restrict__ comp, int *__restrict__ bad)
+{
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i < n_terms) {
+        const I f = frames[i];
+        if (f < 0 || (int64_t)f >= n_frames) atomicOr(bad + 1, 1);
+    }
+    if (i < n_lines) {
+        const I t = terms[i];
+        I f = 0;
+        if (t < 0 || (int64_t)t >= n_terms) atomicOr(bad, 1);
+        else f = frames[t];
+        if (comp) comp[i] = (f < 0 || (int64_t)f >= n_frames) ? I(0) : f;
+    }
+}
+
+__device__ inline unsigned int val_bits(float v) { return __float_as_uint(v); }
+__device__ inline unsigned long long val_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+__device__ inline float bits_val(unsigned int b, float) { return __uint_as_float(b); }
+__device__ inline double bits_val(unsigned long long b, double) { return __longlong_as_double((long long)b); }
+
+// *p = max(*p, v) in the order std::max(*p, v) takes: replaced only if *p < v
+__device__ inline void atomic_max_val(float *p, float v)
+{
+    unsigned int *q = (unsigned int *)p;
+    unsigned int old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (bits_val(old, v) < v) {
+        const unsigned int prev = atomicCAS(q, old, val_bits(v));
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+__device__ inline void atomic_max_val(double *p, double v)
+{
+    unsigned long long *q = (unsigned long long *)p;
+    unsigned long long old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (bits_val(old, v) < v) {
+        const unsigned long long prev = atomicCAS(q, old, val_bits(v));
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+// accumulate_lines, step 3: the per-term images (img, touched: n_terms x hw,
+// in-term reduction done) combined into their frames with the out-term
+// reduction, bresenham.cpp:230-252.  Terms reach a frame in no fixed order,
+// as under the reference's OpenMP loop over terms.
+template <typename T, typename I, int OUT_OV>
+__global__ void k_terms_to_frames(T *__restrict__ out, const T *__restrict__ img, const uint8_t *__restrict__ touched,
+                                  const I *__restrict__ frames, int64_t n_terms, int64_t hw)
+{
+    const int64_t total = n_terms * hw;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += stride) {
+        if (!touched[q]) continue;
+        const int64_t t = q / hw;
+        T *dst = out + (int64_t)frames[t] * hw + (q - t * hw);
+        const T v = img[q];
+        if (OUT_OV == 0) atomicAdd(dst, v);
+        else atomic_max_val(dst, v);
+    }
+}
+
+// accumulate_lines (bresenham.cpp:151-256): lines grouped into terms, each
+// term reduced over its lines (in_overlap), terms reduced into frames
+// (out_overlap).  Equal overlaps compose to draw_lines with idxs =
+// frames[terms] on the tile path; mixed overlaps render every term in
+// reference line order into a per-term scratch (n_terms x H x W values and
+// touched flags, starting from zero like the reference's term buffer) and
+// combine the terms with k_terms_to_frames.
+template <typename T, typename I>
+static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const T *lines, int64_t n_lines,
+                                 const I *terms, const I *frames, int64_t n_terms, T max_val, int kernel,
+                                 int in_overlap, int out_overlap, const cbx_draw_opts *opts, void *stream)
+{
+    if (in_overlap < 0 || in_overlap > 1 || out_overlap < 0 || out_overlap > 1)
+        return fail(CBX_EVALUE, "Invalid in_overlap and out_overlap keywords");
+    if (n_terms < 0) return fail(CBX_EVALUE, "negative size");
+    if (n_lines > 0 && !terms) return fail(CBX_EVALUE, "terms must not be NULL");
+    if (n_terms > 0 && !frames) return fail(CBX_EVALUE, "frames must not be NULL");
+    if (n_terms >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "too many terms");
+    int rc = check_common(n_frames, H, W, n_lines, true, kernel, in_overlap);
+    if (rc) return rc;
+    if (opts && opts->kernels) return fail(CBX_EVALUE, "accumulate_lines takes one profile per call");
+    hipStream_t st = (hipStream_t)stream;
+    const int64_t n_chk = n_lines > n_terms ? n_lines : n_terms;
+    const bool same = in_overlap == out_overlap;
+    DevBuf<I> comp;
+    if (n_chk > 0) {
+        DevBuf<int> bad;
+        CBX_HIP(bad.alloc(2, st));
+        CBX_HIP(hipMemsetAsync(bad.ptr, 0, 2 * sizeof(int), st));
+        if (same && n_lines > 0) CBX_HIP(comp.alloc(n_lines, st));
+        k_term_frames<I><<<grid_for(n_chk, 256), 256, 0, st>>>(terms, frames, n_lines, n_terms, n_frames,
+                                                                 comp.ptr, bad.ptr);
+        CBX_HIP(hipGetLastError());
+        const void *src[2] = {bad.ptr, bad.ptr + 1};
+        const int nb[2] = {4, 4};
+        uint64_t hv[2] = {0, 0};
+        CBX_HIP(read_device_values(st, 2, src, nb, hv));
+        if (hv[1]) return fail(CBX_EINDEX, "frames is outside of (0, n_frames)");
+        if (hv[0]) return fail(CBX_EINDEX, "a term index is outside of (0, len(frames))");
+    }
+    if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
+    if (same) return draw_lines_impl<T, I>(out, n_frames, H, W, lines, n_lines, comp.ptr, max_val, kernel,
+                                           in_overlap, opts, stream);
+
+    const int64_t hw = H * W;
+    if (n_terms > (int64_t(1) << 62) / (hw * (int64_t)(sizeof(T) + 1)))
+        return fail(CBX_EVALUE, "the per-term scratch (n_terms x H x W) exceeds the supported range");
+    SegmentPlan<T> P;
+    if ((rc = build_plan<T, I>(P, lines, terms, kernel, n_lines, n_terms, H, W, 32, 16, true, st))) return rc;
+    if (P.n_segs == 0) return CBX_OK;
+    k_sort_tile_items<<<(unsigned)P.n_tiles, 256, 0, st>>>(P.tile_off.ptr, P.items.ptr, P.flags.ptr);
+    CBX_HIP(hipGetLastError());
+    const int64_t npix = n_terms * hw;
+    DevBuf<T> img;
+    DevBuf<uint8_t> touched;
+    CBX_HIP(img.alloc(npix, st));
+    CBX_HIP(touched.alloc(npix, st));
+    // tiles without items never write their flags
+    CBX_HIP(hipMemsetAsync(touched.ptr, 0, npix, st));
+    if (in_overlap == 0) launch_ordered_k<T, 0, 32, 16, true>(kernel, img.ptr, touched.ptr, nullptr, P, max_val, st);
+    else launch_ordered_k<T, 1, 32, 16, true>(kernel, img.ptr, touched.ptr, nullptr, P, max_val, st);
+    CBX_HIP(hipGetLastError());
+    int fl = 0;
+    {
+        const void *src[1] = {P.flags.ptr};
+        const int nb[1] = {4};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        fl = (int)(uint32_t)hv;
+    }
+    if (fl & FLAG_TILE_FULL)
+        return fail(CBX_ERUNTIME, "a tile holds more than 8192 segments; accumulate_lines cannot sort it");
+    if ((rc = flags_to_status(fl))) return rc;
+    const int64_t blocks = std::min<int64_t>((npix + 255) / 256, 65536);
+    if (out_overlap == 0)
+        k_terms_to_frames<T, I, 0><<<(unsigned)blocks, 256, 0, st>>>(out, img.ptr, touched.ptr, frames, n_terms, hw);
+    else
+        k_terms_to_frames<T, I, 1><<<(unsigned)blocks, 256, 0, st>>>(out, img.ptr, touched.ptr, frames, n_terms, hw);
+    CBX_HIP(hipGetLastError());
+    return CBX_OK;
+}
+
 template <typename T, int K>
 static void launch_adjoint(const T *grad, const SegmentPlan<T> &P, const int64_t *slot_base, T max_val,
                            double *partial, hipStream_t st)
@@ -2303,6 +2454,28 @@ int cbx_draw_pixels_f64(int64_t n_frames, int64_t height, int64_t width, const d
     cbx::Workspace ws((hipStream_t)stream);
     return cbx::draw_pixels_impl<double, int64_t>(n_frames, height, width, lines, n_lines, idxs, max_val, kernel,
                                                   overlap, merge, frame, y, x, value, line, capacity, count, stream);
+}
+
+int cbx_accumulate_lines_f32(float *out, int64_t n_frames, int64_t height, int64_t width, const float *lines,
+                             int64_t n_lines, const int32_t *terms, const int32_t *frames, int64_t n_terms,
+                             float max_val, int kernel, int in_overlap, int out_overlap, const cbx_draw_opts *opts,
+                             void *stream)
+{
+    cbx::Workspace ws((hipStream_t)stream);
+    return cbx::accumulate_lines_impl<float, int32_t>(out, n_frames, height, width, lines, n_lines, terms, frames,
+                                                      n_terms, max_val, kernel, in_overlap, out_overlap, opts,
+                                                      stream);
+}
+
+int cbx_accumulate_lines_f64(double *out, int64_t n_frames, int64_t height, int64_t width, const double *lines,
+                             int64_t n_lines, const int64_t *terms, const int64_t *frames, int64_t n_terms,
+                             double max_val, int kernel, int in_overlap, int out_overlap, const cbx_draw_opts *opts,
+                             void *stream)
+{
+    cbx::Workspace ws((hipStream_t)stream);
+    return cbx::accumulate_lines_impl<double, int64_t>(out, n_frames, height, width, lines, n_lines, terms, frames,
+                                                       n_terms, max_val, kernel, in_overlap, out_overlap, opts,
+                                                       stream);
 }
 
 int cbx_draw_lines_adjoint_f32(const float *grad_out, int64_t n_frames, int64_t height, int64_t width,

@@ -32,7 +32,7 @@ import torch
 from . import _native, device
 from ._arrays import NUMPY, dtype_of, inout, namespace, ptr, result, shape_of, stream_ptr, to_device
 
-__all__ = ['draw_lines', 'draw_pixels', 'PixelList']
+__all__ = ['accumulate_lines', 'draw_lines', 'draw_pixels', 'PixelList']
 
 
 def _line_count(lines: Any) -> int:
@@ -103,6 +103,53 @@ def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, ker
     opts = _opts(tile, seg, deterministic, threads, exact_values, chunk, ptr(t_kern))
     _native.check(fn(t_out.data_ptr(), n_frames, H, W, t_lines.data_ptr(), n_lines, ptr(t_idxs), float(max_val),
                      kid, oid, ctypes.byref(opts), stream_ptr()))
+    return finish()
+
+
+def accumulate_lines(out: Any, lines: Any, terms: Any, frames: Any, max_val: float = 1.0,
+                     kernel: str = 'rectangular', in_overlap: str = 'sum', out_overlap: str = 'sum', *,
+                     tile: Optional[int] = None, seg: Optional[int] = None, threads: Optional[int] = None,
+                     chunk: Optional[int] = None):
+    """Accumulate thick lines grouped into terms across frames (in place); return ``out``.
+
+    Line ``i`` belongs to term ``terms[i]``; each term is drawn over its lines
+    with ``in_overlap`` ('sum' or 'max', starting from zero) and then combined
+    into frame ``frames[term]`` of ``out`` with ``out_overlap``.  Same
+    arguments and errors as cbclib_v2.ndimage.accumulate_lines
+    (functions.py:191-240, bresenham.cpp:151-309): a term outside
+    ``[0, len(frames))`` or a frame outside ``[0, n_frames)`` raises
+    IndexError, ``terms`` of a size other than the number of lines ValueError.
+    ``tile``, ``seg``, ``threads`` and ``chunk`` are the draw_lines launch
+    options (equal overlaps only).
+    """
+    device.require_rocm('accumulate_lines')
+    kid = _native.kernel_id(kernel)
+    if in_overlap not in ('sum', 'max') or out_overlap not in ('sum', 'max'):
+        raise ValueError(f"Invalid in_overlap and out_overlap keywords: {in_overlap} and {out_overlap}")
+    iid, oid = _native.overlap_id(in_overlap), _native.overlap_id(out_overlap)
+    oshape = shape_of(out)
+    odt = dtype_of(out)
+    if odt not in (np.float32, np.float64):
+        raise TypeError(f"out must be float32 or float64, got {odt}")
+    n_lines = _line_count(lines)
+    if len(oshape) < 2:
+        raise ValueError(f"Output array has insufficient number of dimensions ({len(oshape)} < 2)")
+    n_frames, H, W = prod(oshape[:-2]), oshape[-2], oshape[-1]
+    n_terms = prod(shape_of(frames))
+    if prod(shape_of(terms)) != n_lines:
+        raise ValueError(f"Term indices ({prod(shape_of(terms))}) is incompatible with number of lines "
+                         f"({n_lines})")
+    f64 = odt == np.float64
+    it = torch.int64 if f64 else torch.int32
+    t_out, finish = inout(out, torch.float64 if f64 else torch.float32)
+    t_lines = to_device(lines, torch.float64 if f64 else torch.float32)
+    t_terms = to_device(terms, it).reshape(-1)
+    t_frames = to_device(frames, it).reshape(-1)
+    L = _native.lib()
+    fn = L.cbx_accumulate_lines_f64 if f64 else L.cbx_accumulate_lines_f32
+    opts = _opts(tile, seg, False, threads, False, chunk)
+    _native.check(fn(t_out.data_ptr(), n_frames, H, W, t_lines.data_ptr(), n_lines, ptr(t_terms), ptr(t_frames),
+                     n_terms, float(max_val), kid, iid, oid, ctypes.byref(opts), stream_ptr()))
     return finish()
 
 

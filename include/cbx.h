@@ -124,6 +124,29 @@ CBX_API int cbx_draw_pixels_f64(int64_t n_frames, int64_t height, int64_t width,
                         void *stream);
 
 /*
+ * accumulate_lines: lines grouped into terms (terms[i] in [0, n_terms) for
+ * each of the n_lines lines), each term drawn with `in_overlap` over its lines
+ * starting from zero, then every term combined into frame frames[t] of `out`
+ * (n_frames x height x width, in place) with `out_overlap`
+ * (CBX_SUM | CBX_MAX each).  A term or frame index out of range ->
+ * CBX_EINDEX.  Equal overlaps are draw_lines with idxs = frames[terms]; mixed
+ * overlaps use n_terms x height x width x (sizeof value + 1) bytes of
+ * workspace.  opts as for draw_lines (the tile path of equal overlaps);
+ * opts->kernels is not accepted.  Replaces bresenham.accumulate_lines
+ * (bresenham.cpp:151-309, bound at 326-327) and
+ * cuda_draw_lines.accumulate_lines (cuda_draw_lines.cu:948-1057); the f32
+ * entry takes int32 terms/frames and the f64 entry int64.
+ */
+CBX_API int cbx_accumulate_lines_f32(float *out, int64_t n_frames, int64_t height, int64_t width, const float *lines,
+                             int64_t n_lines, const int32_t *terms, const int32_t *frames, int64_t n_terms,
+                             float max_val, int kernel, int in_overlap, int out_overlap, const cbx_draw_opts *opts,
+                             void *stream);
+CBX_API int cbx_accumulate_lines_f64(double *out, int64_t n_frames, int64_t height, int64_t width, const double *lines,
+                             int64_t n_lines, const int64_t *terms, const int64_t *frames, int64_t n_terms,
+                             double max_val, int kernel, int in_overlap, int out_overlap, const cbx_draw_opts *opts,
+                             void *stream);
+
+/*
  * Adjoint of draw_lines (overlap = sum): grad_lines[i] = d<grad_out, image>/d(x0, y0, x1, y1, width)
  * of line i, over the pixels line i draws.  No reference counterpart (the
  * reference has no gradient); the forward it differentiates is

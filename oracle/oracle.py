@@ -11,6 +11,8 @@ The functions mirror the reference call signatures they restate:
   (cbclib_v2/_src/src/bresenham.cpp:38-149, 329-330);
 * :func:`pixel_list` — the per-line ImageBuffer entries the reference
   accumulates (bresenham.cpp:80-86), in emission order;
+* :func:`accumulate_lines` — ``bresenham.accumulate_lines`` with
+  ``num_threads=1`` (bresenham.cpp:151-309), on top of :func:`pixel_list`;
 * :func:`label` — ``label.label`` + ``LabelResult.to_array``
   (label.cpp:109-175, 459-493) returned as an int32 map and a count;
 * :func:`render_closed_f64` / :func:`adjoint_f64` — the FP64 closed form and
@@ -134,6 +136,56 @@ def pixel_list(shape, lines: np.ndarray, idxs: Optional[np.ndarray] = None, max_
         if n <= cap:
             return tuple(a[:n] for a in arrs) + (val[:n], err[:n])
         cap = n
+
+
+def accumulate_lines(out: np.ndarray, lines: np.ndarray, terms: np.ndarray, frames: np.ndarray,
+                     max_val: float = 1.0, kernel: str = 'rectangular', in_overlap: str = 'sum',
+                     out_overlap: str = 'sum') -> np.ndarray:
+    """Single-threaded restatement of bresenham.accumulate_lines (bresenham.cpp:151-252); ``out`` in place.
+
+    Terms are visited in ascending order (one order the reference's OpenMP
+    loop over terms may take).  Each term's pixels, the :func:`pixel_list`
+    entries of its lines in ascending line order, are reduced with
+    ``in_overlap`` into a buffer that starts at zero (bresenham.cpp:215-223);
+    every touched pixel of the buffer is then combined into frame
+    ``frames[term]`` with ``out_overlap`` (bresenham.cpp:230-248).
+    """
+    if kernel not in KERNELS:
+        raise ValueError(f"kernel is missing for {kernel}")
+    if in_overlap not in OVERLAPS or out_overlap not in OVERLAPS:
+        raise ValueError(f"Invalid in_overlap and out_overlap keywords: {in_overlap} and {out_overlap}")
+    assert out.flags.c_contiguous and out.dtype in (np.float32, np.float64)
+    F, H, W = _frames(out.shape)
+    ln = np.asarray(lines, dtype=out.dtype).reshape(-1, 5)
+    terms = np.asarray(terms, dtype=np.int64).ravel()
+    frames = np.asarray(frames, dtype=np.int64).ravel()
+    if np.any((frames < 0) | (frames >= F)):
+        raise IndexError("frames is outside of (0, n_frames)")
+    if terms.size != ln.shape[0]:
+        raise ValueError("Term indices is incompatible with number of lines")
+    if np.any((terms < 0) | (terms >= frames.size)):
+        raise IndexError("Term index is out of range")
+    o = out.reshape(F, H * W)
+    for t in np.unique(terms):
+        sel = np.flatnonzero(terms == t)
+        _, _, y, x, val, _ = pixel_list((1, H, W), ln[sel], None, max_val, kernel, dtype=out.dtype)
+        pix = y.astype(np.int64) * W + x
+        buf = np.zeros(H * W, out.dtype)
+        touched = np.zeros(H * W, bool)
+        for k in range(pix.size):
+            p = pix[k]
+            touched[p] = True
+            if in_overlap == 'max':
+                buf[p] = val[k] if buf[p] < val[k] else buf[p]
+            else:
+                buf[p] = buf[p] + val[k]
+        dst = o[frames[t]]
+        for p in np.flatnonzero(touched):
+            if out_overlap == 'max':
+                dst[p] = buf[p] if dst[p] < buf[p] else dst[p]
+            else:
+                dst[p] = buf[p] + dst[p]
+    return out
 
 
 def render_closed_f64(shape, lines: np.ndarray, idxs: Optional[np.ndarray] = None, max_val: float = 1.0,
