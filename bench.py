@@ -298,6 +298,58 @@ def workload_c5(args):
             "step_GBs": round((12.0 * F * H * W + 68.0 * N + 140.0 * N) / (sec / n) / 1e9, 2)}
 
 
+def workload_acc(args):
+    """accumulate_lines as the rotomap caller uses it (in-term max, out-term
+    sum, gaussian): 16 frames of 512^2, 4000 terms of 16 segments (lengths
+    U[5, 60], widths U[1, 3], seed 5), each term in a random frame.  Also
+    times the equal-overlap (sum/sum) path on the same input and the compiled
+    reference (else the oracle port) on the whole workload."""
+    import numpy as np
+    import torch
+    from cbclib_v2_amd import ndimage
+    from tests.lines_gen import random_segments
+    dev = torch.device('cuda', 0)
+    F, H, W, T, per = 16, 512, 512, 4000, 16
+    rng = np.random.default_rng(5)
+    lines_np = random_segments(rng, T * per, H, W, 5.0, 60.0, 1.0, 3.0).astype(np.float32)
+    terms_np = np.repeat(np.arange(T), per).astype(np.int32)
+    frames_np = rng.integers(0, F, T).astype(np.int32)
+    lines = torch.from_numpy(lines_np).to(dev)
+    terms = torch.from_numpy(terms_np).to(dev)
+    frames = torch.from_numpy(frames_np).to(dev)
+    out = torch.zeros((F, H, W), dtype=torch.float32, device=dev)
+    N = T * per
+
+    def step(i_ov, o_ov):
+        out.zero_()
+        ndimage.accumulate_lines(out, lines, terms, frames, 1.0, 'gaussian', i_ov, o_ov)
+
+    ms = timed(lambda: step('max', 'sum'), args.steps, args.warmup)
+    ms_eq = timed(lambda: step('sum', 'sum'), args.steps, args.warmup)
+    res = {"metric": "accumulate_lines segments/s (in max, out sum)", "value": round(N / (ms * 1e-3) / 1e6, 4),
+           "unit": "Mlines/s", "ms_per_step": round(ms, 4), "steps": args.steps, "warmup": args.warmup,
+           "sum_sum_ms_per_step": round(ms_eq, 4),
+           "config": {"workload": f"accumulate_lines: {F} frames of {H}x{W} f32, {T} terms x {per} segments "
+                                  "(lengths U[5, 60], widths U[1, 3], seed 5), gaussian, in_overlap max, "
+                                  "out_overlap sum"}}
+    if not args.no_cpu:
+        from oracle import oracle as O
+        ref = O.ref_module('bresenham')
+        cores = min(16, os.cpu_count() or 1)
+        o = np.zeros((F, H, W), np.float32)
+        t0 = time.perf_counter()
+        if ref is not None:
+            ref.accumulate_lines(o, lines_np, terms_np, frames_np, 1.0, 'gaussian', 'max', 'sum', cores)
+        else:
+            O.accumulate_lines(o, lines_np, terms_np, frames_np, 1.0, 'gaussian', 'max', 'sum')
+        dt = time.perf_counter() - t0
+        res["cpu_baseline"] = {"value": round(N / dt / 1e6, 6), "unit": "Mlines/s",
+                               "cores": cores if ref is not None else 1,
+                               "kind": "reference" if ref is not None else "port",
+                               "sample": f"the whole workload once ({N} segments, {dt:.2f} s)"}
+    return res
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -310,9 +362,9 @@ def main() -> None:
     ap.add_argument('--sweep', action='store_true', help='time the tile kernel over launch shapes and exit')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
     ap.add_argument('--cpu-seconds', type=float, default=12.0)
-    ap.add_argument('--workload', choices=['c2', 'c3', 'c4', 'c5'], default='c2',
+    ap.add_argument('--workload', choices=['c2', 'c3', 'c4', 'c5', 'acc'], default='c2',
                     help='c2 (default, the headline line); c3 batched uneven forward; c4 forward + adjoint; '
-                         'c5 Adam refinement')
+                         'c5 Adam refinement; acc accumulate_lines (in max, out sum)')
     ap.add_argument('--adam-steps', type=int, default=200)
     ap.add_argument('--calibrate', action='store_true',
                     help='also run an elementwise kernel of known traffic (256 MiB read + 256 MiB written), the '
@@ -322,8 +374,8 @@ def main() -> None:
     if args.gpus != 1 or world != 1:
         fail_multi(max(args.gpus, world))
 
-    if args.workload in ('c3', 'c4', 'c5'):
-        res = {'c3': workload_c3, 'c4': workload_c4, 'c5': workload_c5}[args.workload](args)
+    if args.workload in ('c3', 'c4', 'c5', 'acc'):
+        res = {'c3': workload_c3, 'c4': workload_c4, 'c5': workload_c5, 'acc': workload_acc}[args.workload](args)
         res.update({"n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic"})
         print(json.dumps(res))
         return

@@ -1140,12 +1140,45 @@ k_sort_tile_items(const int *__restrict__ tile_off, int *__restrict__ items, int
 // every pixel sees its contributions in ascending line order, starting from
 // the value already in the frame (image mode) or from its first contribution
 // (list mode: `first` records the first contributing line).
-template <typename T, int K, int OV, int TILE, int SEG, bool LIST>
+__device__ inline unsigned int val_bits(float v) { return __float_as_uint(v); }
+__device__ inline unsigned long long val_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
+__device__ inline float bits_val(unsigned int b, float) { return __uint_as_float(b); }
+__device__ inline double bits_val(unsigned long long b, double) { return __longlong_as_double((long long)b); }
+
+// *p = max(*p, v) in the order std::max(*p, v) takes: replaced only if *p < v
+__device__ inline void atomic_max_val(float *p, float v)
+{
+    unsigned int *q = (unsigned int *)p;
+    unsigned int old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (bits_val(old, v) < v) {
+        const unsigned int prev = atomicCAS(q, old, val_bits(v));
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+__device__ inline void atomic_max_val(double *p, double v)
+{
+    unsigned long long *q = (unsigned long long *)p;
+    unsigned long long old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    while (bits_val(old, v) < v) {
+        const unsigned long long prev = atomicCAS(q, old, val_bits(v));
+        if (prev == old) break;
+        old = prev;
+    }
+}
+
+// ACC >= 0 (accumulate_lines): the plan's frames are terms; every touched
+// pixel of a term tile (accumulated from zero, as with LIST) is combined into
+// frame term_frame[term] of img with atomics, sum (ACC 0) or max (ACC 1).
+template <typename T, int K, int OV, int TILE, int SEG, bool LIST, int ACC>
 __global__ void __launch_bounds__(64)
 k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__restrict__ first_line, Geometry g,
                 const int *__restrict__ tile_off, const int *__restrict__ items, const Segment<T> *__restrict__ segs,
-                const LineSetup<T> *__restrict__ setup, T max_val, int *__restrict__ flags)
+                const LineSetup<T> *__restrict__ setup, T max_val, int *__restrict__ flags,
+                const int32_t *__restrict__ term_frame)
 {
+    static_assert(ACC < 0 || LIST, "accumulation starts from zero");
     static_assert(2 * SEG <= 64, "two lanes per column");
     __shared__ T acc[TILE * TILE];
     __shared__ int32_t first[TILE * TILE];
@@ -1161,7 +1194,7 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
     const int ty = r / g.ntx, tx = r - ty * g.ntx;
     const int X0 = tx * TILE, Y0 = ty * TILE;
     const int X1 = min(X0 + TILE, g.W), Y1 = min(Y0 + TILE, g.H);
-    T *fo = img + (f * g.H) * (int64_t)g.W;
+    T *fo = ACC >= 0 ? img : img + (f * g.H) * (int64_t)g.W;
     const int lane = threadIdx.x;
 
     for (int i = lane; i < TILE * TILE; i += 64) {
@@ -1233,7 +1266,13 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
         const int y = Y0 + i / TILE, x = X0 + (i % TILE);
         if (y >= Y1 || x >= X1) continue;
         const int64_t o = (int64_t)y * g.W + x;
-        if (LIST) {
+        if (ACC >= 0) {
+            if (first[i] >= 0) {
+                T *dst = img + ((int64_t)term_frame[f] * g.H) * (int64_t)g.W + o;
+                if (ACC == 0) atomicAdd(dst, acc[i]);
+                else atomic_max_val(dst, acc[i]);
+            }
+        } else if (LIST) {
             const int64_t q = (f * g.H) * (int64_t)g.W + o;
             touched[q] = first[i] >= 0;
             if (first[i] >= 0) {
@@ -1842,13 +1881,14 @@ static int raw_list_pipeline(SegmentPlan<T> &P, int kernel, T max_val, int32_t *
     return CBX_OK;
 }
 
-template <typename T, int OV, int TILE, int SEG, bool LIST>
+template <typename T, int OV, int TILE, int SEG, bool LIST, int ACC = -1>
 static void launch_ordered_k(int kernel, T *img, uint8_t *touched, int32_t *first, const SegmentPlan<T> &P,
-                             T max_val, hipStream_t st)
+                             T max_val, hipStream_t st, const int32_t *term_frame = nullptr)
 {
 #define CBX_ORD(KK)                                                                                             \
-    k_tiles_ordered<T, KK, OV, TILE, SEG, LIST><<<(unsigned)P.n_tiles, 64, 0, st>>>(                            \
-        img, touched, first, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr)
+    k_tiles_ordered<T, KK, OV, TILE, SEG, LIST, ACC><<<(unsigned)P.n_tiles, 64, 0, st>>>(                       \
+        img, touched, first, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr,    \
+        term_frame)
     switch (kernel) {
     case K_BIWEIGHT: CBX_ORD(K_BIWEIGHT); break;
     case K_GAUSSIAN: CBX_ORD(K_GAUSSIAN); break;
@@ -2005,16 +2045,20 @@ static int draw_pixels_impl(int64_t n_frames, int64_t H, int64_t W, const T *lin
 }
 
 // accumulate_lines, step 1: range checks of terms and frames (bad[0]: a term
-// outside [0, n_terms), bad[1]: a frame outside [0, n_frames)) and, when
-// comp != nullptr, the frame of every line, comp[i] = frames[terms[i]].
+// outside [0, n_terms), bad[1]: a frame outside [0, n_frames)), the frame of
+// every term as int32 (frames32, n_terms entries) and, when comp != nullptr,
+// the frame of every line, comp[i] = frames[terms[i]].
 template <typename I>
 __global__ void k_term_frames(const I *__restrict__ terms, const I *__restrict__ frames, int64_t n_lines,
-                              int64_t n_terms, int64_t n_frames, I *__restrict__ comp, int *__restrict__ bad)
+                              int64_t n_terms, int64_t n_frames, I *__restrict__ comp,
+                              int32_t *__restrict__ frames32, int *__restrict__ bad)
 {
     const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (i < n_terms) {
         const I f = frames[i];
-        if (f < 0 || (int64_t)f >= n_frames) atomicOr(bad + 1, 1);
+        const bool ok = f >= 0 && (int64_t)f < n_frames;
+        if (!ok) atomicOr(bad + 1, 1);
+        frames32[i] = ok ? (int32_t)f : 0;
     }
     if (i < n_lines) {
         const I t = terms[i];
@@ -2025,61 +2069,22 @@ __global__ void k_term_frames(const I *__restrict__ terms, const I *__restrict__
     }
 }
 
-__device__ inline unsigned int val_bits(float v) { return __float_as_uint(v); }
-__device__ inline unsigned long long val_bits(double v) { return (unsigned long long)__double_as_longlong(v); }
-__device__ inline float bits_val(unsigned int b, float) { return __uint_as_float(b); }
-__device__ inline double bits_val(unsigned long long b, double) { return __longlong_as_double((long long)b); }
-
-// *p = max(*p, v) in the order std::max(*p, v) takes: replaced only if *p < v
-__device__ inline void atomic_max_val(float *p, float v)
+template <typename T, int IN_OV, int OUT_OV>
+static void launch_accumulate(int kernel, T *out, const SegmentPlan<T> &P, T max_val, const int32_t *frames32,
+                              hipStream_t st)
 {
-    unsigned int *q = (unsigned int *)p;
-    unsigned int old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    while (bits_val(old, v) < v) {
-        const unsigned int prev = atomicCAS(q, old, val_bits(v));
-        if (prev == old) break;
-        old = prev;
-    }
-}
-
-__device__ inline void atomic_max_val(double *p, double v)
-{
-    unsigned long long *q = (unsigned long long *)p;
-    unsigned long long old = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    while (bits_val(old, v) < v) {
-        const unsigned long long prev = atomicCAS(q, old, val_bits(v));
-        if (prev == old) break;
-        old = prev;
-    }
-}
-
-// accumulate_lines, step 3: the per-term images (img, touched: n_terms x hw,
-// in-term reduction done) combined into their frames with the out-term
-// reduction, bresenham.cpp:230-252.  Terms reach a frame in no fixed order,
-// as under the reference's OpenMP loop over terms.
-template <typename T, typename I, int OUT_OV>
-__global__ void k_terms_to_frames(T *__restrict__ out, const T *__restrict__ img, const uint8_t *__restrict__ touched,
-                                  const I *__restrict__ frames, int64_t n_terms, int64_t hw)
-{
-    const int64_t total = n_terms * hw;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += stride) {
-        if (!touched[q]) continue;
-        const int64_t t = q / hw;
-        T *dst = out + (int64_t)frames[t] * hw + (q - t * hw);
-        const T v = img[q];
-        if (OUT_OV == 0) atomicAdd(dst, v);
-        else atomic_max_val(dst, v);
-    }
+    launch_ordered_k<T, IN_OV, 32, 16, true, OUT_OV>(kernel, out, nullptr, nullptr, P, max_val, st, frames32);
 }
 
 // accumulate_lines (bresenham.cpp:151-256): lines grouped into terms, each
 // term reduced over its lines (in_overlap), terms reduced into frames
 // (out_overlap).  Equal overlaps compose to draw_lines with idxs =
-// frames[terms] on the tile path; mixed overlaps render every term in
-// reference line order into a per-term scratch (n_terms x H x W values and
-// touched flags, starting from zero like the reference's term buffer) and
-// combine the terms with k_terms_to_frames.
+// frames[terms] on the tile path.  Mixed overlaps bin the lines by (term,
+// tile) and run the ordered tile kernel in accumulate mode: each term tile
+// is reduced in LDS in reference line order from zero, like the reference's
+// term buffer, and its touched pixels are combined into frame frames[term]
+// with atomics (bresenham.cpp:230-248).  Terms reach a frame in no fixed
+// order, as under the reference's OpenMP loop over terms.
 template <typename T, typename I>
 static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const T *lines, int64_t n_lines,
                                  const I *terms, const I *frames, int64_t n_terms, T max_val, int kernel,
@@ -2098,13 +2103,15 @@ static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W,
     const int64_t n_chk = n_lines > n_terms ? n_lines : n_terms;
     const bool same = in_overlap == out_overlap;
     DevBuf<I> comp;
+    DevBuf<int32_t> frames32;
     if (n_chk > 0) {
         DevBuf<int> bad;
         CBX_HIP(bad.alloc(2, st));
         CBX_HIP(hipMemsetAsync(bad.ptr, 0, 2 * sizeof(int), st));
         if (same && n_lines > 0) CBX_HIP(comp.alloc(n_lines, st));
+        CBX_HIP(frames32.alloc(n_terms > 0 ? n_terms : 1, st));
         k_term_frames<I><<<grid_for(n_chk, 256), 256, 0, st>>>(terms, frames, n_lines, n_terms, n_frames,
-                                                                 comp.ptr, bad.ptr);
+                                                                 comp.ptr, frames32.ptr, bad.ptr);
         CBX_HIP(hipGetLastError());
         const void *src[2] = {bad.ptr, bad.ptr + 1};
         const int nb[2] = {4, 4};
@@ -2117,23 +2124,16 @@ static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W,
     if (same) return draw_lines_impl<T, I>(out, n_frames, H, W, lines, n_lines, comp.ptr, max_val, kernel,
                                            in_overlap, opts, stream);
 
-    const int64_t hw = H * W;
-    if (n_terms > (int64_t(1) << 62) / (hw * (int64_t)(sizeof(T) + 1)))
-        return fail(CBX_EVALUE, "the per-term scratch (n_terms x H x W) exceeds the supported range");
     SegmentPlan<T> P;
     if ((rc = build_plan<T, I>(P, lines, terms, kernel, n_lines, n_terms, H, W, 32, 16, true, st))) return rc;
     if (P.n_segs == 0) return CBX_OK;
     k_sort_tile_items<<<(unsigned)P.n_tiles, 256, 0, st>>>(P.tile_off.ptr, P.items.ptr, P.flags.ptr);
     CBX_HIP(hipGetLastError());
-    const int64_t npix = n_terms * hw;
-    DevBuf<T> img;
-    DevBuf<uint8_t> touched;
-    CBX_HIP(img.alloc(npix, st));
-    CBX_HIP(touched.alloc(npix, st));
-    // tiles without items never write their flags
-    CBX_HIP(hipMemsetAsync(touched.ptr, 0, npix, st));
-    if (in_overlap == 0) launch_ordered_k<T, 0, 32, 16, true>(kernel, img.ptr, touched.ptr, nullptr, P, max_val, st);
-    else launch_ordered_k<T, 1, 32, 16, true>(kernel, img.ptr, touched.ptr, nullptr, P, max_val, st);
+    {
+        KernelTimer timer("accumulate_tiles", st);
+        if (in_overlap == 0) launch_accumulate<T, 0, 1>(kernel, out, P, max_val, frames32.ptr, st);
+        else launch_accumulate<T, 1, 0>(kernel, out, P, max_val, frames32.ptr, st);
+    }
     CBX_HIP(hipGetLastError());
     int fl = 0;
     {
@@ -2145,14 +2145,7 @@ static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W,
     }
     if (fl & FLAG_TILE_FULL)
         return fail(CBX_ERUNTIME, "a tile holds more than 8192 segments; accumulate_lines cannot sort it");
-    if ((rc = flags_to_status(fl))) return rc;
-    const int64_t blocks = std::min<int64_t>((npix + 255) / 256, 65536);
-    if (out_overlap == 0)
-        k_terms_to_frames<T, I, 0><<<(unsigned)blocks, 256, 0, st>>>(out, img.ptr, touched.ptr, frames, n_terms, hw);
-    else
-        k_terms_to_frames<T, I, 1><<<(unsigned)blocks, 256, 0, st>>>(out, img.ptr, touched.ptr, frames, n_terms, hw);
-    CBX_HIP(hipGetLastError());
-    return CBX_OK;
+    return flags_to_status(fl);
 }
 
 template <typename T, int K>

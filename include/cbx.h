@@ -130,8 +130,10 @@ CBX_API int cbx_draw_pixels_f64(int64_t n_frames, int64_t height, int64_t width,
  * (n_frames x height x width, in place) with `out_overlap`
  * (CBX_SUM | CBX_MAX each).  A term or frame index out of range ->
  * CBX_EINDEX.  Equal overlaps are draw_lines with idxs = frames[terms]; mixed
- * overlaps use n_terms x height x width x (sizeof value + 1) bytes of
- * workspace.  opts as for draw_lines (the tile path of equal overlaps);
+ * overlaps reduce each term per pixel tile in the reference's line order and
+ * add (or max) the tile into its frame with atomics, so terms sharing a frame
+ * pixel combine in no fixed order.  opts as for draw_lines (the tile path of
+ * equal overlaps);
  * opts->kernels is not accepted.  Replaces bresenham.accumulate_lines
  * (bresenham.cpp:151-309, bound at 326-327) and
  * cuda_draw_lines.accumulate_lines (cuda_draw_lines.cu:948-1057); the f32
