@@ -1103,10 +1103,12 @@ constexpr int SORT_MAX = 8192;
 constexpr int FLAG_TILE_FULL = 8;
 
 __global__ void __launch_bounds__(256)
-k_sort_tile_items(const int *__restrict__ tile_off, int *__restrict__ items, int *__restrict__ flags)
+k_sort_tile_items(const int *__restrict__ tile_off, int *__restrict__ items, int *__restrict__ flags,
+                  const int *__restrict__ tile_list)
 {
     __shared__ int buf[SORT_MAX];
-    const int beg = tile_off[blockIdx.x], n = tile_off[blockIdx.x + 1] - beg;
+    const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
+    const int beg = tile_off[tile], n = tile_off[tile + 1] - beg;
     if (n <= 1) return;
     if (n > SORT_MAX) {
         if (threadIdx.x == 0) atomicOr(flags, FLAG_TILE_FULL);
@@ -1176,7 +1178,7 @@ __global__ void __launch_bounds__(64)
 k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__restrict__ first_line, Geometry g,
                 const int *__restrict__ tile_off, const int *__restrict__ items, const Segment<T> *__restrict__ segs,
                 const LineSetup<T> *__restrict__ setup, T max_val, int *__restrict__ flags,
-                const int32_t *__restrict__ term_frame)
+                const int32_t *__restrict__ term_frame, const int *__restrict__ tile_list)
 {
     static_assert(ACC < 0 || LIST, "accumulation starts from zero");
     static_assert(2 * SEG <= 64, "two lanes per column");
@@ -1185,7 +1187,7 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
     __shared__ ColRec<T> rec[64 * SEG];
     __shared__ ItemHdr hdr[64];
 
-    const int tile = blockIdx.x;
+    const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
     const int beg = tile_off[tile], end = tile_off[tile + 1];
     if (beg == end) return;
     const int per_frame = g.ntx * g.nty;
@@ -1881,14 +1883,18 @@ static int raw_list_pipeline(SegmentPlan<T> &P, int kernel, T max_val, int32_t *
     return CBX_OK;
 }
 
+// tile_list = nullptr: one workgroup per plan tile; otherwise one per entry of
+// tile_list (n_list tile ids).
 template <typename T, int OV, int TILE, int SEG, bool LIST, int ACC = -1>
 static void launch_ordered_k(int kernel, T *img, uint8_t *touched, int32_t *first, const SegmentPlan<T> &P,
-                             T max_val, hipStream_t st, const int32_t *term_frame = nullptr)
+                             T max_val, hipStream_t st, const int32_t *term_frame = nullptr,
+                             const int *tile_list = nullptr, int64_t n_list = 0)
 {
+    const unsigned grid = (unsigned)(tile_list ? n_list : P.n_tiles);
 #define CBX_ORD(KK)                                                                                             \
-    k_tiles_ordered<T, KK, OV, TILE, SEG, LIST, ACC><<<(unsigned)P.n_tiles, 64, 0, st>>>(                       \
+    k_tiles_ordered<T, KK, OV, TILE, SEG, LIST, ACC><<<grid, 64, 0, st>>>(                                      \
         img, touched, first, P.g, P.tile_off.ptr, P.items.ptr, P.segs.ptr, P.setup.ptr, max_val, P.flags.ptr,    \
-        term_frame)
+        term_frame, tile_list)
     switch (kernel) {
     case K_BIWEIGHT: CBX_ORD(K_BIWEIGHT); break;
     case K_GAUSSIAN: CBX_ORD(K_GAUSSIAN); break;
@@ -1908,7 +1914,7 @@ static int ordered_pipeline(SegmentPlan<T> &P, int kernel, int overlap, T max_va
 {
     if (count) *count = 0;
     if (P.n_segs == 0) return CBX_OK;
-    k_sort_tile_items<<<(unsigned)P.n_tiles, 256, 0, st>>>(P.tile_off.ptr, P.items.ptr, P.flags.ptr);
+    k_sort_tile_items<<<(unsigned)P.n_tiles, 256, 0, st>>>(P.tile_off.ptr, P.items.ptr, P.flags.ptr, nullptr);
     CBX_HIP(hipGetLastError());
     const int64_t npix = P.g.n_frames * (int64_t)P.g.H * P.g.W;
     DevBuf<T> scratch;
@@ -2069,11 +2075,30 @@ __global__ void k_term_frames(const I *__restrict__ terms, const I *__restrict__
     }
 }
 
+// The ids of the plan tiles that hold items (in no fixed order), one atomic
+// per wave: list[0 .. *count).
+__global__ void __launch_bounds__(256)
+k_nonempty_tiles(const int *__restrict__ tile_off, int64_t n_tiles, int *__restrict__ list, int *__restrict__ count)
+{
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    const bool ne = t < n_tiles && tile_off[t + 1] > tile_off[t];
+    const uint64_t m = __ballot(ne);
+    if (m == 0) return;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(count, __popcll(m));
+    base = __shfl(base, leader);
+    const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+    if (ne) list[base + below] = (int)t;
+}
+
 template <typename T, int IN_OV, int OUT_OV>
 static void launch_accumulate(int kernel, T *out, const SegmentPlan<T> &P, T max_val, const int32_t *frames32,
-                              hipStream_t st)
+                              const int *tile_list, int64_t n_list, hipStream_t st)
 {
-    launch_ordered_k<T, IN_OV, 32, 16, true, OUT_OV>(kernel, out, nullptr, nullptr, P, max_val, st, frames32);
+    launch_ordered_k<T, IN_OV, 32, 16, true, OUT_OV>(kernel, out, nullptr, nullptr, P, max_val, st, frames32,
+                                                     tile_list, n_list);
 }
 
 // accumulate_lines (bresenham.cpp:151-256): lines grouped into terms, each
@@ -2127,12 +2152,29 @@ static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W,
     SegmentPlan<T> P;
     if ((rc = build_plan<T, I>(P, lines, terms, kernel, n_lines, n_terms, H, W, 32, 16, true, st))) return rc;
     if (P.n_segs == 0) return CBX_OK;
-    k_sort_tile_items<<<(unsigned)P.n_tiles, 256, 0, st>>>(P.tile_off.ptr, P.items.ptr, P.flags.ptr);
+    // (term, tile) pairs are mostly empty: launch over the non-empty ones only
+    DevBuf<int> list, n_list;
+    CBX_HIP(list.alloc(P.n_tiles, st));
+    CBX_HIP(n_list.alloc(1, st));
+    CBX_HIP(hipMemsetAsync(n_list.ptr, 0, sizeof(int), st));
+    k_nonempty_tiles<<<grid_for(P.n_tiles, 256), 256, 0, st>>>(P.tile_off.ptr, P.n_tiles, list.ptr, n_list.ptr);
+    CBX_HIP(hipGetLastError());
+    int64_t nl = 0;
+    {
+        const void *src[1] = {n_list.ptr};
+        const int nb[1] = {4};
+        uint64_t hv = 0;
+        CBX_HIP(read_device_values(st, 1, src, nb, &hv));
+        nl = (int64_t)(uint32_t)hv;
+    }
+    if (nl < 0 || nl > P.n_tiles) return fail(CBX_ERUNTIME, "non-empty tile count out of range");
+    if (nl == 0) return CBX_OK;
+    k_sort_tile_items<<<(unsigned)nl, 256, 0, st>>>(P.tile_off.ptr, P.items.ptr, P.flags.ptr, list.ptr);
     CBX_HIP(hipGetLastError());
     {
         KernelTimer timer("accumulate_tiles", st);
-        if (in_overlap == 0) launch_accumulate<T, 0, 1>(kernel, out, P, max_val, frames32.ptr, st);
-        else launch_accumulate<T, 1, 0>(kernel, out, P, max_val, frames32.ptr, st);
+        if (in_overlap == 0) launch_accumulate<T, 0, 1>(kernel, out, P, max_val, frames32.ptr, list.ptr, nl, st);
+        else launch_accumulate<T, 1, 0>(kernel, out, P, max_val, frames32.ptr, list.ptr, nl, st);
     }
     CBX_HIP(hipGetLastError());
     int fl = 0;
