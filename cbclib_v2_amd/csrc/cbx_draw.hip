@@ -2076,21 +2076,29 @@ __global__ void k_term_frames(const I *__restrict__ terms, const I *__restrict__
 }
 
 // The ids of the plan tiles that hold items (in no fixed order), one atomic
-// per wave: list[0 .. *count).
+// per workgroup of 256: list[0 .. *count).
 __global__ void __launch_bounds__(256)
 k_nonempty_tiles(const int *__restrict__ tile_off, int64_t n_tiles, int *__restrict__ list, int *__restrict__ count)
 {
+    __shared__ int wcnt[4];
+    __shared__ int wbase[4];
     const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     const bool ne = t < n_tiles && tile_off[t + 1] > tile_off[t];
     const uint64_t m = __ballot(ne);
-    if (m == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((unsigned long long)m) - 1;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(count, __popcll(m));
-    base = __shfl(base, leader);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) wcnt[wave] = __popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int total = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+        const int base = total ? atomicAdd(count, total) : 0;
+        wbase[0] = base;
+        wbase[1] = base + wcnt[0];
+        wbase[2] = base + wcnt[0] + wcnt[1];
+        wbase[3] = base + wcnt[0] + wcnt[1] + wcnt[2];
+    }
+    __syncthreads();
     const int below = __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-    if (ne) list[base + below] = (int)t;
+    if (ne) list[wbase[wave] + below] = (int)t;
 }
 
 template <typename T, int IN_OV, int OUT_OV>
