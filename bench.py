@@ -49,45 +49,93 @@ def fail_multi(n: int) -> None:
     sys.exit(2)
 
 
-def cpu_baseline(lines, idxs, frames, H, W, seconds):
-    """Reference CPU path (the compiled reference modules, else the oracle port) on a bounded sample."""
+def host_cpu():
+    """Model, sockets, physical cores and logical CPUs of this host (from /proc/cpuinfo)."""
+    model, phys = 'unknown', set()
+    cur = {}
+    try:
+        for line in open('/proc/cpuinfo'):
+            if ':' not in line:
+                if 'physical id' in cur and 'core id' in cur:
+                    phys.add((cur['physical id'], cur['core id']))
+                cur = {}
+                continue
+            k, v = (x.strip() for x in line.split(':', 1))
+            cur[k] = v
+            if k == 'model name':
+                model = v
+        if 'physical id' in cur and 'core id' in cur:
+            phys.add((cur['physical id'], cur['core id']))
+    except OSError:
+        pass
+    sockets = len({p for p, _ in phys}) or 1
+    return {"model": model, "sockets": sockets, "physical_cores": len(phys) or (os.cpu_count() or 1),
+            "logical_cpus": os.cpu_count() or 1}
+
+
+def port_vs_reference():
+    """Ratio of the compiled reference's CPU time to the port's on the same
+    sample, measured in the build container (profiles/cpu_port_vs_reference.txt)."""
+    path = os.path.join(ROOT, 'profiles', 'cpu_port_vs_reference.txt')
+    if not os.path.exists(path):
+        return None
+    rows = {}
+    for line in open(path):
+        parts = line.split()
+        if line.startswith('#') or len(parts) != 6:
+            continue
+        rows[int(parts[0])] = float(parts[5])
+    return {"reference_time_over_port_time": rows, "where": "build container (8-core Xeon), not this host",
+            "source": "profiles/cpu_port_vs_reference.txt"} if rows else None
+
+
+def cpu_baseline(lines, idxs, frames, H, W, runs=5):
+    """CPU baseline: the oracle's OpenMP restatement of the reference's CPU
+    draw_lines + label (bresenham.cpp:62-94, label.cpp:109-175: per-thread
+    pixel buffers and a critical merge; per-thread chunks grown into regions)
+    on frames 0-3 of the workload, timed at this host's core share and at one
+    thread: median of `runs` runs after one warm-up, wall clock around the
+    native calls (the mask `out > 0.5` included)."""
     import numpy as np
     from oracle import oracle as O
-    ref = O.ref_module('bresenham')
-    lab = O.ref_module('label')
-    cores = min(16, os.cpu_count() or 1)
-    per_frame = int(np.sum(idxs == 0))
-    sample_frames = 2
+    from cbclib_v2_amd.label import Structure
+    cpu = host_cpu()
+    try:
+        share = len(os.sched_getaffinity(0))
+    except AttributeError:
+        share = os.cpu_count() or 1
+    cores = max(1, min(16, share, cpu["physical_cores"]))
+    sample_frames = 4
     sel = idxs < sample_frames
     ln, ix = lines[sel], idxs[sel]
+    shifts = Structure([0, 1, 1], 2).shifts()
     out = np.zeros((sample_frames, H, W), np.float32)
-    kind = 'reference' if ref is not None else 'port'
-    runs, t_draw, t_label = 0, 0.0, 0.0
-    t_start = time.perf_counter()
-    while runs == 0 or time.perf_counter() - t_start < seconds:
+
+    def one(threads):
         out[...] = 0
         t0 = time.perf_counter()
-        if ref is not None:
-            ref.draw_lines(out, ln, ix, 1.0, 'gaussian', 'sum', cores)
-        else:
-            O.draw_lines(out, ln, ix, 1.0, 'gaussian', 'sum')
+        O.draw_lines_omp(out, ln, ix, 1.0, 'gaussian', 'sum', threads)
         t1 = time.perf_counter()
-        mask = out > 0.5
-        if lab is not None:
-            lab.label(mask, lab.Structure([0, 1, 1], 2), 1, cores)
-        else:
-            from cbclib_v2_amd.label import Structure
-            O.label(mask, Structure([0, 1, 1], 2).shifts(), 1)
+        O.label_omp(out > 0.5, shifts, 1, threads)
         t2 = time.perf_counter()
-        t_draw += t1 - t0
-        t_label += t2 - t1
-        runs += 1
-    n = runs * len(ln)
-    return {"value": round(n / (t_draw + t_label) / 1e6, 6), "unit": "Mlines/s", "cores": cores if kind == 'reference'
-            else 1, "kind": kind,
-            "sample": f"{sample_frames} of the {frames} frames ({len(ln)} segments, {per_frame} per frame), "
-                      f"draw_lines + label of out > 0.5, {runs} runs in {t_draw + t_label:.1f} s; "
-                      f"draw_lines alone {n / t_draw / 1e6:.4f} Mlines/s"}
+        return t1 - t0, t2 - t1
+
+    res = {}
+    for threads in (cores, 1):
+        one(threads)
+        ts = [one(threads) for _ in range(runs)]
+        tot = sorted(a + b for a, b in ts)[len(ts) // 2]
+        draw = sorted(a for a, _ in ts)[len(ts) // 2]
+        res[threads] = (len(ln) / tot / 1e6, len(ln) / draw / 1e6)
+    return {"value": round(res[cores][0], 6), "unit": "Mlines/s", "cores": cores, "kind": "port",
+            "single_thread": round(res[1][0], 6),
+            "draw_only": {"cores": round(res[cores][1], 6), "single_thread": round(res[1][1], 6)},
+            "cpu_model": cpu["model"], "sockets": cpu["sockets"], "physical_cores": cpu["physical_cores"],
+            "logical_cpus": cpu["logical_cpus"],
+            "port_vs_reference": port_vs_reference(),
+            "sample": f"frames 0-{sample_frames - 1} of the {frames} ({len(ln)} segments): draw_lines (gaussian, sum) "
+                      "+ label(out > 0.5, Structure([0,1,1],2)) through the oracle's OpenMP restatement of the "
+                      f"reference CPU path; median of {runs} runs after 1 warm-up, at {cores} threads and at 1"}
 
 
 def sweep(out, lines, idxs, F, H, W, N):
@@ -236,7 +284,13 @@ def workload_c4(args):
     adj_ms, adj_n = _native.timing_read('adjoint_tiles')
     N = F * per
     step_bytes = 12.0 * F * H * W + 68.0 * N
+    adj_avg = adj_ms / max(adj_n, 1)
+    adj_bytes = 4.0 * F * H * W + 44.0 * N
+    adj_gbs = adj_bytes / (adj_avg * 1e-3) / 1e9 if adj_n else 0.0
     return {"metric": "fwd+adjoint frames/s (C4)", "value": round(F / (ms * 1e-3), 2), "unit": "frames/s",
+            "adjoint_roofline": {"bound": "hbm", "achieved": round(adj_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                 "frac": round(adj_gbs / HBM_PEAK_GBS, 4), "kernel": "k_adjoint_tiles",
+                                 "bytes_per_launch": adj_bytes, "avg_launch_ms": round(adj_avg, 4)},
             "ms_per_step": round(ms, 4), "Mlines_per_s": round(N / (ms * 1e-3) / 1e6, 4),
             "config": {"workload": f"forward + adjoint: {F} frames of {H}x{W} f32, {N} segments ({per}/frame, "
                                    "C2 distributions, seed 3), biweight, loss sum((render - target)^2)"},
@@ -302,8 +356,8 @@ def workload_acc(args):
     """accumulate_lines as the rotomap caller uses it (in-term max, out-term
     sum, gaussian): 16 frames of 512^2, 4000 terms of 16 segments (lengths
     U[5, 60], widths U[1, 3], seed 5), each term in a random frame.  Also
-    times the equal-overlap (sum/sum) path on the same input and the compiled
-    reference (else the oracle port) on the whole workload."""
+    times the equal-overlap (sum/sum) path on the same input; the output is
+    compared with the oracle on the whole workload."""
     import numpy as np
     import torch
     from cbclib_v2_amd import ndimage
@@ -332,21 +386,24 @@ def workload_acc(args):
            "config": {"workload": f"accumulate_lines: {F} frames of {H}x{W} f32, {T} terms x {per} segments "
                                   "(lengths U[5, 60], widths U[1, 3], seed 5), gaussian, in_overlap max, "
                                   "out_overlap sum"}}
+    # the output at this scale against the oracle (single-threaded restatement
+    # of bresenham.accumulate_lines), which is also the CPU baseline
+    from oracle import oracle as O
+    step('max', 'sum')
+    got = out.cpu().numpy()
+    o = np.zeros((F, H, W), np.float32)
+    t0 = time.perf_counter()
+    O.accumulate_lines(o, lines_np, terms_np, frames_np, 1.0, 'gaussian', 'max', 'sum')
+    dt = time.perf_counter() - t0
+    nz = o != 0
+    res["parity"] = {"support_equal": bool(np.array_equal(got != 0, nz)),
+                     "max_abs": float(np.abs(got - o).max()),
+                     "max_rel": float((np.abs(got - o)[nz] / np.abs(o[nz])).max()) if nz.any() else 0.0,
+                     "within_rtol_1e-5_atol_1e-6": bool(np.allclose(got, o, rtol=1e-5, atol=1e-6))}
     if not args.no_cpu:
-        from oracle import oracle as O
-        ref = O.ref_module('bresenham')
-        cores = min(16, os.cpu_count() or 1)
-        o = np.zeros((F, H, W), np.float32)
-        t0 = time.perf_counter()
-        if ref is not None:
-            ref.accumulate_lines(o, lines_np, terms_np, frames_np, 1.0, 'gaussian', 'max', 'sum', cores)
-        else:
-            O.accumulate_lines(o, lines_np, terms_np, frames_np, 1.0, 'gaussian', 'max', 'sum')
-        dt = time.perf_counter() - t0
-        res["cpu_baseline"] = {"value": round(N / dt / 1e6, 6), "unit": "Mlines/s",
-                               "cores": cores if ref is not None else 1,
-                               "kind": "reference" if ref is not None else "port",
-                               "sample": f"the whole workload once ({N} segments, {dt:.2f} s)"}
+        res["cpu_baseline"] = {"value": round(N / dt / 1e6, 6), "unit": "Mlines/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole workload once ({N} segments, {dt:.2f} s), the oracle's "
+                                         "single-threaded restatement of bresenham.accumulate_lines"}
     return res
 
 
@@ -361,11 +418,12 @@ def main() -> None:
     ap.add_argument('--threads', type=int, default=0)
     ap.add_argument('--sweep', action='store_true', help='time the tile kernel over launch shapes and exit')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
-    ap.add_argument('--cpu-seconds', type=float, default=12.0)
     ap.add_argument('--workload', choices=['c2', 'c3', 'c4', 'c5', 'acc'], default='c2',
                     help='c2 (default, the headline line); c3 batched uneven forward; c4 forward + adjoint; '
                          'c5 Adam refinement; acc accumulate_lines (in max, out sum)')
     ap.add_argument('--adam-steps', type=int, default=200)
+    ap.add_argument('--headline-only', action='store_true',
+                    help='the default workload only (no C3 / C4 summaries in the line)')
     ap.add_argument('--calibrate', action='store_true',
                     help='also run an elementwise kernel of known traffic (256 MiB read + 256 MiB written), the '
                          'reference for scaling FETCH_SIZE / WRITE_SIZE counters')
@@ -472,7 +530,7 @@ def main() -> None:
                      "kernel": "k_tiles (draw_lines tile accumulation)",
                      "bytes_per_launch": fwd_bytes, "avg_launch_ms": round(tile_avg, 4), "launches": tiles_n},
         "fwd_Mlines_per_s": round(N / (acc['fwd'] / 2 * 1e-3) / 1e6, 4),
-        "frames_per_s": round(F / (ms_step * 1e-3), 2),
+        "fwd_label_frames_per_s": round(F / (ms_step * 1e-3), 2),
         "breakdown_ms": {k: round(v / 2, 4) for k, v in acc.items()},
         "segments_walk_ms": round(walk_ms / max(walk_n, 1), 4),
         "label_total_ms": round(label_ms / max(label_n, 1), 4),
@@ -486,9 +544,22 @@ def main() -> None:
             torch.mul(src, 2.0, out=dst)
         torch.cuda.synchronize()
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(lines_np, idxs_np, F, H, W, args.cpu_seconds)
+        result["cpu_baseline"] = cpu_baseline(lines_np, idxs_np, F, H, W)
     else:
         result["cpu_baseline"] = None
+    if not args.headline_only:
+        # the other configurations, measured in the same run (summaries)
+        del out, lines, idxs
+        torch.cuda.empty_cache()
+        c3 = workload_c3(args)
+        result["c3"] = {k: c3[k] for k in ("ms_per_step", "value", "frames_per_s", "speedup_vs_per_frame_loop",
+                                           "per_frame_loop", "images_agree", "autotuned", "roofline", "config")}
+        result["c3"]["Mlines_per_s"] = result["c3"].pop("value")
+        torch.cuda.empty_cache()
+        c4 = workload_c4(args)
+        result["c4"] = {"fwd_adjoint_frames_per_s": c4["value"],
+                        **{k: c4[k] for k in ("ms_per_step", "Mlines_per_s", "step_GBs", "step_bytes",
+                                              "adjoint_tiles_ms", "adjoint_roofline", "config")}}
     print(json.dumps(result))
 
 

@@ -68,6 +68,18 @@ class Structure:
     def __repr__(self) -> str:
         return f"<Structure, connectivity = {self.connectivity}, rank = {self.rank}, size = {len(self)}>"
 
+    @classmethod
+    def from_reference(cls, structure: Any) -> 'Structure':
+        """This package's Structure for a reference one (cbclib_v2.label.Structure).
+
+        The reference binding exposes ``shape`` (2r + 1 per axis),
+        ``connectivity`` and ``rank`` (the number of axes), not the radii
+        (label.cpp:303-306), so the radii are recovered from ``shape``.
+        """
+        if isinstance(structure, cls):
+            return structure
+        return cls([d // 2 for d in structure.shape], structure.connectivity)
+
     def shifts(self) -> np.ndarray:
         """The non-zero shifts, (len(self) - 1, rank) int64."""
         return np.asarray(self._shifts[1:], dtype=np.int64).reshape(-1, self.rank)
@@ -196,6 +208,7 @@ class CPLabelResult(NamedTuple):
 
 
 def _shifts_for(structure: Structure, ndim: int) -> np.ndarray:
+    structure = Structure.from_reference(structure)
     if structure.rank != ndim:
         # label.cpp:114-118
         raise ValueError(f"input array dimension ({ndim}) does not match structure rank ({structure.rank})")

@@ -603,3 +603,367 @@ long cbxo_label(const void *inp, int dtype, int ndim, const int64_t *shape, cons
     free(par); free(cnt);
     return n;
 }
+
+/* ------------------------------------------------------------------------ */
+/* CPU baseline: the reference's multi-threaded CPU path restated with the  */
+/* same parallel structure (timed beside the GPU; not a parity checker).    */
+/*                                                                          */
+/* draw_lines (bresenham.cpp:62-94): an OpenMP team; each thread walks its  */
+/* share of the lines (`omp for nowait`, static schedule) into a private    */
+/* buffer of (flat index, frame, value) entries (the 24-byte tuples of      */
+/* ImageBuffer, bresenham.hpp:10-34), then merges the buffer into `out`     */
+/* inside one `omp critical` section.  With one thread the result is        */
+/* cbxo_draw_lines_*'s exactly (same order); with more, sums are in the     */
+/* reference's nondeterministic merge order.                                */
+/* ------------------------------------------------------------------------ */
+#include <omp.h>
+
+typedef struct { uint64_t idx, frame; float v; } ent_f32;   /* std::tuple<size_t, size_t, float> */
+typedef struct { uint64_t idx, frame; double v; } ent_f64;
+
+typedef struct {
+    void *a; long n, cap; size_t esz;
+    long H, W, frame; int kernel; double max_val;
+} omp_buf;
+
+static int buf_grow(omp_buf *b)
+{
+    long cap = b->cap ? 2 * b->cap : 4096;
+    void *p = realloc(b->a, (size_t)cap * b->esz);
+    if (!p) return 0;
+    b->a = p;
+    b->cap = cap;
+    return 1;
+}
+
+static void omp_px_f32(void *vctx, long x, long y, float err)
+{
+    omp_buf *b = (omp_buf *)vctx;
+    if (!(err <= 1.0f) || y < 0 || y >= b->H || x < 0 || x >= b->W) return;
+    if (b->n == b->cap && !buf_grow(b)) return;
+    ent_f32 *e = (ent_f32 *)b->a + b->n++;
+    e->idx = (uint64_t)(y * b->W + x);
+    e->frame = (uint64_t)b->frame;
+    e->v = (float)b->max_val * prof_f32(b->kernel, sqrtf(err));
+}
+
+static void omp_px_f64(void *vctx, long x, long y, double err)
+{
+    omp_buf *b = (omp_buf *)vctx;
+    if (!(err <= 1.0) || y < 0 || y >= b->H || x < 0 || x >= b->W) return;
+    if (b->n == b->cap && !buf_grow(b)) return;
+    ent_f64 *e = (ent_f64 *)b->a + b->n++;
+    e->idx = (uint64_t)(y * b->W + x);
+    e->frame = (uint64_t)b->frame;
+    e->v = b->max_val * prof_f64(b->kernel, sqrt(err));
+}
+
+int cbxo_draw_lines_omp_f32(float *out, long n_frames, long H, long W, const float *lines, long n_lines,
+                            const int32_t *idxs, float max_val, int kernel, int overlap, int threads)
+{
+    if (kernel < 0 || kernel > 4 || overlap < 0 || overlap > 1 || threads < 1) return CBXO_EVALUE;
+    int rc = check_idxs_i32(idxs, n_lines, n_frames);
+    if (rc) return rc;
+    int rc_all = CBXO_OK;
+    const long HW = H * W;
+#pragma omp parallel num_threads(threads)
+    {
+        omp_buf b = {NULL, 0, 0, sizeof(ent_f32), H, W, 0, kernel, max_val};
+#pragma omp for nowait
+        for (long i = 0; i < n_lines; i++) {
+            b.frame = frame_of(i, n_frames, idxs, 0);
+            int r = walk_f32(lines + 5 * i, omp_px_f32, &b);
+            if (r) {
+#pragma omp atomic write
+                rc_all = r;
+            }
+        }
+#pragma omp critical
+        {
+            const ent_f32 *e = (const ent_f32 *)b.a;
+            for (long k = 0; k < b.n; k++) {
+                float *o = out + e[k].idx + HW * e[k].frame;
+                if (overlap) *o = (*o < e[k].v) ? e[k].v : *o;
+                else *o = *o + e[k].v;
+            }
+        }
+        free(b.a);
+    }
+    return rc_all;
+}
+
+int cbxo_draw_lines_omp_f64(double *out, long n_frames, long H, long W, const double *lines, long n_lines,
+                            const int64_t *idxs, double max_val, int kernel, int overlap, int threads)
+{
+    if (kernel < 0 || kernel > 4 || overlap < 0 || overlap > 1 || threads < 1) return CBXO_EVALUE;
+    int rc = check_idxs_i64(idxs, n_lines, n_frames);
+    if (rc) return rc;
+    int rc_all = CBXO_OK;
+    const long HW = H * W;
+#pragma omp parallel num_threads(threads)
+    {
+        omp_buf b = {NULL, 0, 0, sizeof(ent_f64), H, W, 0, kernel, max_val};
+#pragma omp for nowait
+        for (long i = 0; i < n_lines; i++) {
+            b.frame = frame_of(i, n_frames, idxs, 1);
+            int r = walk_f64(lines + 5 * i, omp_px_f64, &b);
+            if (r) {
+#pragma omp atomic write
+                rc_all = r;
+            }
+        }
+#pragma omp critical
+        {
+            const ent_f64 *e = (const ent_f64 *)b.a;
+            for (long k = 0; k < b.n; k++) {
+                double *o = out + e[k].idx + HW * e[k].frame;
+                if (overlap) *o = (*o < e[k].v) ? e[k].v : *o;
+                else *o = *o + e[k].v;
+            }
+        }
+        free(b.a);
+    }
+    return rc_all;
+}
+
+/* ------------------------------------------------------------------------ */
+/* label (label.cpp:109-175): the flat input is cut into one chunk per      */
+/* thread; each thread grows a region (Region::dilate, label.hpp:396-425:   */
+/* breadth-first over the structure's shifts, keeping neighbours equal to   */
+/* the seed's value) from every unvisited truthy pixel of its chunk, marks  */
+/* the region's pixels of its chunk visited, and keeps the region when it   */
+/* has >= npts pixels and its smallest index lies in the chunk.  Regions    */
+/* are concatenated thread by thread, so ids follow the raster order of the */
+/* smallest index; the map is then written as LabelResult.to_array does     */
+/* (label.cpp:459-493).  Region membership uses a per-thread stamp array    */
+/* instead of the reference's std::set.                                     */
+/* ------------------------------------------------------------------------ */
+
+typedef struct { int32_t *px; long n, cap; long *roff; long nr, rcap; } region_list;
+
+static int push_long(long **a, long *n, long *cap, long v)
+{
+    if (*n == *cap) {
+        long c = *cap ? 2 * *cap : 1024;
+        long *p = (long *)realloc(*a, (size_t)c * sizeof(long));
+        if (!p) return 0;
+        *a = p;
+        *cap = c;
+    }
+    (*a)[(*n)++] = v;
+    return 1;
+}
+
+static int push_i32(int32_t **a, long *n, long *cap, int32_t v)
+{
+    if (*n == *cap) {
+        long c = *cap ? 2 * *cap : 4096;
+        int32_t *p = (int32_t *)realloc(*a, (size_t)c * sizeof(int32_t));
+        if (!p) return 0;
+        *a = p;
+        *cap = c;
+    }
+    (*a)[(*n)++] = v;
+    return 1;
+}
+
+long cbxo_label_omp(const void *inp, int dtype, int ndim, const int64_t *shape, const int64_t *shifts, long n_shifts,
+                    long npts, int32_t *out, int threads)
+{
+    if (ndim < 1 || ndim > 16 || threads < 1) return -CBXO_EVALUE;
+    long size = 1;
+    for (int d = 0; d < ndim; d++) size *= (long)shape[d];
+    if (size >= ((long)1 << 31)) return -CBXO_EVALUE;
+    memset(out, 0, sizeof(int32_t) * (size_t)size);
+    if (size == 0) return 0;
+    region_list *rl = (region_list *)calloc((size_t)threads, sizeof(region_list));
+    if (!rl) return -CBXO_ERUNTIME;
+    int failed = 0;
+#pragma omp parallel num_threads(threads)
+    {
+        const int tid = omp_get_thread_num();
+        const long chunk = (size + threads - 1) / threads;
+        const long t0 = tid * chunk, t1 = (t0 + chunk < size) ? t0 + chunk : size;
+        region_list *me = &rl[tid];
+        uint8_t *visited = (t1 > t0) ? (uint8_t *)calloc((size_t)(t1 - t0), 1) : NULL;
+        uint32_t *stamp = (uint32_t *)calloc((size_t)size, sizeof(uint32_t));
+        int32_t *reg = NULL;
+        long rn = 0, rcap = 0;
+        uint32_t cur = 0;
+        long coord[16];
+        if ((t1 > t0 && !visited) || !stamp) {
+#pragma omp atomic write
+            failed = 1;
+        } else {
+            for (long index = t0; index < t1; index++) {
+                if (visited[index - t0] || !val_at(inp, dtype, index)) continue;
+                const long root_v = val_at(inp, dtype, index);
+                cur++;
+                rn = 0;
+                stamp[index] = cur;
+                push_i32(&reg, &rn, &rcap, (int32_t)index);
+                for (long q = 0; q < rn; q++) {
+                    long rem = reg[q];
+                    for (int d = ndim - 1; d >= 0; d--) { coord[d] = rem % shape[d]; rem /= shape[d]; }
+                    for (long s = 0; s < n_shifts; s++) {
+                        const int64_t *sh = shifts + s * ndim;
+                        long k = 0, ok = 1;
+                        for (int d = 0; d < ndim; d++) {
+                            long c = coord[d] + (long)sh[d];
+                            if (c < 0 || c >= shape[d]) { ok = 0; break; }
+                            k = k * (long)shape[d] + c;
+                        }
+                        if (!ok || stamp[k] == cur || val_at(inp, dtype, k) != root_v) continue;
+                        stamp[k] = cur;
+                        push_i32(&reg, &rn, &rcap, (int32_t)k);
+                    }
+                }
+                long mn = reg[0];
+                for (long q = 0; q < rn; q++) {
+                    const long p = reg[q];
+                    if (p >= t0 && p < t1) visited[p - t0] = 1;
+                    if (p < mn) mn = p;
+                }
+                if (rn >= npts && mn >= t0 && mn < t1) {
+                    push_long(&me->roff, &me->nr, &me->rcap, me->n);
+                    for (long q = 0; q < rn; q++) push_i32(&me->px, &me->n, &me->cap, reg[q]);
+                }
+            }
+        }
+        free(reg);
+        free(stamp);
+        free(visited);
+    }
+    long n = 0;
+    if (!failed) {
+        long *first = (long *)malloc(sizeof(long) * (size_t)threads);
+        for (int t = 0; t < threads; t++) { first[t] = n; n += rl[t].nr; }
+#pragma omp parallel for num_threads(threads) schedule(static, 1)
+        for (int t = 0; t < threads; t++) {
+            const region_list *r = &rl[t];
+            for (long j = 0; j < r->nr; j++) {
+                const long a = r->roff[j], b = (j + 1 < r->nr) ? r->roff[j + 1] : r->n;
+                for (long q = a; q < b; q++) out[r->px[q]] = (int32_t)(first[t] + j + 1);
+            }
+        }
+        free(first);
+    }
+    for (int t = 0; t < threads; t++) { free(rl[t].px); free(rl[t].roff); }
+    free(rl);
+    return failed ? -CBXO_ERUNTIME : n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* accumulate_lines with one thread (bresenham.cpp:151-252): lines sorted by */
+/* term (stably here: ascending line order inside a term, one of the orders */
+/* the reference's unstable std::sort may give), terms visited in ascending */
+/* order.  Each term is drawn into a zeroed buffer; a pixel's index is      */
+/* recorded whenever it is drawn while its buffer value is still zero       */
+/* (possibly more than once), the buffer is updated with in_overlap         */
+/* (buffer + v, or std::max(buffer, v)), and after the term every recorded  */
+/* index is combined into frame frames[term] with out_overlap (buffer + out */
+/* or std::max(out, buffer)) and its buffer value reset to zero.            */
+/* Pinned by: the compiled reference (tests/test_accumulate.py) and the     */
+/* independent Python formulation oracle.accumulate_lines_py (a touched     */
+/* mask instead of the index list).                                         */
+/* ------------------------------------------------------------------------ */
+
+typedef struct {
+    void *buf; long H, W; int kernel, in_ov; double max_val;
+    long *idx; long n, cap; int fail;
+} acc_ctx;
+
+static void acc_push(acc_ctx *c, long i)
+{
+    if (c->n == c->cap) {
+        long cap = c->cap ? 2 * c->cap : 4096;
+        long *p = (long *)realloc(c->idx, (size_t)cap * sizeof(long));
+        if (!p) { c->fail = 1; return; }
+        c->idx = p;
+        c->cap = cap;
+    }
+    c->idx[c->n++] = i;
+}
+
+static void acc_px_f32(void *vctx, long x, long y, float err)
+{
+    acc_ctx *c = (acc_ctx *)vctx;
+    if (!(err <= 1.0f) || y < 0 || y >= c->H || x < 0 || x >= c->W) return;
+    const long i = y * c->W + x;
+    float *b = (float *)c->buf + i;
+    if (*b == 0.0f) acc_push(c, i);
+    const float v = (float)c->max_val * prof_f32(c->kernel, sqrtf(err));
+    if (c->in_ov) *b = (*b < v) ? v : *b;
+    else *b = *b + v;
+}
+
+static void acc_px_f64(void *vctx, long x, long y, double err)
+{
+    acc_ctx *c = (acc_ctx *)vctx;
+    if (!(err <= 1.0) || y < 0 || y >= c->H || x < 0 || x >= c->W) return;
+    const long i = y * c->W + x;
+    double *b = (double *)c->buf + i;
+    if (*b == 0.0) acc_push(c, i);
+    const double v = c->max_val * prof_f64(c->kernel, sqrt(err));
+    if (c->in_ov) *b = (*b < v) ? v : *b;
+    else *b = *b + v;
+}
+
+/* order[]: the lines stably sorted by term; toff[t]..toff[t+1] the lines of term t. */
+static int term_order(const void *terms, int wide, long n_lines, long n_terms, long **order, long **toff)
+{
+    *order = (long *)malloc(sizeof(long) * (size_t)(n_lines ? n_lines : 1));
+    *toff = (long *)calloc((size_t)n_terms + 2, sizeof(long));
+    if (!*order || !*toff) return CBXO_ERUNTIME;
+    for (long i = 0; i < n_lines; i++) {
+        long t = wide ? (long)((const int64_t *)terms)[i] : (long)((const int32_t *)terms)[i];
+        if (t < 0 || t >= n_terms) return CBXO_EINDEX;
+        (*toff)[t + 2]++;
+    }
+    for (long t = 0; t < n_terms; t++) (*toff)[t + 2] += (*toff)[t + 1];
+    for (long i = 0; i < n_lines; i++) {
+        long t = wide ? (long)((const int64_t *)terms)[i] : (long)((const int32_t *)terms)[i];
+        (*order)[(*toff)[t + 1]++] = i;
+    }
+    return CBXO_OK;
+}
+
+#define ACCUMULATE_BODY(REAL, WALK, PX, FR_T)                                                       \
+    if (kernel < 0 || kernel > 4 || in_ov < 0 || in_ov > 1 || out_ov < 0 || out_ov > 1)            \
+        return CBXO_EVALUE;                                                                         \
+    for (long t = 0; t < n_terms; t++)                                                              \
+        if (frames[t] < 0 || frames[t] >= n_frames) return CBXO_EINDEX;                             \
+    long *order = NULL, *toff = NULL;                                                               \
+    int rc = term_order(terms, sizeof(FR_T) == 8, n_lines, n_terms, &order, &toff);                 \
+    REAL *buf = (rc == CBXO_OK) ? (REAL *)calloc((size_t)(H * W ? H * W : 1), sizeof(REAL)) : NULL; \
+    if (rc == CBXO_OK && !buf) rc = CBXO_ERUNTIME;                                                  \
+    acc_ctx c = {buf, H, W, kernel, in_ov, max_val, NULL, 0, 0, 0};                                \
+    for (long t = 0; rc == CBXO_OK && t < n_terms; t++) {                                           \
+        REAL *dst = out + (long)frames[t] * H * W;                                                  \
+        for (long j = toff[t]; j < toff[t + 1] && rc == CBXO_OK; j++)                               \
+            rc = WALK(lines + 5 * order[j], PX, &c);                                                \
+        if (c.fail) rc = CBXO_ERUNTIME;                                                             \
+        for (long k = 0; rc == CBXO_OK && k < c.n; k++) {                                           \
+            const long i = c.idx[k];                                                                \
+            if (out_ov) dst[i] = (dst[i] < buf[i]) ? buf[i] : dst[i];                               \
+            else dst[i] = buf[i] + dst[i];                                                          \
+            buf[i] = (REAL)0;                                                                       \
+        }                                                                                           \
+        c.n = 0;                                                                                    \
+    }                                                                                               \
+    free(c.idx); free(buf); free(order); free(toff);                                                \
+    return rc;
+
+int cbxo_accumulate_lines_f32(float *out, long n_frames, long H, long W, const float *lines, long n_lines,
+                              const int32_t *terms, const int32_t *frames, long n_terms, float max_val, int kernel,
+                              int in_ov, int out_ov)
+{
+    ACCUMULATE_BODY(float, walk_f32, acc_px_f32, int32_t)
+}
+
+int cbxo_accumulate_lines_f64(double *out, long n_frames, long H, long W, const double *lines, long n_lines,
+                              const int64_t *terms, const int64_t *frames, long n_terms, double max_val, int kernel,
+                              int in_ov, int out_ov)
+{
+    ACCUMULATE_BODY(double, walk_f64, acc_px_f64, int64_t)
+}

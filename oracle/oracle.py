@@ -60,6 +60,12 @@ def lib() -> ctypes.CDLL:
         _lib.cbxo_adjoint_f64.argtypes = [P, L, L, L, P, P, L, P, D, I, P]
         _lib.cbxo_label.argtypes = [P, I, I, P, P, L, L, P]
         _lib.cbxo_label.restype = L
+        _lib.cbxo_draw_lines_omp_f32.argtypes = [P, L, L, L, P, L, P, F, I, I, I]
+        _lib.cbxo_draw_lines_omp_f64.argtypes = [P, L, L, L, P, L, P, D, I, I, I]
+        _lib.cbxo_label_omp.argtypes = [P, I, I, P, P, L, L, P, I]
+        _lib.cbxo_label_omp.restype = L
+        _lib.cbxo_accumulate_lines_f32.argtypes = [P, L, L, L, P, L, P, P, L, F, I, I, I]
+        _lib.cbxo_accumulate_lines_f64.argtypes = [P, L, L, L, P, L, P, P, L, D, I, I, I]
         _lib.cbxo_plant.argtypes = [I]
         _lib.cbxo_plant.restype = None
         _lib.cbxo_profile_f64.argtypes = [I, D]
@@ -141,6 +147,33 @@ def pixel_list(shape, lines: np.ndarray, idxs: Optional[np.ndarray] = None, max_
 def accumulate_lines(out: np.ndarray, lines: np.ndarray, terms: np.ndarray, frames: np.ndarray,
                      max_val: float = 1.0, kernel: str = 'rectangular', in_overlap: str = 'sum',
                      out_overlap: str = 'sum') -> np.ndarray:
+    """Single-threaded restatement of bresenham.accumulate_lines (bresenham.cpp:151-252) in C
+    (cbxo_accumulate_lines_*: the reference's recorded-index term buffer); ``out`` in place."""
+    if kernel not in KERNELS:
+        raise ValueError(f"kernel is missing for {kernel}")
+    if in_overlap not in OVERLAPS or out_overlap not in OVERLAPS:
+        raise ValueError(f"Invalid in_overlap and out_overlap keywords: {in_overlap} and {out_overlap}")
+    assert out.flags.c_contiguous and out.dtype in (np.float32, np.float64)
+    F, H, W = _frames(out.shape)
+    f64 = out.dtype == np.float64
+    it = np.int64 if f64 else np.int32
+    ln = np.ascontiguousarray(lines, dtype=out.dtype).reshape(-1, 5)
+    tm = np.ascontiguousarray(terms, dtype=it).ravel()
+    fr = np.ascontiguousarray(frames, dtype=it).ravel()
+    if np.any((fr < 0) | (fr >= F)):
+        raise IndexError("frames is outside of (0, n_frames)")
+    if tm.size != ln.shape[0]:
+        raise ValueError("Term indices is incompatible with number of lines")
+    fn = lib().cbxo_accumulate_lines_f64 if f64 else lib().cbxo_accumulate_lines_f32
+    rc = fn(_ptr(out), F, H, W, _ptr(ln), ln.shape[0], _ptr(tm), _ptr(fr), fr.size, float(max_val),
+            KERNELS[kernel], OVERLAPS[in_overlap], OVERLAPS[out_overlap])
+    _raise(rc, "Term index is out of range")
+    return out
+
+
+def accumulate_lines_py(out: np.ndarray, lines: np.ndarray, terms: np.ndarray, frames: np.ndarray,
+                        max_val: float = 1.0, kernel: str = 'rectangular', in_overlap: str = 'sum',
+                        out_overlap: str = 'sum') -> np.ndarray:
     """Single-threaded restatement of bresenham.accumulate_lines (bresenham.cpp:151-252); ``out`` in place.
 
     Terms are visited in ascending order (one order the reference's OpenMP
@@ -200,6 +233,90 @@ def render_closed_f64(shape, lines: np.ndarray, idxs: Optional[np.ndarray] = Non
     return out.reshape(shape)
 
 
+def _profile_f64(kernel: str, x: np.ndarray) -> np.ndarray:
+    """K(x) in FP64 on |x| <= 1 (include.hpp:446-459), vectorised."""
+    ax = np.abs(x)
+    if kernel == 'biweight':
+        return 0.9375 * np.maximum(1.0 - x * x, 0.0) ** 2
+    if kernel == 'gaussian':
+        return np.where(ax <= 1.0, 0.3989422804014327 * np.exp(-((3.0 * x) ** 2) / 2.0), 0.0)
+    if kernel == 'parabolic':
+        return 0.75 * np.maximum(1.0 - x * x, 0.0)
+    if kernel == 'rectangular':
+        return np.where(ax <= 1.0, 1.0, 0.0)
+    if kernel == 'triangular':
+        return np.maximum(1.0 - ax, 0.0)
+    raise ValueError(f"kernel is missing for {kernel}")
+
+
+def segment_window_f64(line, box, max_val: float = 1.0, kernel: str = 'biweight') -> np.ndarray:
+    """FP64 closed form max_val K(dist / w) of one line over the pixel box
+    (y0, y1, x0, x1) (inclusive), 0 where dist > w: the per-line term of
+    :func:`render_closed_f64` (the clamped projection of geometry.hpp:319-339)."""
+    x0, y0, x1, y1, w = (float(v) for v in line)
+    ya, yb, xa, xb = box
+    out = np.zeros((yb - ya + 1, xb - xa + 1))
+    if not w > 0.0 or out.size == 0:
+        return out
+    py, px = np.mgrid[ya:yb + 1, xa:xb + 1].astype(np.float64)
+    tx, ty = x1 - x0, y1 - y0
+    mag = tx * tx + ty * ty
+    cx, cy = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
+    if mag > 0.0:
+        r = np.clip(((px - cx) * tx + (py - cy) * ty) / mag, -0.5, 0.5)
+        qx, qy = r * tx + cx, r * ty + cy
+    else:
+        qx, qy = x0, y0
+    u = ((px - qx) ** 2 + (py - qy) ** 2) / (w * w)
+    inside = u <= 1.0
+    out[inside] = max_val * _profile_f64(kernel, np.sqrt(u[inside]))
+    return out
+
+
+def _line_box(line, H, W, pad=1.0):
+    x0, y0, x1, y1, w = (float(v) for v in line)
+    w = max(w, 0.0) + pad
+    xa = max(0, int(np.floor(min(x0, x1) - w)) - 1)
+    xb = min(W - 1, int(np.ceil(max(x0, x1) + w)) + 1)
+    ya = max(0, int(np.floor(min(y0, y1) - w)) - 1)
+    yb = min(H - 1, int(np.ceil(max(y0, y1) + w)) + 1)
+    return ya, yb, xa, xb
+
+
+def loss_fd_f64(images: dict, targets: dict, lines: np.ndarray, idxs: np.ndarray, pick, h: float = 1e-3,
+                max_val: float = 1.0, kernel: str = 'biweight') -> np.ndarray:
+    """FP64 central differences of loss = sum over frames of sum((I - T)^2)
+    w.r.t. (x0, y0, x1, y1, w) of the lines in ``pick``.
+
+    ``images[f]`` / ``targets[f]``: FP64 closed-form renders (H, W) of the
+    frames the picked lines lie in (:func:`render_closed_f64`).  Only the
+    line's own term changes, so each difference is evaluated on the line's
+    window: (L+ - L-) / 2h = sum (c+ - c-) (2 R + c+ + c-) / 2h, with R the
+    render minus the line's term minus the target.
+    """
+    out = np.zeros((len(pick), 5))
+    for n, i in enumerate(pick):
+        f = int(idxs[i])
+        img, tgt = images[f], targets[f]
+        H, W = img.shape
+        ln = np.asarray(lines[i], np.float64)
+        box = _line_box(ln, H, W, pad=1.0 + 4 * h)
+        ya, yb, xa, xb = box
+        if yb < ya or xb < xa:
+            continue
+        c0 = segment_window_f64(ln, box, max_val, kernel)
+        R = img[ya:yb + 1, xa:xb + 1] - c0 - tgt[ya:yb + 1, xa:xb + 1]
+        for j in range(5):
+            p = ln.copy()
+            p[j] += h
+            m = ln.copy()
+            m[j] -= h
+            cp = segment_window_f64(p, box, max_val, kernel)
+            cm = segment_window_f64(m, box, max_val, kernel)
+            out[n, j] = float(np.sum((cp - cm) * (2.0 * R + cp + cm))) / (2.0 * h)
+    return out
+
+
 def adjoint_f64(grad_img: np.ndarray, lines: np.ndarray, idxs: Optional[np.ndarray] = None,
                 max_val: float = 1.0, kernel: str = 'rectangular', walk: str = 'f32') -> np.ndarray:
     """FP64 gradient of sum(grad_img * image) w.r.t. (x0, y0, x1, y1, w) of every line.
@@ -236,6 +353,44 @@ def label(inp: np.ndarray, shifts: np.ndarray, npts: int = 1) -> Tuple[np.ndarra
     n = lib().cbxo_label(_ptr(a), dt, a.ndim, _ptr(shape), _ptr(sh), sh.shape[0], int(npts), _ptr(out))
     if n < 0:
         _raise(-n, "oracle label failed")
+    return out, int(n)
+
+
+def draw_lines_omp(out: np.ndarray, lines: np.ndarray, idxs: Optional[np.ndarray] = None, max_val: float = 1.0,
+                   kernel: str = 'rectangular', overlap: str = 'sum', threads: int = 1) -> np.ndarray:
+    """The reference's multi-threaded CPU draw_lines restated (bresenham.cpp:62-94): per-thread
+    pixel buffers merged in a critical section.  The CPU baseline; ``out`` in place."""
+    F, H, W = _frames(out.shape)
+    assert out.flags.c_contiguous
+    f64 = out.dtype == np.float64
+    ln = np.ascontiguousarray(lines, dtype=np.float64 if f64 else np.float32).reshape(-1, 5)
+    ix = None if idxs is None else np.ascontiguousarray(idxs, dtype=np.int64 if f64 else np.int32).ravel()
+    fn = lib().cbxo_draw_lines_omp_f64 if f64 else lib().cbxo_draw_lines_omp_f32
+    rc = fn(_ptr(out), F, H, W, _ptr(ln), ln.shape[0], _ptr(ix), float(max_val), KERNELS[kernel],
+            OVERLAPS[overlap], int(threads))
+    _raise(rc, "oracle draw_lines_omp failed")
+    return out
+
+
+def label_omp(inp: np.ndarray, shifts: np.ndarray, npts: int = 1, threads: int = 1) -> Tuple[np.ndarray, int]:
+    """The reference's multi-threaded CPU label restated (label.cpp:109-175): per-thread chunks,
+    region growing, regions kept by the thread holding their smallest index.  The CPU baseline."""
+    a = np.ascontiguousarray(inp)
+    if a.dtype == np.bool_:
+        a, dt = a.view(np.uint8), 0
+    elif a.dtype == np.int32:
+        dt = 1
+    elif a.dtype == np.int64:
+        dt = 2
+    else:
+        raise TypeError(a.dtype)
+    shape = np.asarray(a.shape, np.int64)
+    sh = np.ascontiguousarray(shifts, dtype=np.int64).reshape(-1, a.ndim)
+    out = np.empty(a.shape, np.int32)
+    n = lib().cbxo_label_omp(_ptr(a), dt, a.ndim, _ptr(shape), _ptr(sh), sh.shape[0], int(npts), _ptr(out),
+                             int(threads))
+    if n < 0:
+        _raise(-n, "oracle label_omp failed")
     return out, int(n)
 
 

@@ -10,8 +10,14 @@ available (the oracle's CPU tests compare against them bit for bit).
 Inputs are small and seeded; outputs are stored as computed (single thread,
 so sums are in ascending line order).  Run from the repository root:
 
-    make -C oracle ref && python tests/golden/make_golden.py
+    make -C oracle ref && python tests/golden/make_golden.py [draw label config1 ...]
+
+(no names: every fixture file).  ``config1.npz`` holds the reference's render
+of the 512x512 / 1000-segment workload (seed 0): the rectangular image whole,
+the triangular image as a SHA-256 of its bytes plus rows 200-263, the SHA-256
+of its raw pixel list (line, y, x as int32), and a LabelResult case.
 """
+import hashlib
 import os
 import sys
 
@@ -48,11 +54,54 @@ def draw_case(rng, dtype):
     return (F, H, W), lines.astype(dtype), idxs.astype(it)
 
 
+def sha256(*arrays) -> np.ndarray:
+    h = hashlib.sha256()
+    for a in arrays:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return np.frombuffer(h.digest(), np.uint8).copy()
+
+
+def config1_fixtures(bres, lab):
+    """The 512x512 / 1000-segment workload (tests/lines_gen.py:config1) through
+    the reference, and a regions case for the NumPy LabelResult path."""
+    from tests.lines_gen import config1
+    lines = config1()
+    fx = {'lines_sha256': sha256(lines)}
+    rect = np.zeros((512, 512), np.float32)
+    bres.draw_lines(rect, lines, None, 1.0, 'rectangular', 'sum', 1)
+    fx['rectangular'] = rect
+    tri = np.zeros((512, 512), np.float32)
+    bres.draw_lines(tri, lines, None, 1.0, 'triangular', 'sum', 1)
+    fx['triangular_sha256'] = sha256(tri)
+    fx['triangular_rows_200_263'] = tri[200:264].copy()
+    # raw pixel list: the reference keeps it internal (ImageBuffer), so its
+    # digest comes from the oracle, whose emission order is pinned by the
+    # images above and the draw fixtures
+    line, _, y, x, _, _ = O.pixel_list((512, 512), lines, None, 1.0, 'triangular')
+    fx['pixel_list_count'] = np.array([line.size], np.int64)
+    fx['pixel_list_sha256'] = sha256(line.astype(np.int32), y.astype(np.int32), x.astype(np.int32))
+    rng = np.random.default_rng(7)
+    mask = rng.random((50, 50)) > 0.6
+    res = lab.label(mask, lab.Structure([2, 2], 2), 1, 1)
+    n = len(res.regions)
+    fx['regions_mask'] = mask
+    fx['regions_map'] = res.to_array(np.arange(1, n + 1)).astype(np.int32)
+    fx['regions_first'] = np.array([list(r)[0] for r in res.regions], np.int64)
+    fx['regions_size'] = np.array([len(list(r)) for r in res.regions], np.int64)
+    np.savez_compressed(os.path.join(HERE, 'config1.npz'), **fx)
+    print("wrote config1.npz")
+
+
 def main():
     bres = O.ref_module('bresenham')
     lab = O.ref_module('label')
     if bres is None or lab is None:
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources exist")
+    names = sys.argv[1:] or ['draw', 'label', 'config1']
+    if 'config1' in names:
+        config1_fixtures(bres, lab)
+    if 'draw' not in names and 'label' not in names:
+        return
     rng = np.random.default_rng(20240601)
     draw = {}
     i = 0

@@ -161,12 +161,26 @@ def test_config1_parity(cuda, oracle):
     assert len(raw[0]) > 1_000_000
 
 
-def test_reference_module_on_box(cuda, ref_bresenham):
-    """When the compiled reference travelled with the tree, compare against it directly."""
+def test_config1_golden(cuda):
+    """Config 1 against the reference's own render (tests/golden/config1.npz,
+    written by the compiled reference): deterministic images bit-identical,
+    the raw pixel list identical by digest."""
+    import hashlib
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'config1.npz'))
     lines = config1()
-    ref = ref_bresenham.draw_lines(np.zeros((512, 512), np.float32), lines, None, 1.0, 'rectangular', 'sum', 1)
-    det = gpu_image((512, 512), lines, None, 'rectangular', 'sum', np.float32, max_val=1.0, deterministic=True)
-    assert np.array_equal(det, ref)
+    assert hashlib.sha256(lines.tobytes()).digest() == g['lines_sha256'].tobytes(), "config1 generator changed"
+    rect = gpu_image((512, 512), lines, None, 'rectangular', 'sum', np.float32, max_val=1.0, deterministic=True)
+    assert np.array_equal(rect, g['rectangular'])
+    tri = gpu_image((512, 512), lines, None, 'triangular', 'sum', np.float32, max_val=1.0, deterministic=True)
+    assert np.array_equal(tri[200:264], g['triangular_rows_200_263'])
+    assert hashlib.sha256(tri.tobytes()).digest() == g['triangular_sha256'].tobytes()
+    pl = ndimage.draw_pixels((512, 512), torch.from_numpy(lines).cuda(), None, 1.0, 'triangular', merge=False)
+    assert pl.line.numel() == int(g['pixel_list_count'][0])
+    h = hashlib.sha256()
+    for a in (pl.line, pl.y, pl.x):
+        h.update(a.cpu().numpy().astype(np.int32).tobytes())
+    assert h.digest() == g['pixel_list_sha256'].tobytes()
 
 
 def test_numpy_roundtrip_in_place(cuda, oracle):

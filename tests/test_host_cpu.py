@@ -116,3 +116,39 @@ def test_workspace_calls_without_gpu():
     assert _native.workspace_release() >= 0
     ms, n = _native.timing_read('draw_tiles')
     assert n >= 0 and ms >= 0.0
+
+
+class _RefShapedStructure:
+    """What the reference binding exposes of a Structure (label.cpp:303-306)."""
+
+    def __init__(self, radii, connectivity):
+        self.shape = [2 * r + 1 for r in radii]
+        self.connectivity = connectivity
+        self.rank = len(radii)
+
+
+def _integration_snippet(structure):
+    # the conversion INTEGRATION.md gives for the reference's functions.py label()
+    from cbclib_v2_amd import label as amd_label
+    return amd_label.Structure([d // 2 for d in structure.shape], structure.connectivity)
+
+
+def test_reference_structure_binding():
+    from cbclib_v2_amd.label import Structure
+    for radii, conn in (([1, 1], 1), ([1, 1], 2), ([0, 1, 1], 2), ([2, 1], 3), ([1, 1, 1], 3), ([3], 1)):
+        ref = Structure(radii, conn)
+        for st in (_integration_snippet(_RefShapedStructure(radii, conn)),
+                   Structure.from_reference(_RefShapedStructure(radii, conn))):
+            assert st.radii == radii and st.connectivity == conn and st.rank == len(radii)
+            assert np.array_equal(st.shifts(), ref.shifts())
+
+
+def test_reference_structure_binding_compiled(ref_label):
+    """The same conversion on the compiled reference's own Structure (where it is built)."""
+    from cbclib_v2_amd.label import Structure, _shifts_for
+    for radii, conn in (([1, 1], 1), ([1, 1], 2), ([0, 1, 1], 2), ([2, 1], 3), ([1, 1, 1], 3)):
+        rs = ref_label.Structure(radii, conn)
+        st = _integration_snippet(rs)
+        assert [list(s) for s in st] == [list(s) for s in rs]
+        assert np.array_equal(st.to_array(), rs.to_array())
+        assert np.array_equal(_shifts_for(rs, len(radii)), Structure(radii, conn).shifts())

@@ -73,17 +73,17 @@ def test_label_threshold_fused(path, oracle):
     assert np.array_equal(res.labels.cpu().numpy(), ref)
 
 
-def test_label_numpy_result(cuda, ref_label):
-    """NumPy input returns regions like the reference CPU LabelResult."""
-    rng = np.random.default_rng(7)
-    mask = rng.random((50, 50)) > 0.6
-    st = cl.Structure([2, 2], 2)
-    res = cl.label(mask, st)
-    ref = ref_label.label(mask, ref_label.Structure([2, 2], 2), 1, 1)
-    assert len(res.regions) == len(ref.regions)
-    for a, b in zip(res.regions, ref.regions):
-        assert list(a) == list(b)
-    assert np.array_equal(cl.labels(res), ref.to_array(np.arange(1, len(ref.regions) + 1)))
+def test_label_numpy_result(cuda):
+    """NumPy input returns regions like the reference CPU LabelResult
+    (tests/golden/config1.npz holds the reference's regions of this mask)."""
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), 'golden', 'config1.npz'))
+    mask = g['regions_mask']
+    res = cl.label(mask, cl.Structure([2, 2], 2))
+    assert len(res.regions) == len(g['regions_first'])
+    assert [r.first() for r in res.regions] == list(g['regions_first'])
+    assert [len(r) for r in res.regions] == list(g['regions_size'])
+    assert np.array_equal(cl.labels(res), g['regions_map'])
 
 
 def test_label_errors(cuda):

@@ -318,3 +318,25 @@ def test_label_pins_catch_planted_mistake(oracle):
         st = Structure([1, 1], 2)
         caught += not np.array_equal(oracle.label(inp, st.shifts(), 1)[0], scipy_label(inp, st, 1)[0])
     assert caught > 0
+
+
+def test_closed_form_windows_and_loss_fd(oracle):
+    """The per-line windows of the FD helper sum to render_closed_f64, and
+    its central differences of a quadratic loss agree with the oracle's FP64
+    adjoint of the same closed form (double walk pixel set)."""
+    rng = np.random.default_rng(41)
+    H, W = 60, 70
+    lines = random_segments(rng, 12, H, W, 5.0, 40.0, 2.0, 5.0)
+    idxs = np.zeros(12, np.int64)
+    img = oracle.render_closed_f64((1, H, W), lines, idxs, 1.0, 'biweight')[0]
+    acc = np.zeros((H, W))
+    for ln in lines:
+        box = oracle._line_box(ln, H, W)
+        ya, yb, xa, xb = box
+        acc[ya:yb + 1, xa:xb + 1] += oracle.segment_window_f64(ln, box, 1.0, 'biweight')
+    np.testing.assert_allclose(acc, img, rtol=1e-12, atol=1e-12)
+    tgt = oracle.render_closed_f64((1, H, W), lines + np.r_[rng.normal(0, 0.5, 4), 0.0], idxs, 1.0, 'biweight')[0]
+    fd = oracle.loss_fd_f64({0: img}, {0: tgt}, lines, idxs, range(12), 1e-4, 1.0, 'biweight')
+    an = oracle.adjoint_f64(2.0 * (img - tgt)[None], lines, idxs, 1.0, 'biweight', walk='f64')
+    nrm = np.linalg.norm(an, axis=1, keepdims=True)
+    assert (np.abs(fd - an) / nrm).max() < 1e-3
