@@ -492,12 +492,27 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Phase 1: replay a segment's walk from its checkpoint and record, for each
-// of its columns, the centre and errors at which the reference emits it.
-template <typename T, int SEG>
-__device__ __forceinline__ void walk_columns(const LineSetup<T> &L, const Segment<T> &sg, ColRec<T> *rr, int *flags)
+// Slot of column k of item i in a per-item array of SEG column records.
+// Item i's slots are rotated by (i SEG) / 64: the lanes of one wave then reach
+// distinct LDS banks both when each writes one column of its own item (phase
+// 1: 64 items, about the same k; unrotated, lane strides of SEG words put
+// every 64 / SEG-th lane in the same bank) and when they read all columns of
+// 64 / SEG items (phase 2).
+template <int SEG>
+__device__ __forceinline__ int rec_slot(int i, int k)
 {
-    for (int k = 0; k < SEG; k++) rr[k].s = INT_MIN;
+    static_assert((SEG & (SEG - 1)) == 0 && SEG <= 64, "power-of-two columns per segment");
+    return i * SEG + ((k + ((i * SEG) >> 6)) & (SEG - 1));
+}
+
+// Phase 1: replay a segment's walk from its checkpoint and record, for each
+// of its columns, the centre and errors at which the reference emits it, in
+// the column slots of item i of rr (rec_slot).
+template <typename T, int SEG>
+__device__ __forceinline__ void walk_columns(const LineSetup<T> &L, const Segment<T> &sg, ColRec<T> *rr, int i,
+                                             int *flags)
+{
+    for (int k = 0; k < SEG; k++) rr[rec_slot<SEG>(i, k)].s = INT_MIN;
     const int a = L.axis, b = 1 - a, sa = pick(L.st, a);
     WalkState<T> s;
     s.c[0] = sg.c[0];
@@ -513,9 +528,10 @@ __device__ __forceinline__ void walk_columns(const LineSetup<T> &L, const Segmen
         bool nx, ny;
         walk_next(L, s.Ne, nx, ny);
         if ((a == 0) ? nx : ny) {
-            rr[k].s = pick(s.c, b);
-            rr[k].Te = s.Te;
-            rr[k].Ne = s.Ne;
+            ColRec<T> &r = rr[rec_slot<SEG>(i, k)];
+            r.s = pick(s.c, b);
+            r.Te = s.Te;
+            r.Ne = s.Ne;
         }
         walk_step(L, s, nx, ny);
     }
@@ -692,6 +708,19 @@ __device__ __forceinline__ float fold_fix(float o, uint32_t acc, uint32_t zbit, 
     return (acc | zbit) ? (float)((double)o + add) : o;
 }
 
+// LDS row pitch of a TILE-wide pixel tile, in words: 4 words of padding per
+// row.  The fills of one item write the same short-axis coordinate in 8 or 16
+// consecutive rows (steep lines) or one column at rows s + j and s - j (the
+// two halves); with a pitch of TILE (a multiple of the bank count) those all
+// fall in one bank.  A pitch of TILE + 4 moves each row by 4 banks, and keeps
+// rows 16-byte aligned for the 4-pixel reads of the fold.
+template <int TILE>
+constexpr int tile_pitch() { return TILE + 4; }
+
+#ifndef CBX_KTILES_PAD
+#define CBX_KTILES_PAD 0
+#endif
+
 // Tile kernel.  The workgroup takes its tile's items ((segment, line) pairs)
 // in chunks of CH <= NT: phase 1 gives each of CH threads one item (the walk
 // replay, column records into LDS); phase 2 gives each thread one (item,
@@ -716,8 +745,10 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
 {
     using Acc = typename AccType<T, OV>::type;
     static_assert(OV != 2 || (FAST && sizeof(T) == 4), "fixed-point sums are the float fast path");
-    __shared__ alignas(16) Acc img[TILE * TILE];
-    __shared__ uint32_t zb[OV == 2 ? TILE * TILE / 32 : 1];
+    constexpr int PITCH = TILE + CBX_KTILES_PAD;
+    constexpr int LDS_PX = TILE * PITCH;   // LDS words of the tile (pixel (y, x) at y * PITCH + x)
+    __shared__ alignas(16) Acc img[LDS_PX];
+    __shared__ uint32_t zb[OV == 2 ? LDS_PX / 32 : 1];
     static_assert(CH <= NT, "one phase-1 thread per item");
     __shared__ int32_t rs[CH * SEG + CH];   // + one sink slot per phase-1 lane
     __shared__ T rT[CH * SEG + CH];
@@ -769,12 +800,12 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     }
 
     auto clear = [&]() {
-        for (int i = tid; i < TILE * TILE; i += NT) {
+        for (int i = tid; i < LDS_PX; i += NT) {
             if constexpr (OV == 0) img[i] = Acc(-0.0); // additive identity, keeps -0.0 pixels intact
             else img[i] = Acc(0);                      // below every max code / empty fixed-point sum
         }
         if constexpr (OV == 2)
-            for (int i = tid; i < TILE * TILE / 32; i += NT) zb[i] = 0u;
+            for (int i = tid; i < LDS_PX / 32; i += NT) zb[i] = 0u;
     };
     clear();
 
@@ -826,7 +857,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 }
                 frec[dst] = fill_rec(L, c0, sg.line);
 #pragma unroll
-                for (int k = 0; k < SEG; k++) rs[dst * SEG + k] = INT_MIN;
+                for (int k = 0; k < SEG; k++) rs[rec_slot<SEG>(dst, k)] = INT_MIN;
                 WalkState<T> s;
                 s.c[0] = sg.c[0];
                 s.c[1] = sg.c[1];
@@ -843,7 +874,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                     bool nx, ny;
                     walk_next(L, s.Ne, nx, ny);
                     const bool rec = act && ((a == 0) ? nx : ny);
-                    const int slot = rec ? dst * SEG + k : CH * SEG + tid;
+                    const int slot = rec ? rec_slot<SEG>(dst, k) : CH * SEG + tid;
                     rs[slot] = pick(s.c, 1 - a);
                     rT[slot] = s.Te;
                     rN[slot] = s.Ne;
@@ -857,16 +888,17 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
             for (int u = tid; u < n * SEG * 2; u += NT) {
                 const int i = u / (2 * SEG), rem = u - i * (2 * SEG);
                 const int k = rem >> 1, half = rem & 1;
-                const int cs = rs[i * SEG + k];
+                const int slot = rec_slot<SEG>(i, k);
+                const int cs = rs[slot];
                 if (cs == INT_MIN) continue;
                 const FillRec R = frec[i];
                 const int a = R.meta & FR_AXIS, b = 1 - a;
                 const int c = R.c0 + ((R.meta & FR_SA) ? k : -k);
                 const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
                 // LDS stride of the short axis and offset of the long-axis column
-                const int sstride = (a == 0) ? TILE : 1;
-                const int cbase = (a == 0) ? (c - X0) : (c - Y0) * TILE;
-                T Te = rT[i * SEG + k], Ne = rN[i * SEG + k];
+                const int sstride = (a == 0) ? PITCH : 1;
+                const int cbase = (a == 0) ? (c - X0) : (c - Y0) * PITCH;
+                T Te = rT[slot], Ne = rN[slot];
                 if constexpr (FAST && sizeof(T) == 4) {
                     if (R.meta & FR_SAFE) {
                         fill_half_fast<K, OV>(img, zb, R, setup[R.line], cs, Te, Ne, half, cbase, sstride, s_lo,
@@ -911,8 +943,9 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
 #pragma unroll
                 for (int j = 0; j < PV; j++) {
                     const int i = (tid + j * NT) * V;
-                    acc[j] = *reinterpret_cast<const uint4 *>(&img[i]);
-                    zw[j] = zb[i >> 5] >> (i & 31);
+                    const int li = (i / TILE) * PITCH + (i % TILE);
+                    acc[j] = *reinterpret_cast<const uint4 *>(&img[li]);
+                    zw[j] = zb[li >> 5] >> (li & 31);
                 }
 #pragma unroll
                 for (int j = 0; j < PV; j++) {
@@ -925,7 +958,8 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 for (int i = tid; i < TILE * TILE; i += NT) {
                     const int y = Y0 + i / TILE, x = X0 + (i % TILE);
                     if (y >= Y1 || x >= X1) continue;
-                    const double add = fix_value(img[i], zb, i, scale, (float)max_val, has);
+                    const int li = (i / TILE) * PITCH + (i % TILE);
+                    const double add = fix_value(img[li], zb, li, scale, (float)max_val, has);
                     T *o = fo + (int64_t)y * g.W + x;
                     if (has) *o = (T)((double)*o + add);
                 }
@@ -958,11 +992,12 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 const int i = (tid + j * NT) * V;
                 const int y = Y0 + i / TILE, x = X0 + (i % TILE);
                 if (y >= Y1) continue;
+                const int li = (i / TILE) * PITCH + (i % TILE);
                 float4 ov = pre[j];
-                ov.x = (float)((double)ov.x + img[i]);
-                ov.y = (float)((double)ov.y + img[i + 1]);
-                ov.z = (float)((double)ov.z + img[i + 2]);
-                ov.w = (float)((double)ov.w + img[i + 3]);
+                ov.x = (float)((double)ov.x + img[li]);
+                ov.y = (float)((double)ov.y + img[li + 1]);
+                ov.z = (float)((double)ov.z + img[li + 2]);
+                ov.w = (float)((double)ov.w + img[li + 3]);
                 *reinterpret_cast<float4 *>(fo + (int64_t)y * g.W + x) = ov;
             }
             return;
@@ -972,21 +1007,22 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
         const int y = Y0 + i / TILE, x = X0 + (i % TILE);
         if (y >= Y1) continue;
         T *o = fo + (int64_t)y * g.W + x;
+        const int li = (i / TILE) * PITCH + (i % TILE);
         if (full) {
             if constexpr (OV == 0) {
                 if constexpr (sizeof(T) == 4) {
                     float4 ov = *reinterpret_cast<float4 *>(o);
-                    ov.x = (float)((double)ov.x + img[i]);
-                    ov.y = (float)((double)ov.y + img[i + 1]);
-                    ov.z = (float)((double)ov.z + img[i + 2]);
-                    ov.w = (float)((double)ov.w + img[i + 3]);
+                    ov.x = (float)((double)ov.x + img[li]);
+                    ov.y = (float)((double)ov.y + img[li + 1]);
+                    ov.z = (float)((double)ov.z + img[li + 2]);
+                    ov.w = (float)((double)ov.w + img[li + 3]);
                     *reinterpret_cast<float4 *>(o) = ov;
                 } else {
-                    for (int j = 0; j < V; j++) o[j] = o[j] + img[i + j];
+                    for (int j = 0; j < V; j++) o[j] = o[j] + img[li + j];
                 }
             } else {
                 for (int j = 0; j < V; j++) {
-                    const Acc e = img[i + j];
+                    const Acc e = img[li + j];
                     if (e != Acc(0)) {
                         const T m = ord_dec(e);
                         const T ov = o[j];
@@ -998,9 +1034,9 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
             for (int j = 0; j < V; j++) {
                 if (x + j >= X1) break;
                 if constexpr (OV == 0) {
-                    o[j] = (T)((double)o[j] + img[i + j]);
+                    o[j] = (T)((double)o[j] + img[li + j]);
                 } else {
-                    const Acc e = img[i + j];
+                    const Acc e = img[li + j];
                     if (e != Acc(0)) {
                         const T m = ord_dec(e);
                         const T ov = o[j];
@@ -1189,7 +1225,10 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
 
     const int tile = tile_list ? tile_list[blockIdx.x] : (int)blockIdx.x;
     const int beg = tile_off[tile], end = tile_off[tile + 1];
-    if (beg == end) return;
+    // a tile too large to sort (k_sort_tile_items, earlier on the stream)
+    // fails the call: leave every output untouched
+    if (beg == end || (__hip_atomic_load(flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FLAG_TILE_FULL))
+        return;
     const int per_frame = g.ntx * g.nty;
     const int64_t f = tile / per_frame;
     const int r = tile - (int)f * per_frame;
@@ -1215,13 +1254,13 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
             const LineSetup<T> L = setup[sg.line];
             hdr[lane].seg = sid;
             hdr[lane].line = sg.line;
-            walk_columns<T, SEG>(L, sg, rec + lane * SEG, flags);
+            walk_columns<T, SEG>(L, sg, rec, lane, flags);
         }
         wave_sync();
         for (int j = 0; j < n; j++) {
             if (lane < 2 * SEG) {
                 const int k = lane >> 1, half = lane & 1;
-                const ColRec<T> cr = rec[j * SEG + k];
+                const ColRec<T> cr = rec[rec_slot<SEG>(j, k)];
                 if (cr.s != INT_MIN) {
                     const Segment<T> &sg = segs[hdr[j].seg];
                     const LineSetup<T> &L = setup[hdr[j].line];
@@ -1237,7 +1276,9 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
                         const T v = pixel_value<K>(err, max_val);
                         if (LIST && first[p] < 0) {
                             first[p] = line;
-                            acc[p] = v;
+                            // accumulate_lines' term buffer starts at zero: 0 + v, or
+                            // std::max(0, v) (bresenham.cpp:218-221)
+                            acc[p] = (ACC < 0) ? v : ((OV == 0) ? T(0) + v : ((T(0) < v) ? v : T(0)));
                         } else {
                             if (OV == 0) acc[p] = acc[p] + v;
                             else acc[p] = (acc[p] < v) ? v : acc[p];
@@ -1323,60 +1364,81 @@ __global__ void k_row_write(const uint8_t *__restrict__ touched, const T *__rest
 // adjoint (sum overlap): per (segment, tile) partial gradients, then an
 // ordered per-line reduction
 // ---------------------------------------------------------------------------
-// Per-line constants of the per-pixel gradient terms (double).
+// Per-line constants of the per-pixel gradient terms (double).  The tangent
+// is the exact difference of the end points (the walk's float tangent
+// p1 - p0 is rounded, which moves dist by ~1e-5 relative at these
+// magnitudes); cc and cw fold max_val, the profile's constant and the powers
+// of 1/w of the two gradient factors (see pixel_grad).
 struct GradConst {
-    double x0, y0, tx, ty, inv_mag, w, inv_w, inv_w2, sg;
+    double x0, y0, tx, ty, inv_mag, inv_w2, cc, cw;
 };
 
-template <typename T>
-__device__ __forceinline__ GradConst grad_const(const LineSetup<T> &L, T max_val)
+template <int K, typename T>
+__device__ __forceinline__ GradConst grad_const(const LineSetup<T> &L, const T *line, T max_val)
 {
     GradConst c;
     c.x0 = (double)L.p0[0];
     c.y0 = (double)L.p0[1];
-    c.tx = (double)L.t[0];
-    c.ty = (double)L.t[1];
+    c.tx = (double)line[2] - c.x0;
+    c.ty = (double)line[3] - c.y0;
     c.inv_mag = 1.0 / (c.tx * c.tx + c.ty * c.ty);
-    c.w = (double)L.w;
-    c.inv_w = 1.0 / c.w;
-    c.inv_w2 = c.inv_w * c.inv_w;
-    c.sg = (double)max_val;
+    const double inv_w = 1.0 / (double)L.w;
+    c.inv_w2 = inv_w * inv_w;
+    const double mv = (double)max_val;
+    // K(sqrt(u)) = k(u): dK/d(dist^2) = k'(u) / w^2 and dK/dw = k'(u) (-2 u / w);
+    // k'(u) = -1.875 (1 - u) (biweight), -0.75 (parabolic), -4.5 k(u) (gaussian)
+    double kp;
+    if constexpr (K == K_BIWEIGHT) kp = -1.875;
+    else if constexpr (K == K_PARABOLIC) kp = -0.75;
+    else if constexpr (K == K_GAUSSIAN) kp = -4.5 * 0.3989422804014327;
+    else kp = 0.0;
+    if constexpr (K == K_TRIANGULAR) {
+        // K = 1 - dist / w: dK/d(dist^2) = -1 / (2 w dist), dK/dw = dist / w^2
+        c.cc = mv * inv_w;
+        c.cw = mv * c.inv_w2;
+    } else {
+        c.cc = -2.0 * kp * mv * c.inv_w2;
+        c.cw = -2.0 * kp * mv * inv_w;
+    }
     return c;
 }
 
-// Per-pixel gradient terms of max_val K(dist / w) w.r.t. (x0, y0, x1, y1, w),
-// in double: the triangular profile's derivative carries the unit vector
-// (p - q) / dist, which float loses near the line.
+// Per-pixel gradient terms of max_val K(dist / w), in double.  With q the
+// clamped projection of p (parameter ts in [0, 1]) and (dx, dy) = p - q,
+// d(dist^2)/d(x0, y0) = -2 (1 - ts) (dx, dy) and d(dist^2)/d(x1, y1) =
+// -2 ts (dx, dy).  The accumulators hold
+//   acc[0..1] = sum c (dx, dy),  acc[2..3] = sum c ts (dx, dy),  acc[4] = dL/dw,
+// with c = -2 g max_val dK/d(dist^2); the end-point gradients are then
+// (acc[0] - acc[2], acc[1] - acc[3], acc[2], acc[3]) (grad_finish).
 template <int K>
 __device__ __forceinline__ void pixel_grad(const GradConst &C, int x, int y, double gv, double *acc)
 {
     const double rx = (double)x - C.x0, ry = (double)y - C.y0;
-    double ts = (rx * C.tx + ry * C.ty) * C.inv_mag;
+    double ts = __builtin_fma(ry, C.ty, rx * C.tx) * C.inv_mag;
     ts = (ts < 0.0) ? 0.0 : ((ts > 1.0) ? 1.0 : ts);
-    const double dx = rx - ts * C.tx, dy = ry - ts * C.ty;
-    const double d2 = dx * dx + dy * dy;
-    const double u = d2 * C.inv_w2;
-    double dk_dd2, dk_dw;
+    const double dx = __builtin_fma(-ts, C.tx, rx), dy = __builtin_fma(-ts, C.ty, ry);
+    const double d2 = __builtin_fma(dx, dx, dy * dy);
+    double c, gw;
     if constexpr (K == K_TRIANGULAR) {
         const double dist = sqrt(d2);
-        dk_dd2 = (dist > 0.0) ? -0.5 * C.inv_w / dist : 0.0;
-        dk_dw = dist * C.inv_w2;
+        c = (dist > 0.0) ? gv * C.cc / dist : 0.0;
+        gw = gv * dist;
     } else {
-        double k1;
-        if constexpr (K == K_GAUSSIAN) k1 = -4.5 * 0.3989422804014327 * ::exp(-4.5 * u);
-        else if constexpr (K == K_BIWEIGHT) k1 = -1.875 * (1.0 - u);
-        else if constexpr (K == K_PARABOLIC) k1 = -0.75;
-        else k1 = 0.0;
-        dk_dd2 = k1 * C.inv_w2;
-        dk_dw = -2.0 * k1 * u * C.inv_w;
+        const double u = d2 * C.inv_w2;
+        double f;   // k'(u) / k'(0)
+        if constexpr (K == K_BIWEIGHT) f = 1.0 - u;
+        else if constexpr (K == K_GAUSSIAN) f = ::exp(-4.5 * u);
+        else f = 1.0;
+        const double gf = gv * f;
+        c = gf * C.cc;
+        gw = gf * u;
     }
-    const double sg = gv * C.sg;
-    const double c = sg * dk_dd2 * -2.0;
-    acc[0] += c * (1.0 - ts) * dx;
-    acc[1] += c * (1.0 - ts) * dy;
-    acc[2] += c * ts * dx;
-    acc[3] += c * ts * dy;
-    acc[4] += sg * dk_dw;
+    const double cts = c * ts;
+    acc[0] = __builtin_fma(c, dx, acc[0]);
+    acc[1] = __builtin_fma(c, dy, acc[1]);
+    acc[2] = __builtin_fma(cts, dx, acc[2]);
+    acc[3] = __builtin_fma(cts, dy, acc[3]);
+    acc[4] = __builtin_fma(gw, C.cw, acc[4]);
 }
 
 // The fills of one column for the float fast path (both halves and the
@@ -1392,7 +1454,7 @@ __device__ __forceinline__ void adj_column_fast(const LineSetup<float> &L, const
     auto visit = [&](int s, float T, float N) {
         if (!(err_fast(L, LL, ea, T, N) <= 1.0f)) return;
         const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
-        const float gv = gt[(y - Y0) * TILE + (x - X0)];
+        const float gv = gt[(y - Y0) * tile_pitch<TILE>() + (x - X0)];
         if (gv != 0.0f) pixel_grad<K>(C, x, y, (double)gv, acc);
     };
     if (cs >= s_lo && cs < s_hi) visit(cs, Te0, Ne0);
@@ -1415,14 +1477,14 @@ __device__ __forceinline__ void adj_column_fast(const LineSetup<float> &L, const
 
 template <typename T, int K, int TILE, int SEG, int NT, int CH>
 __global__ void __launch_bounds__(NT)
-k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ tile_off,
+k_adjoint_tiles(const T *__restrict__ grad, const T *__restrict__ lines, Geometry g, const int *__restrict__ tile_off,
                 const int2 *__restrict__ pairs, const Segment<T> *__restrict__ segs,
                 const LineSetup<T> *__restrict__ setup, const int64_t *__restrict__ slot_base, T max_val,
                 double *__restrict__ partial, int *__restrict__ flags)
 {
     constexpr int NW = NT / 64;
     static_assert(64 % SEG == 0, "a segment's columns must share one round of a wave");
-    __shared__ T gt[TILE * TILE];
+    __shared__ T gt[TILE * tile_pitch<TILE>()];
     __shared__ ColRec<T> rec[NW][CH * SEG];
     __shared__ ItemHdr hdr[NW][CH];
 
@@ -1448,7 +1510,10 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
         gv[j] = (y < Y1 && x < X1) ? __builtin_nontemporal_load(&fg[(int64_t)y * g.W + x]) : T(0);
     }
 #pragma unroll
-    for (int j = 0; j < PER; j++) gt[threadIdx.x + j * NT] = gv[j];
+    for (int j = 0; j < PER; j++) {
+        const int i = threadIdx.x + j * NT;
+        gt[(i / TILE) * tile_pitch<TILE>() + (i % TILE)] = gv[j];
+    }
     __syncthreads();
 
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -1461,7 +1526,7 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
             const LineSetup<T> L = setup[it.y];
             hdr[wave][lane].seg = it.x;
             hdr[wave][lane].line = it.y;
-            walk_columns<T, SEG>(L, sg, R + lane * SEG, flags);
+            walk_columns<T, SEG>(L, sg, R, lane, flags);
         }
         wave_sync();
         const int rounds = (n * SEG + 63) / 64;
@@ -1470,14 +1535,14 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
             const int i = u / SEG, k = u - i * SEG;
             double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             if (i < n) {
-                const ColRec<T> cr = R[u];
+                const ColRec<T> cr = R[rec_slot<SEG>(i, k)];
                 if (cr.s != INT_MIN) {
                     const Segment<T> &sg = segs[hdr[wave][i].seg];
                     const LineSetup<T> &L = setup[hdr[wave][i].line];
                     const int a = L.axis;
                     const int c = pick(sg.c, a) + k * pick(L.st, a);
                     const int s_lo = (a == 0) ? Y0 : X0, s_hi = (a == 0) ? Y1 : X1;
-                    const GradConst C = grad_const(L, max_val);
+                    const GradConst C = grad_const<K>(L, lines + 5 * (int64_t)hdr[wave][i].line, max_val);
                     bool done = false;
                     if constexpr (sizeof(T) == 4) {
                         if (L.safe) {
@@ -1491,7 +1556,7 @@ k_adjoint_tiles(const T *__restrict__ grad, Geometry g, const int *__restrict__ 
                             const T err = err_of(L, Te, Ne);
                             if (!(err <= T(1))) return;
                             const int x = (a == 0) ? c : s, y = (a == 0) ? s : c;
-                            const T gv = gt[(y - Y0) * TILE + (x - X0)];
+                            const T gv = gt[(y - Y0) * tile_pitch<TILE>() + (x - X0)];
                             if (gv != T(0)) pixel_grad<K>(C, x, y, (double)gv, acc);
                         });
                     }
@@ -1539,8 +1604,12 @@ __global__ void k_line_grad(const int64_t *__restrict__ line_base, const LineSet
     for (int64_t s = s0; s < s1; s++)
 #pragma unroll
         for (int j = 0; j < 5; j++) acc[j] += partial[5 * s + j];
-#pragma unroll
-    for (int j = 0; j < 5; j++) grad_lines[5 * l + j] = (T)acc[j];
+    // (sum c dx - sum c ts dx, ..., sum c ts dx, ...): see pixel_grad
+    grad_lines[5 * l + 0] = (T)(acc[0] - acc[2]);
+    grad_lines[5 * l + 1] = (T)(acc[1] - acc[3]);
+    grad_lines[5 * l + 2] = (T)acc[2];
+    grad_lines[5 * l + 3] = (T)acc[3];
+    grad_lines[5 * l + 4] = (T)acc[4];
 }
 
 // ---------------------------------------------------------------------------
@@ -2154,7 +2223,11 @@ static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W,
         if (hv[0]) return fail(CBX_EINDEX, "a term index is outside of (0, len(frames))");
     }
     if (n_lines == 0 || n_frames * H * W == 0) return CBX_OK;
-    if (same) return draw_lines_impl<T, I>(out, n_frames, H, W, lines, n_lines, comp.ptr, max_val, kernel,
+    // max/max: the term buffer starts at zero and takes std::max(buffer, v),
+    // so with max_val < 0 every drawn pixel holds 0 when its term ends: the
+    // draw_lines max of zero values
+    if (same) return draw_lines_impl<T, I>(out, n_frames, H, W, lines, n_lines, comp.ptr,
+                                           (in_overlap == 1 && max_val < T(0)) ? T(0) : max_val, kernel,
                                            in_overlap, opts, stream);
 
     SegmentPlan<T> P;
@@ -2199,22 +2272,22 @@ static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W,
 }
 
 template <typename T, int K>
-static void launch_adjoint(const T *grad, const SegmentPlan<T> &P, const int64_t *slot_base, T max_val,
-                           double *partial, hipStream_t st)
+static void launch_adjoint(const T *grad, const T *lines, const SegmentPlan<T> &P, const int64_t *slot_base,
+                           T max_val, double *partial, hipStream_t st)
 {
     constexpr int NT = 256;
     if (P.g.tile == 64 && P.g.seg == 16)
         k_adjoint_tiles<T, K, 64, 16, NT, 16><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-            grad, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+            grad, lines, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
     else if (P.g.tile == 64)
         k_adjoint_tiles<T, K, 64, 8, NT, 32><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-            grad, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+            grad, lines, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
     else if (P.g.seg == 16)
         k_adjoint_tiles<T, K, 32, 16, NT, 16><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-            grad, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+            grad, lines, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
     else
         k_adjoint_tiles<T, K, 32, 8, NT, 32><<<(unsigned)P.n_tiles, NT, 0, st>>>(
-            grad, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
+            grad, lines, P.g, P.tile_off.ptr, P.pairs.ptr, P.segs.ptr, P.setup.ptr, slot_base, max_val, partial, P.flags.ptr);
 }
 
 template <typename T, typename I>
@@ -2249,10 +2322,10 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     CBX_HIP(partial.alloc(5 * P.n_items, st));
     KernelTimer timer("adjoint_tiles", st);
     switch (kernel) {
-    case K_BIWEIGHT: launch_adjoint<T, K_BIWEIGHT>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
-    case K_GAUSSIAN: launch_adjoint<T, K_GAUSSIAN>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
-    case K_PARABOLIC: launch_adjoint<T, K_PARABOLIC>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
-    default: launch_adjoint<T, K_TRIANGULAR>(grad, P, sbase.ptr, max_val, partial.ptr, st); break;
+    case K_BIWEIGHT: launch_adjoint<T, K_BIWEIGHT>(grad, lines, P, sbase.ptr, max_val, partial.ptr, st); break;
+    case K_GAUSSIAN: launch_adjoint<T, K_GAUSSIAN>(grad, lines, P, sbase.ptr, max_val, partial.ptr, st); break;
+    case K_PARABOLIC: launch_adjoint<T, K_PARABOLIC>(grad, lines, P, sbase.ptr, max_val, partial.ptr, st); break;
+    default: launch_adjoint<T, K_TRIANGULAR>(grad, lines, P, sbase.ptr, max_val, partial.ptr, st); break;
     }
     CBX_HIP(hipGetLastError());
     k_line_grad<T><<<grid_for(n_lines, blk), blk, 0, st>>>(line_base.ptr, P.setup.ptr, P.g, sbase.ptr, partial.ptr,

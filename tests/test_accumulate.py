@@ -41,13 +41,32 @@ def tol(dtype):
 @pytest.mark.parametrize('dtype', [np.float32, np.float64])
 @pytest.mark.parametrize('kernel', ['gaussian', 'rectangular', 'biweight'])
 @pytest.mark.parametrize('modes', MODES)
-def test_oracle_matches_compiled_reference(oracle, ref_bresenham, dtype, kernel, modes):
+@pytest.mark.parametrize('max_val', [0.8, -0.8])
+def test_oracle_matches_compiled_reference(oracle, ref_bresenham, dtype, kernel, modes, max_val):
+    """Negative max_val and negative frame values exercise the zero start of
+    the term buffer (std::max(0, v) in max mode)."""
     for seed in range(6):
         shape, lines, terms, frames = acc_case(100 + seed, dtype)
-        base = np.random.default_rng(seed).uniform(0.0, 0.5, shape).astype(dtype)
-        ref = ref_bresenham.accumulate_lines(base.copy(), lines, terms, frames, 0.8, kernel, modes[0], modes[1], 1)
-        got = oracle.accumulate_lines(base.copy(), lines, terms, frames, 0.8, kernel, modes[0], modes[1])
+        lo = 0.0 if max_val > 0 else -0.5
+        base = np.random.default_rng(seed).uniform(lo, 0.5, shape).astype(dtype)
+        ref = ref_bresenham.accumulate_lines(base.copy(), lines, terms, frames, max_val, kernel, modes[0], modes[1],
+                                             1)
+        got = oracle.accumulate_lines(base.copy(), lines, terms, frames, max_val, kernel, modes[0], modes[1])
         np.testing.assert_allclose(got, ref, **tol(dtype), err_msg=f"seed {seed}")
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+@pytest.mark.parametrize('modes', MODES)
+@pytest.mark.parametrize('max_val', [1.1, -0.7])
+def test_oracle_matches_python_formulation(oracle, dtype, modes, max_val):
+    """The C restatement (recorded-index term buffer) against the independent
+    Python one (touched mask over the pixel list): same order, same bits."""
+    for seed in range(5):
+        shape, lines, terms, frames = acc_case(150 + seed, dtype)
+        base = np.random.default_rng(seed).uniform(-0.5, 0.5, shape).astype(dtype)
+        a = oracle.accumulate_lines(base.copy(), lines, terms, frames, max_val, 'triangular', *modes)
+        b = oracle.accumulate_lines_py(base.copy(), lines, terms, frames, max_val, 'triangular', *modes)
+        assert np.array_equal(a.view(np.uint8), b.view(np.uint8)), f"seed {seed}"
 
 
 @pytest.mark.parametrize('kernel', KERNELS)
@@ -107,31 +126,50 @@ def test_oracle_index_errors(oracle):
 @pytest.mark.parametrize('dtype', [np.float32, np.float64])
 @pytest.mark.parametrize('kernel', KERNELS)
 @pytest.mark.parametrize('modes', MODES)
-def test_accumulate_matches_oracle(oracle, cuda, dtype, kernel, modes):
+@pytest.mark.parametrize('max_val', [0.8, -0.8])
+def test_accumulate_matches_oracle(oracle, cuda, dtype, kernel, modes, max_val):
     from cbclib_v2_amd import ndimage
     for seed in range(5):
         shape, lines, terms, frames = acc_case(500 + seed, dtype)
-        base = np.random.default_rng(seed).uniform(0.0, 0.5, shape).astype(dtype)
-        print(f"accumulate seed={seed} shape={shape} n={lines.shape[0]} modes={modes}", flush=True)
-        ref = oracle.accumulate_lines(base.copy(), lines, terms, frames, 0.8, kernel, *modes)
-        got = ndimage.accumulate_lines(base.copy(), lines, terms, frames, 0.8, kernel, *modes)
-        np.testing.assert_allclose(got, ref, **tol(dtype), err_msg=f"seed {seed}")
+        lo = 0.0 if max_val > 0 else -0.5
+        base = np.random.default_rng(seed).uniform(lo, 0.5, shape).astype(dtype)
+        print(f"accumulate seed={seed} shape={shape} n={lines.shape[0]} modes={modes} max_val={max_val}",
+              flush=True)
+        ref = oracle.accumulate_lines(base.copy(), lines, terms, frames, max_val, kernel, *modes)
+        got = ndimage.accumulate_lines(base.copy(), lines, terms, frames, max_val, kernel, *modes)
+        t = tol(dtype)
+        np.testing.assert_allclose(got, ref, rtol=t['rtol'], atol=t['atol'] * abs(max_val), err_msg=f"seed {seed}")
+
+
+def many_terms(seed, n, T, F, H, W):
+    rng = np.random.default_rng(seed)
+    lines = random_segments(rng, n, H, W, 2.0, 30.0, 0.5, 4.0).astype(np.float32)
+    terms = rng.integers(0, T, n).astype(np.int32)
+    frames = rng.integers(0, F, T).astype(np.int32)
+    return lines, terms, frames
 
 
 @pytest.mark.gpu
 @pytest.mark.parametrize('modes', MODES)
-def test_accumulate_many_terms(oracle, cuda, modes):
-    """2000 lines in 300 terms over 3 frames of 128 x 96."""
+@pytest.mark.parametrize('size', [(7, 2000, 300, 3, 128, 96), (8, 6000, 300, 4, 256, 256)])
+def test_accumulate_many_terms(oracle, cuda, modes, size):
+    """Many lines per term and many terms per frame (2000 lines in 300 terms
+    over 3 frames of 128 x 96; 6000 lines in 300 terms over 4 frames of 256^2).
+    Mixed overlaps reduce each term exactly in the reference's order and differ
+    from the oracle only in the order terms reach a frame; sum/sum runs the
+    fixed-point tile path (per term at most 2^-(clz(items)+1) * max_val from
+    the float sum, items <= 4095 per round): all within rtol 1e-5 / atol
+    1e-6 * max_val, with the support exact."""
     from cbclib_v2_amd import ndimage
-    rng = np.random.default_rng(7)
-    lines = random_segments(rng, 2000, 128, 96, 2.0, 30.0, 0.5, 4.0).astype(np.float32)
-    lines[:, [0, 2]], lines[:, [1, 3]] = lines[:, [0, 2]] * 0.75, lines[:, [1, 3]] * 1.33
-    terms = rng.integers(0, 300, 2000).astype(np.int32)
-    frames = rng.integers(0, 3, 300).astype(np.int32)
-    ref = oracle.accumulate_lines(np.zeros((3, 128, 96), np.float32), lines, terms, frames, 1.0, 'gaussian', *modes)
-    got = ndimage.accumulate_lines(np.zeros((3, 128, 96), np.float32), lines, terms, frames, 1.0, 'gaussian',
-                                   *modes)
-    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6 * 2000)
+    seed, n, T, F, H, W = size
+    lines, terms, frames = many_terms(seed, n, T, F, H, W)
+    ref = oracle.accumulate_lines(np.zeros((F, H, W), np.float32), lines, terms, frames, 1.0, 'gaussian', *modes)
+    got = ndimage.accumulate_lines(np.zeros((F, H, W), np.float32), lines, terms, frames, 1.0, 'gaussian', *modes)
+    assert np.array_equal(got != 0, ref != 0)
+    nz = ref != 0
+    print(f"{modes} {size}: max-abs {np.abs(got - ref).max():.3e}, max-rel "
+          f"{(np.abs(got - ref)[nz] / np.abs(ref[nz])).max():.3e}", flush=True)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6)
 
 
 @pytest.mark.gpu
