@@ -25,7 +25,7 @@ EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32
            'cbx_draw_lines_adjoint_f64', 'cbx_label', 'cbx_label_threshold_f32', 'cbx_timing_enable',
            'cbx_timing_reset', 'cbx_timing_read', 'cbx_workspace_release', 'cbx_workspace_bytes',
            'cbx_label_force_path', 'cbx_plan_create_f32', 'cbx_plan_draw_f32', 'cbx_plan_check', 'cbx_plan_bytes',
-           'cbx_plan_destroy')
+           'cbx_plan_destroy', 'cbx_label_noalloc', 'cbx_label_noalloc_capacity')
 
 
 class DrawOpts(ctypes.Structure):
@@ -71,6 +71,10 @@ def _declare(L: ctypes.CDLL) -> None:
         fn.argtypes = [P, I64, I64, I64, P, I64, P, R, I32, P, OP, P]
     L.cbx_label.restype = I32
     L.cbx_label.argtypes = [P, I32, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
+    L.cbx_label_noalloc.restype = I32
+    L.cbx_label_noalloc.argtypes = [P, I32, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
+    L.cbx_label_noalloc_capacity.restype = I64
+    L.cbx_label_noalloc_capacity.argtypes = []
     L.cbx_label_threshold_f32.restype = I32
     L.cbx_label_threshold_f32.argtypes = [P, F, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
     L.cbx_timing_enable.restype = None

@@ -34,14 +34,24 @@ int fail(int code, const std::string &msg);
 // A cache already open on another host thread is not shared: that call opens
 // a cache of its own.  Without an open Workspace, DevBuf falls back to
 // hipMalloc / hipFree.
+//
+// A fixed Workspace (the second constructor) carves its buffers out of one
+// caller-provided device region instead and never allocates: a request that
+// does not fit fails with hipErrorOutOfMemory and sets overflow().  While a
+// fixed Workspace is open, read_device_values and KernelTimer allocate
+// nothing either (no pinned mailbox, no events).
 class Workspace {
   public:
     explicit Workspace(hipStream_t s);
+    Workspace(hipStream_t s, void *base, size_t capacity);
     ~Workspace();
     Workspace(const Workspace &) = delete;
     Workspace &operator=(const Workspace &) = delete;
     hipError_t take(void **p, size_t bytes);
     hipError_t status() const { return status_; }
+    bool fixed() const { return fixed_base_ != nullptr; }
+    bool overflow() const { return overflow_; }
+    size_t requested() const { return requested_; }
     static Workspace *current();
 
   private:
@@ -49,7 +59,17 @@ class Workspace {
     Workspace *outer_ = nullptr;
     size_t block_ = 0, off_ = 0;
     hipError_t status_ = hipSuccess;
+    char *fixed_base_ = nullptr;
+    size_t fixed_cap_ = 0, requested_ = 0;
+    bool overflow_ = false;
 };
+
+// True while a fixed Workspace is open on this host thread.
+inline bool no_alloc_mode()
+{
+    const Workspace *ws = Workspace::current();
+    return ws && ws->fixed();
+}
 
 template <typename V>
 struct DevBuf {
@@ -91,7 +111,7 @@ struct KernelTimer {
     hipEvent_t a = nullptr, b = nullptr;
     KernelTimer(const char *n, hipStream_t s) : name(n), stream(s)
     {
-        if (!timing_on()) return;
+        if (!timing_on() || no_alloc_mode()) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
             a = b = nullptr;
             return;

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1155,11 +1156,18 @@ static int label_dispatch(In in, int ndim, const int64_t *shape, const int64_t *
 
     const int blk = 256;
     const int grid = grid_for(P, blk);
+    // every buffer before the first kernel: a failed allocation leaves `out` untouched
     DevBuf<int32_t> dsh;
     DevBuf<int64_t> doff;
+    DevBuf<int32_t> size, keep, rank;
     if (g.n_shifts) {
         CBX_HIP(dsh.alloc(sh.size(), st));
         CBX_HIP(doff.alloc(off.size(), st));
+    }
+    if (npts > 1) CBX_HIP(size.alloc(P, st));
+    CBX_HIP(keep.alloc(P + 1, st));
+    CBX_HIP(rank.alloc(P + 1, st));
+    if (g.n_shifts) {
         CBX_HIP(hipMemcpyAsync(dsh.ptr, sh.data(), sizeof(int32_t) * sh.size(), hipMemcpyHostToDevice, st));
         CBX_HIP(hipMemcpyAsync(doff.ptr, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice, st));
     }
@@ -1169,14 +1177,10 @@ static int label_dispatch(In in, int ndim, const int64_t *shape, const int64_t *
     k_flatten<<<grid, blk, 0, st>>>(P, out);
     CBX_HIP(hipGetLastError());
 
-    DevBuf<int32_t> size, keep, rank;
     if (npts > 1) {
-        CBX_HIP(size.alloc(P, st));
         CBX_HIP(hipMemsetAsync(size.ptr, 0, sizeof(int32_t) * P, st));
         k_count<<<grid, blk, 0, st>>>(P, out, size.ptr);
     }
-    CBX_HIP(keep.alloc(P + 1, st));
-    CBX_HIP(rank.alloc(P + 1, st));
     CBX_HIP(hipMemsetAsync(keep.ptr + P, 0, sizeof(int32_t), st));
     k_keep<<<grid, blk, 0, st>>>(P, out, npts > 1 ? size.ptr : nullptr, npts, keep.ptr);
     CBX_HIP(exclusive_scan_i32(keep.ptr, rank.ptr, P + 1, st));
@@ -1195,9 +1199,81 @@ static int label_dispatch(In in, int ndim, const int64_t *shape, const int64_t *
     return CBX_OK;
 }
 
+// ---------------------------------------------------------------------------
+// cbx_label_noalloc: the intermediate buffers come from `scratch`, a device
+// array of NOALLOC_BYTES that every GPU holds from the library's load until
+// the process ends.  In host code `scratch` names a host-side stand-in, not
+// the device array: its device address is looked up once per device with
+// hipGetSymbolAddress and only that address is used.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_dev[NOALLOC_MAX_DEVICES];
+
+// device address of this GPU's scratch array (the current device's)
+static int scratch_address(void **p)
+{
+    *p = nullptr;
+    int dev = 0;
+    CBX_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(CBX_ERUNTIME, "cbx_label_noalloc: device id out of range");
+    std::lock_guard<std::mutex> g(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *q = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !q)
+            return fail(CBX_ERUNTIME, std::string("cbx_label_noalloc: hipGetSymbolAddress(scratch): ") +
+                                          hipGetErrorString(e));
+        g_scratch_dev[dev] = q;
+    }
+    *p = g_scratch_dev[dev];
+    return CBX_OK;
+}
+
+template <typename Run>
+static int run_noalloc(hipStream_t st, Run &&run)
+{
+    void *base = nullptr;
+    int rc = scratch_address(&base);
+    if (rc) return rc;
+    Workspace ws(st, base, NOALLOC_BYTES);
+    rc = run();
+    if (rc != CBX_OK && ws.overflow())
+        return fail(CBX_EVALUE, "cbx_label_noalloc: the input needs more than the " +
+                                    std::to_string(NOALLOC_BYTES >> 20) + " MiB of scratch (at least " +
+                                    std::to_string(ws.requested()) + " bytes); use cbx_label");
+    return rc;
+}
+
 } // namespace cbx
 
 extern "C" {
+
+int cbx_label_noalloc(const void *inp, int dtype, int ndim, const int64_t *shape, const int64_t *shifts,
+                      int64_t n_shifts, int64_t npts, int32_t *out, int64_t *n_labels, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    return cbx::run_noalloc(st, [&]() {
+        switch (dtype) {
+        case CBX_BOOL:
+            return cbx::label_dispatch(cbx::ValueIn<uint8_t>{(const uint8_t *)inp}, ndim, shape, shifts, n_shifts,
+                                       npts, out, n_labels, st);
+        case CBX_INT32:
+            return cbx::label_dispatch(cbx::ValueIn<int32_t>{(const int32_t *)inp}, ndim, shape, shifts, n_shifts,
+                                       npts, out, n_labels, st);
+        case CBX_INT64:
+            return cbx::label_dispatch(cbx::ValueIn<int64_t>{(const int64_t *)inp}, ndim, shape, shifts, n_shifts,
+                                       npts, out, n_labels, st);
+        default:
+            return cbx::fail(CBX_EVALUE, "unsupported label input dtype");
+        }
+    });
+}
+
+int64_t cbx_label_noalloc_capacity(void) { return (int64_t)cbx::NOALLOC_BYTES; }
 
 int cbx_label(const void *inp, int dtype, int ndim, const int64_t *shape, const int64_t *shifts, int64_t n_shifts,
               int64_t npts, int32_t *out, int64_t *n_labels, void *stream)

@@ -118,8 +118,20 @@ Workspace::Workspace(hipStream_t s)
     g_ws_current = this;
 }
 
+Workspace::Workspace(hipStream_t, void *base, size_t capacity)
+    : fixed_base_(static_cast<char *>(base)), fixed_cap_(capacity)
+{
+    if (!base) status_ = hipErrorInvalidValue;
+    outer_ = g_ws_current;
+    g_ws_current = this;
+}
+
 Workspace::~Workspace()
 {
+    if (fixed_base_) {
+        g_ws_current = outer_;
+        return;
+    }
     if (!cache_) return;
     g_ws_current = outer_;
     std::lock_guard<std::mutex> g(g_ws_mu);
@@ -132,6 +144,16 @@ hipError_t Workspace::take(void **p, size_t bytes)
     if (status_ != hipSuccess) return status_;
     bytes = (bytes + WS_ALIGN - 1) / WS_ALIGN * WS_ALIGN;
     if (bytes == 0) bytes = WS_ALIGN;
+    if (fixed_base_) {
+        requested_ += bytes;
+        if (overflow_ || fixed_cap_ - off_ < bytes) {
+            overflow_ = true;
+            return hipErrorOutOfMemory;
+        }
+        *p = fixed_base_ + off_;
+        off_ += bytes;
+        return hipSuccess;
+    }
     auto &bl = cache_->blocks;
     while (block_ < bl.size()) {
         if (bl[block_].second - off_ >= bytes) {
@@ -216,8 +238,21 @@ struct Mailbox {
 hipError_t read_device_values(hipStream_t s, int n, const void *const *src, const int *bytes, uint64_t *out)
 {
     if (n < 1 || n > 4) return hipErrorInvalidValue;
+    hipError_t e;
+    if (no_alloc_mode()) {
+        // no pinned mailbox: one synchronous copy per value into host memory
+        for (int i = 0; i < n; i++) {
+            if (bytes[i] != 4 && bytes[i] != 8) return hipErrorInvalidValue;
+            uint64_t v = 0;
+            e = hipMemcpyAsync(&v, src[i], (size_t)bytes[i], hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return e;
+            out[i] = v;
+        }
+        return hipSuccess;
+    }
     int dev = 0;
-    hipError_t e = hipGetDevice(&dev);
+    e = hipGetDevice(&dev);
     if (e != hipSuccess) return e;
     thread_local std::vector<Mailbox> boxes;
     if ((int)boxes.size() <= dev) boxes.resize(dev + 1);

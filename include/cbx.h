@@ -175,6 +175,24 @@ CBX_API int cbx_label(const void *inp, int dtype, int ndim, const int64_t *shape
               int64_t npts, int32_t *out, int64_t *n_labels, void *stream);
 
 /*
+ * cbx_label without allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  Same arguments, results
+ * and errors as cbx_label; its intermediate buffers come from a device array
+ * of cbx_label_noalloc_capacity() bytes (64 MiB) that every GPU holds from
+ * the library's load until the process ends, so the call makes no hipMalloc /
+ * hipFree / pinned-host allocation.  An input that needs more scratch than
+ * that (about 2.2 bytes per element for binary inputs with a left neighbour,
+ * up to 12 for the general path) -> CBX_EVALUE before any output is written.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams or host threads) race and must be serialised by the
+ * caller.  Still waits on `stream` for *n_labels, like cbx_label.
+ */
+CBX_API int cbx_label_noalloc(const void *inp, int dtype, int ndim, const int64_t *shape, const int64_t *shifts,
+                              int64_t n_shifts, int64_t npts, int32_t *out, int64_t *n_labels, void *stream);
+CBX_API int64_t cbx_label_noalloc_capacity(void);
+
+/*
  * label of (image > vmin) without materialising the mask: the threshold of
  * the detection pipeline (RegionDetector.detect_regions,
  * data_processing.py:630-631) fused into the first labelling pass.
