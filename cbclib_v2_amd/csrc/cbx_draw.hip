@@ -492,27 +492,14 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Slot of column k of item i in a per-item array of SEG column records.
-// Item i's slots are rotated by (i SEG) / 64: the lanes of one wave then reach
-// distinct LDS banks both when each writes one column of its own item (phase
-// 1: 64 items, about the same k; unrotated, lane strides of SEG words put
-// every 64 / SEG-th lane in the same bank) and when they read all columns of
-// 64 / SEG items (phase 2).
-template <int SEG>
-__device__ __forceinline__ int rec_slot(int i, int k)
-{
-    static_assert((SEG & (SEG - 1)) == 0 && SEG <= 64, "power-of-two columns per segment");
-    return i * SEG + ((k + ((i * SEG) >> 6)) & (SEG - 1));
-}
-
 // Phase 1: replay a segment's walk from its checkpoint and record, for each
 // of its columns, the centre and errors at which the reference emits it, in
-// the column slots of item i of rr (rec_slot).
+// the column slots i * SEG + k of item i of rr.
 template <typename T, int SEG>
 __device__ __forceinline__ void walk_columns(const LineSetup<T> &L, const Segment<T> &sg, ColRec<T> *rr, int i,
                                              int *flags)
 {
-    for (int k = 0; k < SEG; k++) rr[rec_slot<SEG>(i, k)].s = INT_MIN;
+    for (int k = 0; k < SEG; k++) rr[i * SEG + k].s = INT_MIN;
     const int a = L.axis, b = 1 - a, sa = pick(L.st, a);
     WalkState<T> s;
     s.c[0] = sg.c[0];
@@ -528,7 +515,7 @@ __device__ __forceinline__ void walk_columns(const LineSetup<T> &L, const Segmen
         bool nx, ny;
         walk_next(L, s.Ne, nx, ny);
         if ((a == 0) ? nx : ny) {
-            ColRec<T> &r = rr[rec_slot<SEG>(i, k)];
+            ColRec<T> &r = rr[i * SEG + k];
             r.s = pick(s.c, b);
             r.Te = s.Te;
             r.Ne = s.Ne;
@@ -708,18 +695,14 @@ __device__ __forceinline__ float fold_fix(float o, uint32_t acc, uint32_t zbit, 
     return (acc | zbit) ? (float)((double)o + add) : o;
 }
 
-// LDS row pitch of a TILE-wide pixel tile, in words: 4 words of padding per
-// row.  The fills of one item write the same short-axis coordinate in 8 or 16
-// consecutive rows (steep lines) or one column at rows s + j and s - j (the
-// two halves); with a pitch of TILE (a multiple of the bank count) those all
-// fall in one bank.  A pitch of TILE + 4 moves each row by 4 banks, and keeps
-// rows 16-byte aligned for the 4-pixel reads of the fold.
+// LDS row pitch of the adjoint's gradient tile, in words: 4 words of padding
+// per row.  The lanes of one item read the same short-axis coordinate in 8 or
+// 16 consecutive rows (steep lines); with a pitch of TILE (a multiple of the
+// bank count) those reads fall in one bank, with TILE + 4 in distinct banks.
+// (The tile kernel keeps a pitch of TILE: its padded variants measured slower,
+// see k_tiles.)
 template <int TILE>
 constexpr int tile_pitch() { return TILE + 4; }
-
-#ifndef CBX_KTILES_PAD
-#define CBX_KTILES_PAD 0
-#endif
 
 // Tile kernel.  The workgroup takes its tile's items ((segment, line) pairs)
 // in chunks of CH <= NT: phase 1 gives each of CH threads one item (the walk
@@ -745,7 +728,12 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
 {
     using Acc = typename AccType<T, OV>::type;
     static_assert(OV != 2 || (FAST && sizeof(T) == 4), "fixed-point sums are the float fast path");
-    constexpr int PITCH = TILE + CBX_KTILES_PAD;
+    // LDS pitch TILE: the bank conflicts of the fills (steep lines write one
+    // bank from consecutive rows, the two halves of a column share a bank) and
+    // of the strided item records cost less than removing them did (a padded
+    // pitch and rotated record slots spilled registers at 5 waves per SIMD or
+    // dropped to 4)
+    constexpr int PITCH = TILE;
     constexpr int LDS_PX = TILE * PITCH;   // LDS words of the tile (pixel (y, x) at y * PITCH + x)
     __shared__ alignas(16) Acc img[LDS_PX];
     __shared__ uint32_t zb[OV == 2 ? LDS_PX / 32 : 1];
@@ -857,7 +845,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 }
                 frec[dst] = fill_rec(L, c0, sg.line);
 #pragma unroll
-                for (int k = 0; k < SEG; k++) rs[rec_slot<SEG>(dst, k)] = INT_MIN;
+                for (int k = 0; k < SEG; k++) rs[dst * SEG + k] = INT_MIN;
                 WalkState<T> s;
                 s.c[0] = sg.c[0];
                 s.c[1] = sg.c[1];
@@ -874,7 +862,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                     bool nx, ny;
                     walk_next(L, s.Ne, nx, ny);
                     const bool rec = act && ((a == 0) ? nx : ny);
-                    const int slot = rec ? rec_slot<SEG>(dst, k) : CH * SEG + tid;
+                    const int slot = rec ? dst * SEG + k : CH * SEG + tid;
                     rs[slot] = pick(s.c, 1 - a);
                     rT[slot] = s.Te;
                     rN[slot] = s.Ne;
@@ -888,7 +876,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
             for (int u = tid; u < n * SEG * 2; u += NT) {
                 const int i = u / (2 * SEG), rem = u - i * (2 * SEG);
                 const int k = rem >> 1, half = rem & 1;
-                const int slot = rec_slot<SEG>(i, k);
+                const int slot = i * SEG + k;
                 const int cs = rs[slot];
                 if (cs == INT_MIN) continue;
                 const FillRec R = frec[i];
@@ -1260,7 +1248,7 @@ k_tiles_ordered(T *__restrict__ img, uint8_t *__restrict__ touched, int32_t *__r
         for (int j = 0; j < n; j++) {
             if (lane < 2 * SEG) {
                 const int k = lane >> 1, half = lane & 1;
-                const ColRec<T> cr = rec[rec_slot<SEG>(j, k)];
+                const ColRec<T> cr = rec[j * SEG + k];
                 if (cr.s != INT_MIN) {
                     const Segment<T> &sg = segs[hdr[j].seg];
                     const LineSetup<T> &L = setup[hdr[j].line];
@@ -1535,7 +1523,7 @@ k_adjoint_tiles(const T *__restrict__ grad, const T *__restrict__ lines, Geometr
             const int i = u / SEG, k = u - i * SEG;
             double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
             if (i < n) {
-                const ColRec<T> cr = R[rec_slot<SEG>(i, k)];
+                const ColRec<T> cr = R[u];
                 if (cr.s != INT_MIN) {
                     const Segment<T> &sg = segs[hdr[wave][i].seg];
                     const LineSetup<T> &L = setup[hdr[wave][i].line];
