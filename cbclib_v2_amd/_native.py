@@ -25,7 +25,10 @@ EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32
            'cbx_draw_lines_adjoint_f64', 'cbx_label', 'cbx_label_threshold_f32', 'cbx_timing_enable',
            'cbx_timing_reset', 'cbx_timing_read', 'cbx_workspace_release', 'cbx_workspace_bytes',
            'cbx_label_force_path', 'cbx_plan_create_f32', 'cbx_plan_draw_f32', 'cbx_plan_check', 'cbx_plan_bytes',
-           'cbx_plan_destroy', 'cbx_label_noalloc', 'cbx_label_noalloc_capacity')
+           'cbx_plan_destroy', 'cbx_label_noalloc', 'cbx_label_noalloc_capacity', 'cbx_region_moments_f32',
+           'cbx_region_moments_f64', 'cbx_p_values_f32', 'cbx_p_values_f64')
+MOMENTS = {'total_mass': 0, 'mean': 1, 'center_of_mass': 2, 'moment_of_inertia': 3, 'covariance_matrix': 4,
+           'line_fit': 5}
 
 
 class DrawOpts(ctypes.Structure):
@@ -71,6 +74,13 @@ def _declare(L: ctypes.CDLL) -> None:
         fn.argtypes = [P, I64, I64, I64, P, I64, P, R, I32, P, OP, P]
     L.cbx_label.restype = I32
     L.cbx_label.argtypes = [P, I32, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
+    for name, R in (('f32', F), ('f64', D)):
+        fn = getattr(L, 'cbx_region_moments_' + name)
+        fn.restype = I32
+        fn.argtypes = [P, P, I32, P, P, I64, I32, P, P]
+        fn = getattr(L, 'cbx_p_values_' + name)
+        fn.restype = I32
+        fn.argtypes = [P, P, I32, P, P, I64, P, R, R, R, P, P, P]
     L.cbx_label_noalloc.restype = I32
     L.cbx_label_noalloc.argtypes = [P, I32, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
     L.cbx_label_noalloc_capacity.restype = I64

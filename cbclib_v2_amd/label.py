@@ -24,7 +24,8 @@ from . import _native, device
 from ._arrays import NUMPY, TORCH, dtype_of, namespace, shape_of, stream_ptr, to_device
 
 __all__ = ['Structure', 'Region', 'LabelResult', 'NPLabelResult', 'CPLabelResult', 'label', 'label_threshold',
-           'labels', 'index']
+           'labels', 'index', 'center_of_mass', 'covariance_matrix', 'ellipse_fit', 'line_fit', 'p_values',
+           'total_mass', 'mean', 'moment_of_inertia']
 
 
 def _compute_index(axis: int, size: int, msg: str) -> int:
@@ -277,3 +278,142 @@ def labels(labels: Union[LabelResult, CPLabelResult]):
     if isinstance(labels, CPLabelResult):
         return labels.labels
     raise ValueError("Invalid labels type. Expected NPLabelResult or CPLabelResult.")
+
+
+# ---------------------------------------------------------------------------
+# region moments and p-values (functions.py:401-641; label.cpp:176-255,
+# 534-583; cuda_label.cu:380-829)
+# ---------------------------------------------------------------------------
+def _label_map(lab: Union[LabelResult, CPLabelResult]):
+    """(int32 label map, int64 ids) on the GPU, and the namespace of results."""
+    if isinstance(lab, CPLabelResult):
+        return to_device(lab.labels, torch.int32), to_device(lab.index, torch.int64).reshape(-1), namespace(lab.labels)
+    if isinstance(lab, LabelResult):
+        n = len(lab)
+        m = lab.to_array(np.arange(1, n + 1), np.zeros(lab.shape, np.int32))
+        return to_device(m, torch.int32), to_device(np.arange(1, n + 1, dtype=np.int64), torch.int64), NUMPY
+    raise ValueError("Invalid labels type. Expected NPLabelResult or CPLabelResult.")
+
+
+def _data_for(data: Any, t_map: torch.Tensor) -> torch.Tensor:
+    shape = shape_of(data)
+    if len(shape) != t_map.dim():
+        # cuda_label.cu:495-499
+        raise ValueError(f"labels array dimension ({t_map.dim()}) does not match data array dimension "
+                         f"({len(shape)})")
+    if tuple(shape) != tuple(t_map.shape):
+        raise ValueError(f"data shape {tuple(shape)} does not match the label map shape {tuple(t_map.shape)}")
+    dt = dtype_of(data)
+    return to_device(data, torch.float32 if dt == np.float32 else torch.float64)
+
+
+def _region_moments(what: str, lab: Union[LabelResult, CPLabelResult], data: Any):
+    device.require_rocm(what)
+    t_map, t_idx, like = _label_map(lab)
+    t_data = _data_for(data, t_map)
+    N = t_map.dim()
+    n = t_idx.numel()
+    M = {'total_mass': 1, 'mean': N, 'center_of_mass': N, 'moment_of_inertia': N * N,
+         'covariance_matrix': N * N, 'line_fit': 2 * N}[what]
+    out = torch.empty((n, M), dtype=t_data.dtype, device=t_data.device)
+    f64 = t_data.dtype == torch.float64
+    fn = _native.lib().cbx_region_moments_f64 if f64 else _native.lib().cbx_region_moments_f32
+    shp = np.asarray(t_map.shape, dtype=np.int64)
+    _native.check(fn(t_map.data_ptr(), t_data.data_ptr(), N, shp.ctypes.data, t_idx.data_ptr(), n,
+                     _native.MOMENTS[what], out.data_ptr(), stream_ptr()))
+    if what == 'total_mass':
+        out = out.reshape(n)
+    elif what in ('moment_of_inertia', 'covariance_matrix'):
+        out = out.reshape(n, N, N)
+    from ._arrays import result
+    return result(out, like)
+
+
+def total_mass(labels: Union[LabelResult, CPLabelResult], data: Any):
+    """Sum of max(data, 0) over each region (label.cpp:534-541)."""
+    return _region_moments('total_mass', labels, data)
+
+
+def mean(labels: Union[LabelResult, CPLabelResult], data: Any):
+    """Raw first moment sum max(data, 0) * p of each region, zyx order (label.cpp:543-549)."""
+    return _region_moments('mean', labels, data)
+
+
+def center_of_mass(labels: Union[LabelResult, CPLabelResult], data: Any):
+    """Centre of mass of each region, (n, ndim) in the input's axis order
+    (functions.py:401-439); the region's first pixel when its mass is zero."""
+    return _region_moments('center_of_mass', labels, data)
+
+
+def moment_of_inertia(labels: Union[LabelResult, CPLabelResult], data: Any):
+    """Raw second moment matrix of each region, (n, ndim, ndim) (label.cpp:559-565)."""
+    return _region_moments('moment_of_inertia', labels, data)
+
+
+def covariance_matrix(labels: Union[LabelResult, CPLabelResult], data: Any):
+    """Covariance matrix of each region, (n, ndim, ndim) in the input's axis
+    order (functions.py:441-479); zero when the region's mass is zero."""
+    return _region_moments('covariance_matrix', labels, data)
+
+
+def _to_ellipse(matrix):
+    """functions.py:534-545 on a numpy array or a torch tensor."""
+    if isinstance(matrix, torch.Tensor):
+        xp, atan, stack = torch, torch.atan, torch.stack
+    else:
+        xp, atan, stack = np, np.arctan, np.stack
+    if matrix.shape[0] == 0:
+        return xp.zeros((0, 3), dtype=matrix.dtype) if xp is np else torch.zeros((0, 3), dtype=matrix.dtype,
+                                                                                   device=matrix.device)
+    mu_xx, mu_xy, mu_yy = matrix[..., -1, -1], matrix[..., -1, -2], matrix[..., -2, -2]
+    theta = 0.5 * atan(2 * mu_xy / (mu_xx - mu_yy))
+    delta = xp.sqrt(4 * mu_xy ** 2 + (mu_xx - mu_yy) ** 2)
+    a = xp.sqrt(2 * np.log(2) * (mu_xx + mu_yy + delta))
+    b = xp.sqrt(2 * np.log(2) * (mu_xx + mu_yy - delta))
+    return stack((a, b, theta), -1)
+
+
+def ellipse_fit(labels: Union[LabelResult, CPLabelResult], data: Any):
+    """FWHM major and minor axes and orientation (a, b, theta) of each region's
+    covariance ellipse, (n, 3) (functions.py:547-580)."""
+    return _to_ellipse(covariance_matrix(labels, data))
+
+
+def line_fit(labels: Union[LabelResult, CPLabelResult], data: Any):
+    """The major axis of each region's covariance ellipse as a segment,
+    (n, 2 ndim) end points in xyz order (functions.py:582-620;
+    CentralMomentsND::line, label.hpp:603-619)."""
+    return _region_moments('line_fit', labels, data)
+
+
+def p_values(labels: Union[LabelResult, CPLabelResult], lines: Any, data: Any, p0: float, vmin: float,
+             xtol: float, counts: bool = False):
+    """Log binomial tail of each region (functions.py:622-657, label.cpp:228-255):
+    of the n pixels of region j within ``xtol`` of ``lines[j]`` (shape
+    (n_regions, 2 ndim), xyz end points), k reach ``vmin``; returns
+    log P(X >= k), X ~ Bin(n, p0).  ``counts=True`` also returns the (n, k)
+    pairs (no reference counterpart)."""
+    device.require_rocm('p_values')
+    t_map, t_idx, like = _label_map(labels)
+    t_data = _data_for(data, t_map)
+    N = t_map.dim()
+    lshape = shape_of(lines)
+    if len(lshape) != 2 or lshape[1] != 2 * N:
+        # label.cpp:246-249
+        raise ValueError("lines array must have shape (n_lines, data.ndim() * 2)")
+    n = t_idx.numel()
+    if lshape[0] < n:
+        raise ValueError(f"lines has {lshape[0]} rows for {n} regions")
+    t_lines = to_device(lines, t_data.dtype)
+    out = torch.empty(n, dtype=t_data.dtype, device=t_data.device)
+    t_cnt = torch.empty((n, 2), dtype=torch.int64, device=t_data.device) if counts else None
+    f64 = t_data.dtype == torch.float64
+    fn = _native.lib().cbx_p_values_f64 if f64 else _native.lib().cbx_p_values_f32
+    shp = np.asarray(t_map.shape, dtype=np.int64)
+    _native.check(fn(t_map.data_ptr(), t_data.data_ptr(), N, shp.ctypes.data, t_idx.data_ptr(), n,
+                     t_lines.data_ptr(), float(p0), float(vmin), float(xtol), out.data_ptr(),
+                     None if t_cnt is None else t_cnt.data_ptr(), stream_ptr()))
+    from ._arrays import result
+    if counts:
+        return result(out, like), result(t_cnt, like)
+    return result(out, like)

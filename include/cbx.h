@@ -200,6 +200,47 @@ CBX_API int64_t cbx_label_noalloc_capacity(void);
 CBX_API int cbx_label_threshold_f32(const float *image, float vmin, int ndim, const int64_t *shape, const int64_t *shifts,
                             int64_t n_shifts, int64_t npts, int32_t *out, int64_t *n_labels, void *stream);
 
+/* Region quantities of cbx_region_moments (the reference's per-region functions, label.cpp:534-583). */
+#define CBX_TOTAL_MASS 0        /* (n)          sum of max(v, 0) */
+#define CBX_MEAN 1              /* (n, N)       raw first moment sum v p (MomentsND::first) */
+#define CBX_CENTER_OF_MASS 2    /* (n, N)       centroid, the region's first pixel when the mass is 0 */
+#define CBX_MOMENT_OF_INERTIA 3 /* (n, N, N)    raw second moment (MomentsND::second) */
+#define CBX_COVARIANCE_MATRIX 4 /* (n, N, N)    central second moment, zero when the mass is 0 */
+#define CBX_LINE_FIT 5          /* (n, 2N)      major axis end points (x0, y0, .., x1, y1, ..), xyz order */
+
+/*
+ * Region moments of a label map: for each of the n_index entries of `index`
+ * (DEVICE, int64 label ids), the quantity `what` of the region of pixels of
+ * `labels` (DEVICE int32, ndim = 2..7 dims of `shape`, HOST) holding that id,
+ * over the values max(data, 0) (DEVICE, same shape).  Coordinates and
+ * matrices in the input's axis order (zyx), line end points in xyz order,
+ * as the reference returns them.  Sums in double (the reference sums in the
+ * data's type).  Unsupported ndim -> CBX_EVALUE.  Replaces
+ * cuda_label.center_of_mass / covariance_matrix (cuda_label.cu:380-724) and
+ * label.{total_mass, mean, center_of_mass, moment_of_inertia,
+ * covariance_matrix, line_fit} (label.cpp:534-583); line_fit is
+ * functions.py:582-620 (to_line on the GPU).
+ */
+CBX_API int cbx_region_moments_f32(const int32_t *labels, const float *data, int ndim, const int64_t *shape,
+                                   const int64_t *index, int64_t n_index, int what, float *out, void *stream);
+CBX_API int cbx_region_moments_f64(const int32_t *labels, const double *data, int ndim, const int64_t *shape,
+                                   const int64_t *index, int64_t n_index, int what, double *out, void *stream);
+
+/*
+ * p_values of the regions (label.cpp:228-255, cuda_label.cu:726-829): for
+ * region j, n = its pixels within xtol of lines[j] (LineND::distance, the
+ * clamped projection; lines DEVICE, n_index x 2N, xyz end points) and k = those
+ * of them with data >= vmin; out[j] = log of the binomial upper tail
+ * P(X >= k), X ~ Bin(n, p0) (detail::logbinom, label.hpp:57-95).  counts
+ * (DEVICE, n_index x 2 int64: n, k) may be NULL.
+ */
+CBX_API int cbx_p_values_f32(const int32_t *labels, const float *data, int ndim, const int64_t *shape,
+                             const int64_t *index, int64_t n_index, const float *lines, float p0, float vmin,
+                             float xtol, float *out, int64_t *counts, void *stream);
+CBX_API int cbx_p_values_f64(const int32_t *labels, const double *data, int ndim, const int64_t *shape,
+                             const int64_t *index, int64_t n_index, const double *lines, double p0, double vmin,
+                             double xtol, double *out, int64_t *counts, void *stream);
+
 /*
  * Plans (no reference counterpart): the tile path of cbx_draw_lines_f32 split
  * into sizing and drawing, for repeated draws of similar line sets (batched

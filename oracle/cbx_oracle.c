@@ -967,3 +967,247 @@ int cbxo_accumulate_lines_f64(double *out, long n_frames, long H, long W, const 
 {
     ACCUMULATE_BODY(double, walk_f64, acc_px_f64, int64_t)
 }
+
+/* ------------------------------------------------------------------------ */
+/* Region moments (label.hpp:554-722 MomentsND / CentralMomentsND,          */
+/* label.cpp:534-583 total_mass, mean, center_of_mass, moment_of_inertia,   */
+/* covariance_matrix, line_fit) and p_values (label.cpp:228-255 with        */
+/* LineND::distance, geometry.hpp:319-339, and detail::logbinom,            */
+/* label.hpp:57-95).  A region is the set of pixels of a label map holding  */
+/* index[j], visited in ascending flat order (the reference's Region is a   */
+/* sorted set).  Arithmetic in T, in the reference's order: coordinates in  */
+/* xyz order (x = last axis) relative to the region's first pixel, value =  */
+/* std::max(value, 0), sums in T; outputs in zyx order like the reference.  */
+/* what: 0 total_mass (1), 1 mean (N: raw first moment), 2 center_of_mass   */
+/* (N), 3 moment_of_inertia (N*N raw), 4 covariance_matrix (N*N),           */
+/* 5 line_fit (2N, xyz end points).                                         */
+/* Pinned by: the compiled reference label module (golden fixtures and      */
+/* random inputs, tests/test_moments.py).                                   */
+/* ------------------------------------------------------------------------ */
+
+#define MOM_MAXN 7
+
+/* regions[j]: pixels of label index[j], ascending (CSR over a counting sort).
+ * own[j]: the first entry of index with the same label, which holds the
+ * pixels (later duplicates have empty ranges and read the owner's). */
+static int region_lists(const int32_t *labels, long size, const int64_t *index, long n_index, long **off, long **pix,
+                        long **own)
+{
+    int64_t maxl = 0;
+    for (long j = 0; j < n_index; j++) if (index[j] > maxl) maxl = index[j];
+    long *slot = (long *)malloc(sizeof(long) * (size_t)(maxl + 1));
+    *off = (long *)calloc((size_t)n_index + 1, sizeof(long));
+    *own = (long *)malloc(sizeof(long) * (size_t)(n_index ? n_index : 1));
+    if (!slot || !*off || !*own) { free(slot); return CBXO_ERUNTIME; }
+    for (long l = 0; l <= maxl; l++) slot[l] = -1;
+    for (long j = n_index - 1; j >= 0; j--) if (index[j] > 0) slot[index[j]] = j;  /* first occurrence wins */
+    for (long j = 0; j < n_index; j++) (*own)[j] = (index[j] > 0) ? slot[index[j]] : j;
+    for (long i = 0; i < size; i++) {
+        long l = labels[i];
+        if (l > 0 && l <= maxl && slot[l] >= 0) (*off)[slot[l] + 1]++;
+    }
+    for (long j = 0; j < n_index; j++) (*off)[j + 1] += (*off)[j];
+    *pix = (long *)malloc(sizeof(long) * (size_t)((*off)[n_index] ? (*off)[n_index] : 1));
+    long *cur = (long *)malloc(sizeof(long) * (size_t)(n_index ? n_index : 1));
+    if (!*pix || !cur) { free(slot); free(cur); return CBXO_ERUNTIME; }
+    for (long j = 0; j < n_index; j++) cur[j] = (*off)[j];
+    for (long i = 0; i < size; i++) {
+        long l = labels[i];
+        if (l > 0 && l <= maxl && slot[l] >= 0) (*pix)[cur[slot[l]]++] = i;
+    }
+    free(slot); free(cur);
+    return CBXO_OK;
+}
+
+static void point_of(long idx, int ndim, const int64_t *shape, long *xyz)
+{
+    for (int n = 0; n < ndim; n++) {           /* xyz[0] = last axis */
+        long d = shape[ndim - 1 - n];
+        xyz[n] = idx % d;
+        idx /= d;
+    }
+}
+
+#define MOMENTS_BODY(REAL, SQRT, ATAN2, COS, SIN)                                                       \
+    const int N = ndim;                                                                                 \
+    if (N < 1 || N > MOM_MAXN || what < 0 || what > 5 || (what == 5 && N < 2)) return CBXO_EVALUE;    \
+    long size = 1;                                                                                      \
+    for (int d = 0; d < N; d++) size *= (long)shape[d];                                                 \
+    long *off = NULL, *pix = NULL, *own = NULL;                                                         \
+    int rc = region_lists(labels, size, index, n_index, &off, &pix, &own);                             \
+    if (rc) { free(off); free(pix); free(own); return rc; }                                             \
+    const int NP = N * (N - 1) / 2;                                                                     \
+    int pi[MOM_MAXN * MOM_MAXN], pj[MOM_MAXN * MOM_MAXN];                                               \
+    { int a = 0, b = 0; for (int n = 0; n < NP; n++) { if (++b == N) { ++a; b = a + 1; } pi[n] = a; pj[n] = b; } } \
+    const int M = (what == 0) ? 1 : (what <= 2) ? N : (what <= 4) ? N * N : 2 * N;                   \
+    for (long j = 0; j < n_index; j++) {                                                                \
+        const long s = own[j];                                                                          \
+        const long a0 = off[s], a1 = off[s + 1];                                                        \
+        REAL org[MOM_MAXN] = {0}, mu = 0, mx[MOM_MAXN] = {0}, mxx[MOM_MAXN] = {0}, mxy[MOM_MAXN * MOM_MAXN] = {0}; \
+        long p[MOM_MAXN];                                                                               \
+        if (a1 > a0) {                                                                                  \
+            point_of(pix[a0], N, shape, p);                                                             \
+            for (int n = 0; n < N; n++) org[n] = (REAL)p[n];                                            \
+        }                                                                                               \
+        for (long q = a0; q < a1; q++) {                                                                \
+            point_of(pix[q], N, shape, p);                                                              \
+            REAL r[MOM_MAXN];                                                                           \
+            for (int n = 0; n < N; n++) r[n] = (REAL)p[n] - org[n];                                     \
+            REAL v = data[pix[q]];                                                                      \
+            v = (v < (REAL)0) ? (REAL)0 : v;                /* std::max(value, T()) */                 \
+            mu += v;                                                                                    \
+            for (int n = 0; n < N; n++) mx[n] += r[n] * v;                                              \
+            for (int n = 0; n < N; n++) mxx[n] += r[n] * r[n] * v;                                      \
+            for (int n = 0; n < NP; n++) mxy[n] += r[pi[n]] * r[pj[n]] * v;                             \
+        }                                                                                               \
+        REAL *o = out + j * M;                                                                          \
+        if (what == 0) { o[0] = mu; continue; }                                                         \
+        if (what == 1) { for (int n = 0; n < N; n++) o[N - 1 - n] = mx[n] + org[n] * mu; continue; }   \
+        if (what == 3) {                                                                                \
+            for (int n = 0; n < N; n++) { int z = N - n - 1;                                            \
+                o[n + N * n] = mxx[z] + 2 * org[z] * mx[z] + org[z] * org[z] * mu; }                    \
+            for (int n = 0; n < NP; n++) { int a = N - pi[n] - 1, b = N - pj[n] - 1;                    \
+                REAL m = mxy[n] + org[a] * mx[b] + org[b] * mx[a] + org[a] * org[b] * mu;               \
+                o[a + N * b] = m; o[b + N * a] = m; }                                                   \
+            continue;                                                                                   \
+        }                                                                                               \
+        /* central moments (CentralMomentsND): zero when the mass is zero */                          \
+        REAL cx[MOM_MAXN] = {0}, cxx[MOM_MAXN] = {0}, cxy[MOM_MAXN * MOM_MAXN] = {0};                    \
+        if (mu != (REAL)0) {                                                                            \
+            for (int n = 0; n < N; n++) cx[n] = mx[n] / mu;                                             \
+            for (int n = 0; n < N; n++) cxx[n] = mxx[n] / mu - cx[n] * cx[n];                           \
+            for (int n = 0; n < NP; n++) cxy[n] = mxy[n] / mu - cx[pi[n]] * cx[pj[n]];                  \
+        }                                                                                               \
+        if (what == 2) { for (int n = 0; n < N; n++) o[N - 1 - n] = cx[n] + org[n]; continue; }        \
+        if (what == 4) {                                                                                \
+            for (int n = 0; n < N; n++) o[n + N * n] = cxx[N - n - 1];                                  \
+            for (int n = 0; n < NP; n++) { int a = N - pi[n] - 1, b = N - pj[n] - 1;                    \
+                o[a + N * b] = cxy[n]; o[b + N * a] = cxy[n]; }                                         \
+            continue;                                                                                   \
+        }                                                                                               \
+        /* line(): theta = 0.5 atan2(2 mu_xy, mu_xx - mu_yy) (a double product) */                     \
+        REAL angle = (REAL)(0.5 * (double)ATAN2(2 * cxy[0], cxx[0] - cxx[1]));                          \
+        REAL c0[MOM_MAXN];                                                                              \
+        for (int n = 0; n < N; n++) c0[n] = cx[n] + org[n];                                             \
+        if (angle != angle) { for (int n = 0; n < N; n++) { o[n] = c0[n]; o[N + n] = c0[n]; } continue; } \
+        REAL tau[MOM_MAXN] = {0};                                                                       \
+        tau[0] = COS(angle); tau[1] = SIN(angle);                                                       \
+        REAL dl = SQRT(4 * cxy[0] * cxy[0] + (cxx[0] - cxx[1]) * (cxx[0] - cxx[1]));                    \
+        REAL hw = (REAL)sqrt(2 * log(2.0) * (double)(cxx[0] + cxx[1] + dl));                            \
+        for (int n = 0; n < N; n++) { o[n] = c0[n] + hw * tau[n]; o[N + n] = c0[n] - hw * tau[n]; }     \
+    }                                                                                                   \
+    free(off); free(pix); free(own);                                                                    \
+    return CBXO_OK;
+
+int cbxo_region_moments_f32(const int32_t *labels, const float *data, int ndim, const int64_t *shape,
+                            const int64_t *index, long n_index, int what, float *out)
+{
+    MOMENTS_BODY(float, sqrtf, atan2f, cosf, sinf)
+}
+
+int cbxo_region_moments_f64(const int32_t *labels, const double *data, int ndim, const int64_t *shape,
+                            const int64_t *index, long n_index, int what, double *out)
+{
+    MOMENTS_BODY(double, sqrt, atan2, cos, sin)
+}
+
+/* detail::logbinom (label.hpp:57-95): log of the binomial upper tail in T. */
+static double logaddexp_f64(double a, double b)
+{
+    if (a == -INFINITY) return b;
+    if (b == -INFINITY) return a;
+    double m = (a > b) ? a : b;
+    return m + log(exp(a - m) + exp(b - m));
+}
+
+static float logaddexp_f32(float a, float b)
+{
+    if (a == -INFINITY) return b;
+    if (b == -INFINITY) return a;
+    float m = (a > b) ? a : b;
+    return m + logf(expf(a - m) + expf(b - m));
+}
+
+/* The integer-argument calls (std::lgamma(n + 1), std::log(i)) take the
+ * double overloads whatever T is; k * log_p is computed in T; sums of a
+ * double and a T are double, assigned back to T. */
+#define LOGBINOM_BODY(REAL, LOG, LOG1P, ADDEXP)                                                        \
+    if (k <= 0) return (REAL)0;                                                                         \
+    if (k > n) return -INFINITY;                                                                        \
+    if (p <= (REAL)0) return (k <= 0) ? (REAL)0 : -INFINITY;                                            \
+    if (p >= (REAL)1) return (k <= n) ? (REAL)0 : -INFINITY;                                            \
+    REAL log_p = LOG(p), log_q = LOG1P(-p);                                                             \
+    REAL t = (REAL)(lgamma((double)(n + 1)) - lgamma((double)(k + 1)) - lgamma((double)(n - k + 1)) +  \
+                    (double)((REAL)k * log_p) + (double)((REAL)(n - k) * log_q));                       \
+    REAL tail = t;                                                                                      \
+    for (long i = k + 1; i < n + 1; ++i) {                                                              \
+        t = (REAL)((double)t + (log((double)(n - i + 1)) - log((double)i) + (double)log_p - (double)log_q)); \
+        tail = ADDEXP(tail, t);                                                                         \
+    }                                                                                                   \
+    return tail;
+
+static double logbinom_f64(long n, long k, double p) { LOGBINOM_BODY(double, log, log1p, logaddexp_f64) }
+static float logbinom_f32(long n, long k, float p) { LOGBINOM_BODY(float, logf, log1pf, logaddexp_f32) }
+
+double cbxo_logbinom_f64(long n, long k, double p) { return logbinom_f64(n, k, p); }
+float cbxo_logbinom_f32(long n, long k, float p) { return logbinom_f32(n, k, p); }
+
+/* LineND<T, N>::distance(point) (geometry.hpp:319-339) with an integer point. */
+#define SEGDIST_BODY(REAL, SQRT)                                                                        \
+    REAL tau[MOM_MAXN], ctr[MOM_MAXN], mag = 0;                                                         \
+    for (int n = 0; n < N; n++) tau[n] = ln[N + n] - ln[n];                                             \
+    for (int n = 0; n < N; n++) mag += tau[n] * tau[n];                                                 \
+    REAL q[MOM_MAXN];                                                                                   \
+    if (mag != (REAL)0) {                                                                               \
+        for (int n = 0; n < N; n++) ctr[n] = (ln[n] + ln[N + n]) / (REAL)2;                             \
+        REAL d = 0;                                                                                     \
+        for (int n = 0; n < N; n++) d += tau[n] * ((REAL)p[n] - ctr[n]);                                \
+        REAL rt = d / mag;                                                                              \
+        rt = (rt < (REAL)-0.5) ? (REAL)-0.5 : ((rt > (REAL)0.5) ? (REAL)0.5 : rt);                      \
+        for (int n = 0; n < N; n++) q[n] = rt * tau[n] + ctr[n];                                        \
+    } else {                                                                                            \
+        for (int n = 0; n < N; n++) q[n] = ln[n];                                                       \
+    }                                                                                                   \
+    REAL m2 = 0;                                                                                        \
+    for (int n = 0; n < N; n++) m2 += ((REAL)p[n] - q[n]) * ((REAL)p[n] - q[n]);                        \
+    return SQRT(m2);
+
+static double segdist_f64(const double *ln, const long *p, int N) { SEGDIST_BODY(double, sqrt) }
+static float segdist_f32(const float *ln, const long *p, int N) { SEGDIST_BODY(float, sqrtf) }
+
+#define PVALUES_BODY(REAL, DIST, LOGBINOM)                                                              \
+    const int N = ndim;                                                                                 \
+    if (N < 1 || N > MOM_MAXN) return CBXO_EVALUE;                                                      \
+    long size = 1;                                                                                      \
+    for (int d = 0; d < N; d++) size *= (long)shape[d];                                                 \
+    long *off = NULL, *pix = NULL, *own = NULL;                                                         \
+    int rc = region_lists(labels, size, index, n_index, &off, &pix, &own);                             \
+    if (rc) { free(off); free(pix); free(own); return rc; }                                             \
+    for (long j = 0; j < n_index; j++) {                                                                \
+        const long s = own[j];                                                                          \
+        long n = 0, k = 0, p[MOM_MAXN];                                                                 \
+        for (long q = off[s]; q < off[s + 1]; q++) {                                                    \
+            point_of(pix[q], N, shape, p);                                                              \
+            if (DIST(lines + 2 * N * j, p, N) < xtol) {                                                 \
+                n++;                                                                                    \
+                if (data[pix[q]] >= vmin) k++;                                                          \
+            }                                                                                           \
+        }                                                                                               \
+        out[j] = LOGBINOM(n, k, p0);                                                                    \
+        if (counts) { counts[2 * j] = n; counts[2 * j + 1] = k; }                                       \
+    }                                                                                                   \
+    free(off); free(pix); free(own);                                                                    \
+    return CBXO_OK;
+
+int cbxo_p_values_f32(const int32_t *labels, const float *data, int ndim, const int64_t *shape, const int64_t *index,
+                      long n_index, const float *lines, float p0, float vmin, float xtol, float *out, long *counts)
+{
+    PVALUES_BODY(float, segdist_f32, logbinom_f32)
+}
+
+int cbxo_p_values_f64(const int32_t *labels, const double *data, int ndim, const int64_t *shape,
+                      const int64_t *index, long n_index, const double *lines, double p0, double vmin, double xtol,
+                      double *out, long *counts)
+{
+    PVALUES_BODY(double, segdist_f64, logbinom_f64)
+}

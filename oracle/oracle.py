@@ -64,6 +64,14 @@ def lib() -> ctypes.CDLL:
         _lib.cbxo_draw_lines_omp_f64.argtypes = [P, L, L, L, P, L, P, D, I, I, I]
         _lib.cbxo_label_omp.argtypes = [P, I, I, P, P, L, L, P, I]
         _lib.cbxo_label_omp.restype = L
+        _lib.cbxo_region_moments_f32.argtypes = [P, P, I, P, P, L, I, P]
+        _lib.cbxo_region_moments_f64.argtypes = [P, P, I, P, P, L, I, P]
+        _lib.cbxo_p_values_f32.argtypes = [P, P, I, P, P, L, P, F, F, F, P, P]
+        _lib.cbxo_p_values_f64.argtypes = [P, P, I, P, P, L, P, D, D, D, P, P]
+        _lib.cbxo_logbinom_f64.argtypes = [L, L, D]
+        _lib.cbxo_logbinom_f64.restype = D
+        _lib.cbxo_logbinom_f32.argtypes = [L, L, F]
+        _lib.cbxo_logbinom_f32.restype = F
         _lib.cbxo_accumulate_lines_f32.argtypes = [P, L, L, L, P, L, P, P, L, F, I, I, I]
         _lib.cbxo_accumulate_lines_f64.argtypes = [P, L, L, L, P, L, P, P, L, D, I, I, I]
         _lib.cbxo_plant.argtypes = [I]
@@ -392,6 +400,59 @@ def label_omp(inp: np.ndarray, shifts: np.ndarray, npts: int = 1, threads: int =
     if n < 0:
         _raise(-n, "oracle label_omp failed")
     return out, int(n)
+
+
+MOMENTS = {'total_mass': 0, 'mean': 1, 'center_of_mass': 2, 'moment_of_inertia': 3, 'covariance_matrix': 4,
+           'line_fit': 5}
+
+
+def region_moments(what: str, labels: np.ndarray, index, data: np.ndarray) -> np.ndarray:
+    """The reference's per-region functions (label.cpp:534-583) on the regions
+    of a label map: region j = the pixels holding index[j], ascending.
+    Arithmetic in data's type, in the reference's order (MomentsND,
+    label.hpp:554-722).  Shapes: total_mass (n,), mean / center_of_mass
+    (n, ndim), moment_of_inertia / covariance_matrix (n, ndim, ndim),
+    line_fit (n, 2 ndim)."""
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    d = np.ascontiguousarray(data)
+    if d.dtype not in (np.float32, np.float64):
+        d = d.astype(np.float64)
+    if lab.shape != d.shape:
+        raise ValueError("labels and data shapes differ")
+    idx = np.ascontiguousarray(index, dtype=np.int64).ravel()
+    N = lab.ndim
+    w = MOMENTS[what]
+    M = {0: 1, 1: N, 2: N, 3: N * N, 4: N * N, 5: 2 * N}[w]
+    out = np.zeros((idx.size, M), d.dtype)
+    shape = np.asarray(lab.shape, np.int64)
+    fn = lib().cbxo_region_moments_f64 if d.dtype == np.float64 else lib().cbxo_region_moments_f32
+    _raise(fn(_ptr(lab), _ptr(d), N, _ptr(shape), _ptr(idx), idx.size, w, _ptr(out)), "oracle moments failed")
+    if w == 0:
+        return out.reshape(-1)
+    if w in (3, 4):
+        return out.reshape(-1, N, N)
+    return out
+
+
+def p_values(labels: np.ndarray, index, lines: np.ndarray, data: np.ndarray, p0: float, vmin: float,
+             xtol: float, counts: bool = False):
+    """label.p_values (label.cpp:228-255): log binomial tail of the pixels of
+    each region within xtol of its line that reach vmin, in data's type;
+    ``counts=True`` also returns the (n, k) pairs."""
+    lab = np.ascontiguousarray(labels, dtype=np.int32)
+    d = np.ascontiguousarray(data)
+    idx = np.ascontiguousarray(index, dtype=np.int64).ravel()
+    N = lab.ndim
+    ln = np.ascontiguousarray(lines, dtype=d.dtype).reshape(-1, 2 * N)
+    if ln.shape[0] < idx.size:
+        raise ValueError("lines array must have shape (n_lines, data.ndim() * 2)")
+    out = np.zeros(idx.size, d.dtype)
+    cnt = np.zeros((idx.size, 2), np.int64)
+    shape = np.asarray(lab.shape, np.int64)
+    fn = lib().cbxo_p_values_f64 if d.dtype == np.float64 else lib().cbxo_p_values_f32
+    _raise(fn(_ptr(lab), _ptr(d), N, _ptr(shape), _ptr(idx), idx.size, _ptr(ln), float(p0), float(vmin),
+              float(xtol), _ptr(out), _ptr(cnt)), "oracle p_values failed")
+    return (out, cnt) if counts else out
 
 
 class planted:

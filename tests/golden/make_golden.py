@@ -12,7 +12,8 @@ so sums are in ascending line order).  Run from the repository root:
 
     make -C oracle ref && python tests/golden/make_golden.py [draw label config1 ...]
 
-(no names: every fixture file).  ``config1.npz`` holds the reference's render
+(no names: every fixture file).  ``moments_cases.npz`` holds the reference's
+region moments and p-values of labelled random masks.  ``config1.npz`` holds the reference's render
 of the 512x512 / 1000-segment workload (seed 0): the rectangular image whole,
 the triangular image as a SHA-256 of its bytes plus rows 200-263, the SHA-256
 of its raw pixel list (line, y, x as int32), and a LabelResult case.
@@ -92,14 +93,48 @@ def config1_fixtures(bres, lab):
     print("wrote config1.npz")
 
 
+MOMENT_FUNCS = ('total_mass', 'mean', 'center_of_mass', 'moment_of_inertia', 'covariance_matrix', 'line_fit')
+
+
+def moments_fixtures(lab):
+    """The reference's per-region functions and p_values (label.cpp:176-255,
+    534-583) on labelled random masks: 2D f32/f64, a per-frame 3D stack and a
+    4D case; each case stores the label map, the data, the outputs and the
+    p_values arguments."""
+    rng = np.random.default_rng(99)
+    fx = {}
+    cases = [((40, 60), [1, 1], 2, np.float32), ((40, 60), [1, 1], 1, np.float64),
+             ((3, 30, 35), [0, 1, 1], 2, np.float64), ((3, 30, 35), [0, 1, 1], 2, np.float32),
+             ((3, 4, 6, 7), [1, 1, 1, 1], 1, np.float64)]
+    for c, (shape, radii, conn, dt) in enumerate(cases):
+        mask = rng.random(shape) > 0.5
+        data = (rng.random(shape) * 4.0 - 1.0).astype(dt)
+        res = lab.label(mask, lab.Structure(radii, conn), 1, 1)
+        n = len(res.regions)
+        fx[f'c{c}_labels'] = res.to_array(np.arange(1, n + 1)).astype(np.int32)
+        fx[f'c{c}_data'] = data
+        for name in MOMENT_FUNCS:
+            fx[f'c{c}_{name}'] = np.asarray(getattr(lab, name)(res, data))
+        lines = np.asarray(lab.line_fit(res, data)).astype(dt)
+        fx[f'c{c}_lines'] = lines
+        fx[f'c{c}_pargs'] = np.array([0.3, 1.0, 1.5], np.float64)      # p0, vmin, xtol
+        # Python floats, as functions.py passes them: pybind11 then picks the overload of the
+        # data's type (NumPy float32 scalars would fail its no-convert pass and select double)
+        fx[f'c{c}_p_values'] = np.asarray(lab.p_values(res, lines, data, 0.3, 1.0, 1.5))
+    np.savez_compressed(os.path.join(HERE, 'moments_cases.npz'), **fx)
+    print(f"wrote moments_cases.npz ({len(cases)} cases)")
+
+
 def main():
     bres = O.ref_module('bresenham')
     lab = O.ref_module('label')
     if bres is None or lab is None:
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources exist")
-    names = sys.argv[1:] or ['draw', 'label', 'config1']
+    names = sys.argv[1:] or ['draw', 'label', 'config1', 'moments']
     if 'config1' in names:
         config1_fixtures(bres, lab)
+    if 'moments' in names:
+        moments_fixtures(lab)
     if 'draw' not in names and 'label' not in names:
         return
     rng = np.random.default_rng(20240601)

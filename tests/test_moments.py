@@ -1,0 +1,243 @@
+"""Region moments and p-values of label maps (the reference's
+center_of_mass, covariance_matrix, line_fit, p_values and the per-region
+functions of its CPU label module).
+
+CPU: the oracle (oracle.region_moments / oracle.p_values, a restatement of
+MomentsND and LineND::distance in the data's type) against the golden
+fixtures written by the compiled reference (bit for bit), against the
+compiled module on fresh inputs where it is built, and, independently of the
+reference's code, the centre of mass against scipy.ndimage.center_of_mass
+and the covariance against numpy's weighted covariance.
+
+GPU: cbx_region_moments / cbx_p_values against the oracle.  The GPU sums in
+double: f64 data within 1e-10 relative of the oracle (atol 1e-10 of each
+output's scale: cancelled off-diagonals); f32 data within 2e-6 relative of
+the FP64 oracle (the f32 output rounding; atol 2e-6 of the scale) and, for
+information, its distance to the reference's own float arithmetic printed.
+p-value counts exact; p-values within 1e-5 relative (f32, device log and
+lgamma) or 1e-12 (f64).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from cbclib_v2_amd.label import Structure
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), 'golden')
+FUNCS = ('total_mass', 'mean', 'center_of_mass', 'moment_of_inertia', 'covariance_matrix', 'line_fit')
+
+
+def golden_cases():
+    with np.load(os.path.join(GOLD, 'moments_cases.npz'), allow_pickle=False) as z:
+        data = {k: z[k] for k in z.files}
+    n = len({k.split('_')[0] for k in data})
+    return [{k.split('_', 1)[1]: v for k, v in data.items() if k.split('_')[0] == f'c{i}'} for i in range(n)]
+
+
+CASES = golden_cases()
+
+
+# --------------------------------------------------------------------- CPU
+
+@pytest.mark.parametrize('c', range(len(CASES)))
+def test_oracle_matches_golden(oracle, c):
+    case = CASES[c]
+    lab, data = case['labels'], case['data']
+    index = np.arange(1, int(lab.max()) + 1)
+    for name in FUNCS:
+        got = oracle.region_moments(name, lab, index, data)
+        exp = case[name].reshape(got.shape)
+        assert got.dtype == exp.dtype
+        assert np.array_equal(got.view(np.uint8), exp.view(np.uint8)), (c, name)
+    p0, vmin, xtol = case['pargs']
+    dt = data.dtype.type
+    got = oracle.p_values(lab, index, case['lines'], data, dt(p0), dt(vmin), dt(xtol))
+    assert np.array_equal(got, case['p_values']), c
+
+
+def random_labels(seed, shape, radii, conn, dtype, density=0.5):
+    from oracle import oracle as O
+    rng = np.random.default_rng(seed)
+    mask = rng.random(shape) < density
+    lab, n = O.label(mask, Structure(radii, conn).shifts(), 1)
+    data = (rng.random(shape) * 4.0 - 1.0).astype(dtype)
+    return lab, n, data
+
+
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_oracle_matches_compiled_reference(oracle, ref_label, dtype):
+    for seed, (shape, radii, conn) in enumerate([((50, 70), [1, 1], 2), ((4, 25, 30), [0, 1, 1], 1),
+                                                 ((5, 9, 12), [1, 1, 1], 3)]):
+        rng = np.random.default_rng(seed)
+        mask = rng.random(shape) > 0.5
+        data = (rng.random(shape) * 4.0 - 1.0).astype(dtype)
+        res = ref_label.label(mask, ref_label.Structure(radii, conn), 1, 1)
+        n = len(res.regions)
+        lab = res.to_array(np.arange(1, n + 1)).astype(np.int32)
+        for name in FUNCS:
+            got = oracle.region_moments(name, lab, np.arange(1, n + 1), data)
+            exp = np.asarray(getattr(ref_label, name)(res, data)).reshape(got.shape)
+            assert np.array_equal(got, exp), (seed, name)
+
+
+def test_center_of_mass_and_covariance_closed_form(oracle):
+    """Independent of the reference's code: scipy's centre of mass and numpy's
+    weighted covariance of the non-negative values, per region."""
+    import scipy.ndimage as ndi
+    lab, n, data = random_labels(5, (60, 80), [1, 1], 2, np.float64)
+    idx = np.arange(1, n + 1)
+    w = np.maximum(data, 0.0)
+    com = oracle.region_moments('center_of_mass', lab, idx, data)
+    cov = oracle.region_moments('covariance_matrix', lab, idx, data)
+    ref = np.array(ndi.center_of_mass(w, lab, idx))
+    ok = np.array([w[lab == i].sum() > 0 for i in idx])
+    np.testing.assert_allclose(com[ok], ref[ok], rtol=1e-12, atol=1e-12)
+    yy, xx = np.indices(lab.shape)
+    for i in np.flatnonzero(ok)[:40]:
+        sel = lab == i + 1
+        pts = np.stack([yy[sel], xx[sel]]).astype(np.float64)
+        c = np.cov(pts, aweights=w[sel], bias=True)
+        np.testing.assert_allclose(cov[i], c, rtol=1e-9, atol=1e-9)
+
+
+def test_oracle_index_variants(oracle):
+    """Entries of index are independent: a permuted subset with duplicates
+    and an absent id gives, per entry, the plain computation's row (zeros for
+    the absent id)."""
+    lab, n, data = random_labels(11, (80, 100), [1, 1], 1, np.float64)
+    rng = np.random.default_rng(3)
+    index = np.concatenate([rng.permutation(np.arange(1, n + 1))[: n // 2], [3, 3, n + 5, 2]]).astype(np.int64)
+    for name in FUNCS:
+        full = oracle.region_moments(name, lab, np.arange(1, n + 1), data)
+        got = oracle.region_moments(name, lab, index, data)
+        for j, l in enumerate(index):
+            exp = full[l - 1] if l <= n else np.zeros_like(full[0])
+            assert np.array_equal(got[j], exp), (name, j, l)
+
+
+def test_oracle_moment_pins_catch_mistakes(oracle):
+    """Pins above fail for plausible mistakes: the transposed (xyz) matrix
+    order, or values not clamped at zero."""
+    case = CASES[2]                     # 3D, f64
+    lab, data = case['labels'], case['data']
+    idx = np.arange(1, int(lab.max()) + 1)
+    cov = oracle.region_moments('covariance_matrix', lab, idx, data)
+    assert not np.array_equal(cov[:, ::-1, ::-1], case['covariance_matrix'])
+    com = oracle.region_moments('center_of_mass', lab, idx, np.abs(data))
+    assert not np.array_equal(com, case['center_of_mass'])
+
+
+# --------------------------------------------------------------------- GPU
+
+def gpu_moments(name, lab, index, data):
+    import torch
+    from cbclib_v2_amd import label as cl
+    res = cl.CPLabelResult(labels=torch.from_numpy(lab).cuda(), index=torch.from_numpy(index).cuda())
+    out = getattr(cl, name)(res, torch.from_numpy(data).cuda())
+    return out.cpu().numpy()
+
+
+def check_moments(oracle, lab, index, data, what):
+    f64 = data.dtype == np.float64
+    worst_ref = 0.0
+    for name in FUNCS:
+        got = gpu_moments(name, lab, index, data)
+        exp = oracle.region_moments(name, lab, index, data.astype(np.float64))
+        assert got.shape == exp.shape and got.dtype == data.dtype, (what, name)
+        scale = np.abs(exp).reshape(len(index), -1).max(axis=0) if exp.size else np.zeros(1)
+        tol = 1e-10 if f64 else 2e-6
+        err = np.abs(got.astype(np.float64) - exp).reshape(len(index), -1)
+        bad = err > tol * np.abs(exp).reshape(len(index), -1) + tol * scale
+        bad |= np.isnan(got).reshape(len(index), -1) != np.isnan(exp).reshape(len(index), -1)
+        assert not bad.any(), (what, name, float(np.nanmax(err)), int(bad.sum()))
+        if not f64:
+            # distance to the reference's own float arithmetic (information)
+            r32 = oracle.region_moments(name, lab, index, data).astype(np.float64)
+            d = np.abs(got - r32).reshape(len(index), -1) / np.maximum(np.abs(r32).reshape(len(index), -1), scale)
+            worst_ref = max(worst_ref, float(d.max()) if d.size else 0.0)
+    return worst_ref
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+@pytest.mark.parametrize('shape,radii,conn,density', [((90, 130), [1, 1], 2, 0.5), ((5, 70, 90), [0, 1, 1], 2, 0.5),
+                                                      ((6, 14, 15), [1, 1, 1], 3, 0.12),
+                                                      ((3, 5, 6, 9), [1, 1, 1, 1], 1, 0.2),
+                                                      ((2, 3, 3, 4, 5), [1, 1, 1, 1, 1], 1, 0.15),
+                                                      ((2, 2, 3, 3, 4, 5, 6), [1] * 7, 1, 0.1)])
+def test_moments_match_oracle(cuda, oracle, dtype, shape, radii, conn, density):
+    lab, n, data = random_labels(sum(shape), shape, radii, conn, dtype, density)
+    print(f"moments {shape} {np.dtype(dtype).name}: {n} regions", flush=True)
+    w = check_moments(oracle, lab, np.arange(1, n + 1), data, shape)
+    if dtype == np.float32:
+        print(f"  f32: max distance to the reference's float arithmetic {w:.2e} (relative to each output's scale)",
+              flush=True)
+
+
+@pytest.mark.gpu
+def test_moments_index_variants(cuda, oracle):
+    """A subset of the labels, in another order, with duplicates, absent ids and
+    a zero-mass region (its first pixel as centre, zero covariance)."""
+    lab, n, data = random_labels(11, (80, 100), [1, 1], 1, np.float64)
+    data[lab == 3] = -1.0
+    rng = np.random.default_rng(3)
+    index = np.concatenate([rng.permutation(np.arange(1, n + 1))[: n // 2], [3, 3, n + 5, 2]]).astype(np.int64)
+    check_moments(oracle, lab, index, data, 'index variants')
+    com = gpu_moments('center_of_mass', lab, index, data)
+    first = np.argwhere(lab == 3)[0]
+    assert np.array_equal(com[index == 3][0], first.astype(np.float64))
+    assert not gpu_moments('covariance_matrix', lab, index, data)[index == 3].any()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+@pytest.mark.parametrize('shape,radii', [((120, 140), [1, 1]), ((4, 60, 70), [0, 1, 1]), ((5, 9, 12), [1, 1, 1])])
+def test_p_values_match_oracle(cuda, oracle, dtype, shape, radii):
+    import torch
+    from cbclib_v2_amd import label as cl
+    lab, n, data = random_labels(sum(shape) + 1, shape, radii, 2, dtype)
+    index = np.arange(1, n + 1)
+    lines = oracle.region_moments('line_fit', lab, index, data)
+    res = cl.CPLabelResult(labels=torch.from_numpy(lab).cuda(), index=torch.from_numpy(index).cuda())
+    for p0, vmin, xtol in ((0.3, 1.0, 1.5), (0.05, 0.5, 0.8)):
+        dt = np.dtype(dtype).type
+        ref, cnt = oracle.p_values(lab, index, lines, data, dt(p0), dt(vmin), dt(xtol), counts=True)
+        got, gcnt = cl.p_values(res, torch.from_numpy(lines).cuda(), torch.from_numpy(data).cuda(), p0, vmin, xtol,
+                                counts=True)
+        assert np.array_equal(gcnt.cpu().numpy(), cnt), (shape, p0)
+        np.testing.assert_allclose(got.cpu().numpy(), ref, rtol=1e-5 if dtype == np.float32 else 1e-12, atol=0)
+
+
+@pytest.mark.gpu
+def test_moments_numpy_label_result(cuda, oracle):
+    """A NumPy LabelResult (the CPU label path's result type) gives NumPy outputs."""
+    from cbclib_v2_amd import label as cl
+    rng = np.random.default_rng(21)
+    mask = rng.random((70, 90)) > 0.5
+    data = rng.random((70, 90))
+    res = cl.label(mask, cl.Structure([1, 1], 2))
+    lab = cl.labels(res).astype(np.int32)
+    idx = np.arange(1, len(res) + 1)
+    for name in ('center_of_mass', 'covariance_matrix', 'line_fit'):
+        got = getattr(cl, name)(res, data)
+        assert isinstance(got, np.ndarray)
+        np.testing.assert_allclose(got, oracle.region_moments(name, lab, idx, data), rtol=1e-10, atol=1e-10)
+    ell = cl.ellipse_fit(res, data)
+    assert ell.shape == (len(res), 3)
+
+
+@pytest.mark.gpu
+def test_moments_errors(cuda):
+    import torch
+    from cbclib_v2_amd import label as cl
+    res = cl.CPLabelResult(labels=torch.ones((4, 5), dtype=torch.int32, device='cuda'),
+                           index=torch.arange(1, 2, device='cuda'))
+    with pytest.raises(ValueError):
+        cl.center_of_mass(res, torch.ones((4, 5, 1), device='cuda'))
+    res1 = cl.CPLabelResult(labels=torch.ones(6, dtype=torch.int32, device='cuda'),
+                            index=torch.arange(1, 2, device='cuda'))
+    with pytest.raises(ValueError):
+        cl.center_of_mass(res1, torch.ones(6, device='cuda'))          # 1D: unsupported, as in the reference
+    with pytest.raises(ValueError):
+        cl.p_values(res, torch.zeros((1, 3), device='cuda'), torch.ones((4, 5), device='cuda'), 0.5, 0.5, 1.0)
