@@ -26,7 +26,8 @@ EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32
            'cbx_timing_reset', 'cbx_timing_read', 'cbx_workspace_release', 'cbx_workspace_bytes',
            'cbx_label_force_path', 'cbx_plan_create_f32', 'cbx_plan_draw_f32', 'cbx_plan_check', 'cbx_plan_bytes',
            'cbx_plan_destroy', 'cbx_label_noalloc', 'cbx_label_noalloc_capacity', 'cbx_region_moments_f32',
-           'cbx_region_moments_f64', 'cbx_p_values_f32', 'cbx_p_values_f64')
+           'cbx_region_moments_f64', 'cbx_p_values_f32', 'cbx_p_values_f64', 'cbx_draw_lines3_f32',
+           'cbx_draw_lines3_f64', 'cbx_accumulate_lines3_f32', 'cbx_accumulate_lines3_f64')
 MOMENTS = {'total_mass': 0, 'mean': 1, 'center_of_mass': 2, 'moment_of_inertia': 3, 'covariance_matrix': 4,
            'line_fit': 5}
 
@@ -72,6 +73,12 @@ def _declare(L: ctypes.CDLL) -> None:
         fn = getattr(L, 'cbx_draw_lines_adjoint_' + name)
         fn.restype = I32
         fn.argtypes = [P, I64, I64, I64, P, I64, P, R, I32, P, OP, P]
+        fn = getattr(L, 'cbx_draw_lines3_' + name)
+        fn.restype = I32
+        fn.argtypes = [P, I64, I64, I64, I64, P, I64, P, R, I32, I32, P]
+        fn = getattr(L, 'cbx_accumulate_lines3_' + name)
+        fn.restype = I32
+        fn.argtypes = [P, I64, I64, I64, I64, P, I64, P, P, I64, R, I32, I32, I32, P]
     L.cbx_label.restype = I32
     L.cbx_label.argtypes = [P, I32, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
     for name, R in (('f32', F), ('f64', D)):

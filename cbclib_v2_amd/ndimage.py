@@ -5,7 +5,9 @@ pixel-list mode :func:`draw_pixels`.
 Same arguments, defaults, in-place semantics and exceptions as the reference:
 
 * ``out`` (..., H, W) float32 or float64 is accumulated in place and returned;
-* ``lines`` (..., 5) rows ``[x0, y0, x1, y1, width]`` (x = column);
+* ``lines`` (..., 5) rows ``[x0, y0, x1, y1, width]`` (x = column), or
+  (..., 7) rows ``[x0, y0, z0, x1, y1, z1, width]`` of 3D lines drawn into
+  volumes (..., D, H, W) (draw_line_3d, bresenham.hpp:389-430);
 * ``idxs`` frame of each line in the flattened leading dims of ``out``, or
   None (then there must be one frame or one line per frame,
   bresenham.cpp:5-20); wrong size -> ValueError, out of range -> IndexError
@@ -35,15 +37,29 @@ from ._arrays import NUMPY, dtype_of, inout, namespace, ptr, result, shape_of, s
 __all__ = ['accumulate_lines', 'draw_lines', 'draw_pixels', 'PixelList']
 
 
-def _line_count(lines: Any) -> int:
+def _line_rows(lines: Any) -> tuple:
+    """(number of lines, spatial dimensions): rows of 5 are 2D lines, rows of 7
+    3D lines (draw_lines_2d_3d, bresenham.cpp:126-139)."""
     shape = shape_of(lines)
     L = shape[-1] if len(shape) else 0
-    if L == 7:
-        raise NotImplementedError("3D lines (rows of 7) are not supported by cbclib_v2_amd")
-    if L != 5:
+    if L not in (5, 7):
         # bresenham.cpp:137 / cuda_draw_lines.cu:936
         raise RuntimeError(f"Invalid lines size ({L}) at axis {max(len(shape) - 1, 0)}")
-    return prod(shape[:-1])
+    return prod(shape[:-1]), (L - 1) // 2
+
+
+def _line_count(lines: Any) -> int:
+    n_lines, ndim = _line_rows(lines)
+    if ndim == 3:
+        raise NotImplementedError("pixel lists of 3D lines (rows of 7) are not provided; use draw_lines")
+    return n_lines
+
+
+def _frames_of(oshape: tuple, ndim: int) -> tuple:
+    """(n_frames, frame shape) of an output whose last ``ndim`` axes are a frame."""
+    if len(oshape) < ndim:
+        raise ValueError(f"Output array has insufficient number of dimensions ({len(oshape)} < {ndim})")
+    return prod(oshape[:-ndim]), tuple(oshape[-ndim:])
 
 
 def _check_idxs(idxs: Any, n_lines: int, n_frames: int) -> None:
@@ -82,10 +98,8 @@ def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, ker
     odt = dtype_of(out)
     if odt not in (np.float32, np.float64):
         raise TypeError(f"out must be float32 or float64, got {odt}")
-    n_lines = _line_count(lines)
-    if len(oshape) < 2:
-        raise ValueError(f"Output array has insufficient number of dimensions ({len(oshape)} < 2)")
-    n_frames, H, W = prod(oshape[:-2]), oshape[-2], oshape[-1]
+    n_lines, ndim = _line_rows(lines)
+    n_frames, fshape = _frames_of(oshape, ndim)
     _check_idxs(idxs, n_lines, n_frames)
     f64 = odt == np.float64
     t_out, finish = inout(out, torch.float64 if f64 else torch.float32)
@@ -94,6 +108,14 @@ def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, ker
     t_lines = to_device(lines, torch.float64 if f64 else torch.float32)
     t_idxs = None if idxs is None else to_device(idxs, torch.int64 if f64 else torch.int32)
     L = _native.lib()
+    if ndim == 3:
+        if kernels is not None:
+            raise ValueError("kernels (one profile per line) is available for 2D lines only")
+        fn = L.cbx_draw_lines3_f64 if f64 else L.cbx_draw_lines3_f32
+        _native.check(fn(t_out.data_ptr(), n_frames, *fshape, t_lines.data_ptr(), n_lines, ptr(t_idxs),
+                         float(max_val), kid, oid, stream_ptr()))
+        return finish()
+    H, W = fshape
     fn = L.cbx_draw_lines_f64 if f64 else L.cbx_draw_lines_f32
     t_kern = None
     if kernels is not None:
@@ -131,10 +153,8 @@ def accumulate_lines(out: Any, lines: Any, terms: Any, frames: Any, max_val: flo
     odt = dtype_of(out)
     if odt not in (np.float32, np.float64):
         raise TypeError(f"out must be float32 or float64, got {odt}")
-    n_lines = _line_count(lines)
-    if len(oshape) < 2:
-        raise ValueError(f"Output array has insufficient number of dimensions ({len(oshape)} < 2)")
-    n_frames, H, W = prod(oshape[:-2]), oshape[-2], oshape[-1]
+    n_lines, ndim = _line_rows(lines)
+    n_frames, fshape = _frames_of(oshape, ndim)
     n_terms = prod(shape_of(frames))
     if prod(shape_of(terms)) != n_lines:
         raise ValueError(f"Term indices ({prod(shape_of(terms))}) is incompatible with number of lines "
@@ -146,9 +166,14 @@ def accumulate_lines(out: Any, lines: Any, terms: Any, frames: Any, max_val: flo
     t_terms = to_device(terms, it).reshape(-1)
     t_frames = to_device(frames, it).reshape(-1)
     L = _native.lib()
+    if ndim == 3:
+        fn = L.cbx_accumulate_lines3_f64 if f64 else L.cbx_accumulate_lines3_f32
+        _native.check(fn(t_out.data_ptr(), n_frames, *fshape, t_lines.data_ptr(), n_lines, ptr(t_terms),
+                         ptr(t_frames), n_terms, float(max_val), kid, iid, oid, stream_ptr()))
+        return finish()
     fn = L.cbx_accumulate_lines_f64 if f64 else L.cbx_accumulate_lines_f32
     opts = _opts(tile, seg, False, threads, False, chunk)
-    _native.check(fn(t_out.data_ptr(), n_frames, H, W, t_lines.data_ptr(), n_lines, ptr(t_terms), ptr(t_frames),
+    _native.check(fn(t_out.data_ptr(), n_frames, *fshape, t_lines.data_ptr(), n_lines, ptr(t_terms), ptr(t_frames),
                      n_terms, float(max_val), kid, iid, oid, ctypes.byref(opts), stream_ptr()))
     return finish()
 

@@ -162,6 +162,39 @@ CBX_API int cbx_draw_lines_adjoint_f64(const double *grad_out, int64_t n_frames,
                                int kernel, double *grad_lines, const cbx_draw_opts *opts, void *stream);
 
 /*
+ * 3D lines: draw_lines on volumes.  out: n_frames x depth x height x width
+ * (z, y, x); lines: n_lines x 7 rows [x0, y0, z0, x1, y1, z1, width].  Same
+ * idxs rules, error codes and overlap values as cbx_draw_lines_*; the pixels
+ * and values are draw_line_3d's (bresenham.hpp:389-430).  CBX_MAX is exact;
+ * CBX_SUM adds with atomics (no fixed order).  A line whose walk would visit
+ * more than 2^30 pixels -> CBX_EVALUE.  Replaces bresenham.draw_lines on
+ * rows of 7 (draw_lines_2d_3d, bresenham.cpp:126-149, the L = 7 case).
+ */
+CBX_API int cbx_draw_lines3_f32(float *out, int64_t n_frames, int64_t depth, int64_t height, int64_t width,
+                                const float *lines, int64_t n_lines, const int32_t *idxs, float max_val, int kernel,
+                                int overlap, void *stream);
+CBX_API int cbx_draw_lines3_f64(double *out, int64_t n_frames, int64_t depth, int64_t height, int64_t width,
+                                const double *lines, int64_t n_lines, const int64_t *idxs, double max_val, int kernel,
+                                int overlap, void *stream);
+
+/*
+ * 3D lines: accumulate_lines on volumes (arguments as cbx_accumulate_lines_*
+ * with depth and rows of 7).  Equal overlaps are cbx_draw_lines3 with
+ * idxs = frames[terms]; mixed overlaps draw the terms into zeroed term
+ * volumes (batches of at most 256 MiB) and combine every touched voxel into
+ * its frame.  Replaces bresenham.accumulate_lines on rows of 7
+ * (accumulate_lines_2d_3d, bresenham.cpp:284-309, the L = 7 case).
+ */
+CBX_API int cbx_accumulate_lines3_f32(float *out, int64_t n_frames, int64_t depth, int64_t height, int64_t width,
+                                      const float *lines, int64_t n_lines, const int32_t *terms,
+                                      const int32_t *frames, int64_t n_terms, float max_val, int kernel,
+                                      int in_overlap, int out_overlap, void *stream);
+CBX_API int cbx_accumulate_lines3_f64(double *out, int64_t n_frames, int64_t depth, int64_t height, int64_t width,
+                                      const double *lines, int64_t n_lines, const int64_t *terms,
+                                      const int64_t *frames, int64_t n_terms, double max_val, int kernel,
+                                      int in_overlap, int out_overlap, void *stream);
+
+/*
  * label: connected components of equal-valued non-zero elements of `inp`
  * (dtype CBX_BOOL / CBX_INT32 / CBX_INT64, ndim dims of `shape`) under the
  * neighbour `shifts` (n_shifts x ndim, HOST array, the structure's non-zero

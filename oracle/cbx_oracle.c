@@ -871,6 +871,7 @@ long cbxo_label_omp(const void *inp, int dtype, int ndim, const int64_t *shape, 
 typedef struct {
     void *buf; long H, W; int kernel, in_ov; double max_val;
     long *idx; long n, cap; int fail;
+    long D;   /* depth of 3D frames (1 for 2D) */
 } acc_ctx;
 
 static void acc_push(acc_ctx *c, long i)
@@ -928,20 +929,24 @@ static int term_order(const void *terms, int wide, long n_lines, long n_terms, l
     return CBXO_OK;
 }
 
-#define ACCUMULATE_BODY(REAL, WALK, PX, FR_T)                                                       \
-    if (kernel < 0 || kernel > 4 || in_ov < 0 || in_ov > 1 || out_ov < 0 || out_ov > 1)            \
+#define ACCUMULATE_BODY(REAL, WALK, PX, FR_T) ACCUMULATE_ND(REAL, WALK, PX, FR_T, 5, 1)
+
+/* ROW: values per line (5 or 7); DEPTH: frame depth (1 for 2D frames) */
+#define ACCUMULATE_ND(REAL, WALK, PX, FR_T, ROW, DEPTH)                                             \
+    if (kernel < 0 || kernel > 4 || in_ov < 0 || in_ov > 1 || out_ov < 0 || out_ov > 1)             \
         return CBXO_EVALUE;                                                                         \
     for (long t = 0; t < n_terms; t++)                                                              \
         if (frames[t] < 0 || frames[t] >= n_frames) return CBXO_EINDEX;                             \
     long *order = NULL, *toff = NULL;                                                               \
     int rc = term_order(terms, sizeof(FR_T) == 8, n_lines, n_terms, &order, &toff);                 \
-    REAL *buf = (rc == CBXO_OK) ? (REAL *)calloc((size_t)(H * W ? H * W : 1), sizeof(REAL)) : NULL; \
+    const long fsz = (long)(DEPTH) * H * W;                                                         \
+    REAL *buf = (rc == CBXO_OK) ? (REAL *)calloc((size_t)(fsz ? fsz : 1), sizeof(REAL)) : NULL;     \
     if (rc == CBXO_OK && !buf) rc = CBXO_ERUNTIME;                                                  \
-    acc_ctx c = {buf, H, W, kernel, in_ov, max_val, NULL, 0, 0, 0};                                \
+    acc_ctx c = {buf, H, W, kernel, in_ov, max_val, NULL, 0, 0, 0, (long)(DEPTH)};                  \
     for (long t = 0; rc == CBXO_OK && t < n_terms; t++) {                                           \
-        REAL *dst = out + (long)frames[t] * H * W;                                                  \
+        REAL *dst = out + (long)frames[t] * fsz;                                                    \
         for (long j = toff[t]; j < toff[t + 1] && rc == CBXO_OK; j++)                               \
-            rc = WALK(lines + 5 * order[j], PX, &c);                                                \
+            rc = WALK(lines + (ROW) * order[j], PX, &c);                                            \
         if (c.fail) rc = CBXO_ERUNTIME;                                                             \
         for (long k = 0; rc == CBXO_OK && k < c.n; k++) {                                           \
             const long i = c.idx[k];                                                                \
@@ -1210,4 +1215,228 @@ int cbxo_p_values_f64(const int32_t *labels, const double *data, int ndim, const
                       double *out, long *counts)
 {
     PVALUES_BODY(double, segdist_f64, logbinom_f64)
+}
+
+/* ------------------------------------------------------------------------ */
+/* 3D lines (rows of 7: x0 y0 z0 x1 y1 z1 w): draw_line_3d                  */
+/* (bresenham.hpp:389-430) on BresenhamPlotter<T, 3> and LineIterator<T, 3>  */
+/* (118-348): a tangent error and one normal error per axis pair            */
+/* ((0,1), (0,2), (1,2): UniquePairs, geometry.hpp:404-437), the 3-axis      */
+/* Bresenham step, and at every long-axis step the orthogonal plane filled  */
+/* in four quadrants by a 2D walk with single-axis rays.  Same arithmetic   */
+/* contract as the 2D walk (T increments, pow squares in double).           */
+/* Pinned by: the compiled reference bresenham module on 3D inputs (golden  */
+/* fixtures tests/golden/draw3_cases.npz and random inputs).                 */
+/* ------------------------------------------------------------------------ */
+
+#define W3_BODY(REAL, ROUND, CEIL, FABS, SQRT)                                                          \
+    typedef struct { long step[3], cur[3]; int next[3]; REAL te, ne[3]; } it3;                          \
+    static const int PA[3] = {0, 0, 1}, PB[3] = {1, 2, 2};                                               \
+    REAL p0[3] = {ln[0], ln[1], ln[2]}, p1[3] = {ln[3], ln[4], ln[5]}, w = ln[6];                       \
+    REAL t[3], nrm[3][3];                                                                               \
+    for (int i = 0; i < 3; i++) t[i] = p1[i] - p0[i];                                                   \
+    for (int q = 0; q < 3; q++) {                                                                       \
+        for (int i = 0; i < 3; i++) nrm[q][i] = (REAL)0;                                                \
+        nrm[q][PA[q]] = p1[PB[q]] - p0[PB[q]];                                                          \
+        nrm[q][PB[q]] = p0[PA[q]] - p1[PA[q]];                                                          \
+    }                                                                                                   \
+    if (!(w > (REAL)0)) return CBXO_OK;                                                                 \
+    REAL mag = (REAL)0;                                                                                 \
+    for (int i = 0; i < 3; i++) mag += t[i] * t[i];                                                     \
+    const REAL len = SQRT(mag);                                                                         \
+    if (!(len > (REAL)0) || !isfinite(len) || !isfinite(w)) return CBXO_OK;                            \
+    for (int i = 0; i < 3; i++) if (!isfinite(p0[i]) || !isfinite(p1[i])) return CBXO_OK;              \
+    int axis = 0;                                                                                       \
+    for (int i = 1; i < 3; i++) if (FABS(t[axis]) < FABS(t[i])) axis = i;   /* first max |t| */         \
+    const long offset = (long)(CEIL(w) + (REAL)1);                                                      \
+    REAL tau[3];                                                                                        \
+    for (int i = 0; i < 3; i++) tau[i] = t[i] / len;                                                    \
+    const REAL sc = FABS((REAL)offset / tau[axis]);                                                     \
+    long step[3], m0[3], m1[3];                                                                         \
+    for (int i = 0; i < 3; i++) {                                                                       \
+        step[i] = (p1[i] >= p0[i]) ? 1 : -1;                                                            \
+        REAL a0 = ROUND(p0[i] - sc * tau[i]), a1 = ROUND(p1[i] + sc * tau[i]);                          \
+        if (!(FABS(a0) < (REAL)1e9) || !(FABS(a1) < (REAL)1e9)) return CBXO_EVALUE;                     \
+        m0[i] = (long)a0;                                                                               \
+        m1[i] = (long)a1 + step[i];                                                                     \
+    }                                                                                                   \
+    if (labs(m1[axis] - m0[axis]) > CBXO_MAX_WALK) return CBXO_EVALUE;                                  \
+    const REAL w2 = w * w;                                                                              \
+    /* iterator operations */                                                                           \
+    void upd(it3 *it) {                                                                                 \
+        for (int i = 0; i < 3; i++) it->next[i] = it->step[i] != 0;                                     \
+        for (int q = 0; q < 3; q++) {                                                                   \
+            const int a = PA[q], b = PB[q];                                                             \
+            if (!(it->step[a] && it->step[b])) continue;                                                \
+            REAL ex = it->ne[q] + (REAL)it->step[a] * nrm[q][a];                                        \
+            REAL ey = it->ne[q] + (REAL)it->step[b] * nrm[q][b];                                        \
+            REAL exy = ex + (REAL)it->step[b] * nrm[q][b];                                              \
+            int f, s;                                                                                   \
+            if (FABS(ey) < FABS(ex)) { if (FABS(exy) < FABS(ey)) { f = 1; s = 1; } else { f = 0; s = 1; } } \
+            else if (FABS(exy) < FABS(ex)) { f = 1; s = 1; }                                            \
+            else { f = 1; s = 0; }                                                                      \
+            it->next[a] &= f;                                                                           \
+            it->next[b] &= s;                                                                           \
+        }                                                                                               \
+    }                                                                                                   \
+    void inc(it3 *it, long x, int ax) {                                                                 \
+        it->cur[ax] += x;                                                                               \
+        it->te += (REAL)x * t[ax];                                                                      \
+        for (int q = 0; q < 3; q++) if (PA[q] == ax || PB[q] == ax) it->ne[q] += (REAL)x * nrm[q][ax];  \
+    }                                                                                                   \
+    void adv(it3 *it) {                                                                                 \
+        for (int i = 0; i < 3; i++) if (it->next[i]) inc(it, it->step[i], i);                           \
+        upd(it);                                                                                        \
+    }                                                                                                   \
+    REAL nerr(const it3 *it) {                                                                          \
+        REAL e = (REAL)0;                                                                               \
+        for (int q = 0; q < 3; q++) { double v = (double)(it->ne[q] / len); e = (REAL)((double)e + v * v); } \
+        return e;                                                                                       \
+    }                                                                                                   \
+    REAL errf(const it3 *it) {                                                                          \
+        REAL a = -it->te, b = it->te - len * len;                                                       \
+        REAL m = (a < b) ? b : a;                                                                       \
+        REAL at = (m < (REAL)0) ? (REAL)0 : m;                                                          \
+        double qt = (double)(at / len);                                                                 \
+        double e = qt * qt + (double)nerr(it);                                                          \
+        return (REAL)(e / (double)w2);                                                                  \
+    }                                                                                                   \
+    it3 it;                                                                                             \
+    for (int i = 0; i < 3; i++) { it.step[i] = step[i]; it.cur[i] = m0[i]; }                           \
+    {                                                                                                   \
+        REAL d[3];                                                                                      \
+        for (int i = 0; i < 3; i++) d[i] = (REAL)m0[i] - p0[i];                                         \
+        it.te = (REAL)0;                                                                                \
+        for (int i = 0; i < 3; i++) it.te += t[i] * d[i];                                               \
+        for (int q = 0; q < 3; q++) {                                                                   \
+            it.ne[q] = (REAL)0;                                                                         \
+            for (int i = 0; i < 3; i++) it.ne[q] += nrm[q][i] * d[i];                                   \
+        }                                                                                               \
+    }                                                                                                   \
+    upd(&it);                                                                                           \
+    const int ax1 = (axis + 1) % 3, ax2 = (axis + 2) % 3;                                               \
+    while (!(it.cur[0] == m1[0] || it.cur[1] == m1[1] || it.cur[2] == m1[2])) {                        \
+        if (it.next[axis]) {                                                                            \
+            emit(ctx, it.cur, errf(&it));                                                               \
+            for (int i = 0; i < 4; i++) {                                                               \
+                it3 xy = it;                                                                            \
+                xy.step[axis] = 0; upd(&xy);                          /* collapse(iter, axis) */        \
+                if (i & 1) xy.step[ax1] *= -1;                                                          \
+                if (i >> 1) xy.step[ax2] *= -1;                                                         \
+                upd(&xy);                                             /* flip */                        \
+                const int mv = (1 + ((i & 1) ^ (i >> 1)) == 1) ? ax1 : ax2;                             \
+                inc(&xy, xy.step[mv], mv); upd(&xy);                  /* move */                        \
+                for (; nerr(&xy) < w2; adv(&xy)) {                                                      \
+                    emit(ctx, xy.cur, errf(&xy));                                                       \
+                    for (int j = 1; j < 3; j++) {                                                       \
+                        const int aj = (axis + j) % 3;                                                  \
+                        if (!xy.next[aj]) continue;                                                     \
+                        it3 x = xy;                                                                     \
+                        x.step[aj] = 0; upd(&x);                      /* collapse(iter_xy, axis(j)) */  \
+                        for (adv(&x); nerr(&x) < w2; adv(&x)) emit(ctx, x.cur, errf(&x));              \
+                    }                                                                                   \
+                }                                                                                       \
+            }                                                                                           \
+        }                                                                                               \
+        adv(&it);                                                                                       \
+    }                                                                                                   \
+    return CBXO_OK;
+
+typedef void (*emit3_f32_t)(void *ctx, const long *xyz, float err);
+typedef void (*emit3_f64_t)(void *ctx, const long *xyz, double err);
+
+static int walk3_f32(const float *ln, emit3_f32_t emit, void *ctx) { W3_BODY(float, roundf, ceilf, fabsf, sqrtf) }
+static int walk3_f64(const double *ln, emit3_f64_t emit, void *ctx) { W3_BODY(double, round, ceil, fabs, sqrt) }
+
+typedef struct { void *out; long D, H, W, frame; int kernel, overlap; double max_val; } draw3_ctx;
+
+static void draw3_px_f32(void *vctx, const long *p, float err)
+{
+    draw3_ctx *c = (draw3_ctx *)vctx;
+    if (!(err <= 1.0f) || p[0] < 0 || p[0] >= c->W || p[1] < 0 || p[1] >= c->H || p[2] < 0 || p[2] >= c->D) return;
+    float v = (float)c->max_val * prof_f32(c->kernel, sqrtf(err));
+    float *o = (float *)c->out + ((c->frame * c->D + p[2]) * c->H + p[1]) * c->W + p[0];
+    if (c->overlap) *o = (*o < v) ? v : *o;
+    else *o = *o + v;
+}
+
+static void draw3_px_f64(void *vctx, const long *p, double err)
+{
+    draw3_ctx *c = (draw3_ctx *)vctx;
+    if (!(err <= 1.0) || p[0] < 0 || p[0] >= c->W || p[1] < 0 || p[1] >= c->H || p[2] < 0 || p[2] >= c->D) return;
+    double v = c->max_val * prof_f64(c->kernel, sqrt(err));
+    double *o = (double *)c->out + ((c->frame * c->D + p[2]) * c->H + p[1]) * c->W + p[0];
+    if (c->overlap) *o = (*o < v) ? v : *o;
+    else *o = *o + v;
+}
+
+/* draw_lines_nd_impl<T, I, 3> with one thread (bresenham.cpp:38-101): out is
+ * n_frames volumes of D x H x W (z, y, x), lines n_lines rows of 7. */
+int cbxo_draw_lines3_f32(float *out, long n_frames, long D, long H, long W, const float *lines, long n_lines,
+                         const int32_t *idxs, float max_val, int kernel, int overlap)
+{
+    if (kernel < 0 || kernel > 4 || overlap < 0 || overlap > 1) return CBXO_EVALUE;
+    int rc = check_idxs_i32(idxs, n_lines, n_frames);
+    if (rc) return rc;
+    draw3_ctx c = {out, D, H, W, 0, kernel, overlap, max_val};
+    for (long i = 0; i < n_lines; i++) {
+        c.frame = frame_of(i, n_frames, idxs, 0);
+        if ((rc = walk3_f32(lines + 7 * i, draw3_px_f32, &c))) return rc;
+    }
+    return CBXO_OK;
+}
+
+int cbxo_draw_lines3_f64(double *out, long n_frames, long D, long H, long W, const double *lines, long n_lines,
+                         const int64_t *idxs, double max_val, int kernel, int overlap)
+{
+    if (kernel < 0 || kernel > 4 || overlap < 0 || overlap > 1) return CBXO_EVALUE;
+    int rc = check_idxs_i64(idxs, n_lines, n_frames);
+    if (rc) return rc;
+    draw3_ctx c = {out, D, H, W, 0, kernel, overlap, max_val};
+    for (long i = 0; i < n_lines; i++) {
+        c.frame = frame_of(i, n_frames, idxs, 1);
+        if ((rc = walk3_f64(lines + 7 * i, draw3_px_f64, &c))) return rc;
+    }
+    return CBXO_OK;
+}
+
+/* accumulate_lines_nd_impl<T, I, 3, ...> with one thread (bresenham.cpp:151-252):
+ * the 2D term-buffer procedure above on D x H x W frames and 3D walks.
+ * Pinned by: the compiled reference on 3D inputs (tests/test_draw3.py). */
+static void acc3_px_f32(void *vctx, const long *p, float err)
+{
+    acc_ctx *c = (acc_ctx *)vctx;
+    if (!(err <= 1.0f) || p[0] < 0 || p[0] >= c->W || p[1] < 0 || p[1] >= c->H || p[2] < 0 || p[2] >= c->D) return;
+    const long i = (p[2] * c->H + p[1]) * c->W + p[0];
+    float *b = (float *)c->buf + i;
+    if (*b == 0.0f) acc_push(c, i);
+    const float v = (float)c->max_val * prof_f32(c->kernel, sqrtf(err));
+    if (c->in_ov) *b = (*b < v) ? v : *b;
+    else *b = *b + v;
+}
+
+static void acc3_px_f64(void *vctx, const long *p, double err)
+{
+    acc_ctx *c = (acc_ctx *)vctx;
+    if (!(err <= 1.0) || p[0] < 0 || p[0] >= c->W || p[1] < 0 || p[1] >= c->H || p[2] < 0 || p[2] >= c->D) return;
+    const long i = (p[2] * c->H + p[1]) * c->W + p[0];
+    double *b = (double *)c->buf + i;
+    if (*b == 0.0) acc_push(c, i);
+    const double v = c->max_val * prof_f64(c->kernel, sqrt(err));
+    if (c->in_ov) *b = (*b < v) ? v : *b;
+    else *b = *b + v;
+}
+
+int cbxo_accumulate_lines3_f32(float *out, long n_frames, long D, long H, long W, const float *lines, long n_lines,
+                               const int32_t *terms, const int32_t *frames, long n_terms, float max_val, int kernel,
+                               int in_ov, int out_ov)
+{
+    ACCUMULATE_ND(float, walk3_f32, acc3_px_f32, int32_t, 7, D)
+}
+
+int cbxo_accumulate_lines3_f64(double *out, long n_frames, long D, long H, long W, const double *lines,
+                               long n_lines, const int64_t *terms, const int64_t *frames, long n_terms,
+                               double max_val, int kernel, int in_ov, int out_ov)
+{
+    ACCUMULATE_ND(double, walk3_f64, acc3_px_f64, int64_t, 7, D)
 }

@@ -64,6 +64,10 @@ def lib() -> ctypes.CDLL:
         _lib.cbxo_draw_lines_omp_f64.argtypes = [P, L, L, L, P, L, P, D, I, I, I]
         _lib.cbxo_label_omp.argtypes = [P, I, I, P, P, L, L, P, I]
         _lib.cbxo_label_omp.restype = L
+        _lib.cbxo_draw_lines3_f32.argtypes = [P, L, L, L, L, P, L, P, F, I, I]
+        _lib.cbxo_draw_lines3_f64.argtypes = [P, L, L, L, L, P, L, P, D, I, I]
+        _lib.cbxo_accumulate_lines3_f32.argtypes = [P, L, L, L, L, P, L, P, P, L, F, I, I, I]
+        _lib.cbxo_accumulate_lines3_f64.argtypes = [P, L, L, L, L, P, L, P, P, L, D, I, I, I]
         _lib.cbxo_region_moments_f32.argtypes = [P, P, I, P, P, L, I, P]
         _lib.cbxo_region_moments_f64.argtypes = [P, P, I, P, P, L, I, P]
         _lib.cbxo_p_values_f32.argtypes = [P, P, I, P, P, L, P, F, F, F, P, P]
@@ -130,6 +134,29 @@ def draw_lines(out: np.ndarray, lines: np.ndarray, idxs: Optional[np.ndarray] = 
     return out
 
 
+def draw_lines3(out: np.ndarray, lines: np.ndarray, idxs: Optional[np.ndarray] = None, max_val: float = 1.0,
+                kernel: str = 'rectangular', overlap: str = 'sum') -> np.ndarray:
+    """Single-threaded restatement of bresenham.draw_lines for 3D lines (rows
+    of 7, draw_line_3d, bresenham.hpp:389-430) into out (..., D, H, W); in place."""
+    if kernel not in KERNELS:
+        raise ValueError(f"kernel is missing for {kernel}")
+    if overlap not in OVERLAPS:
+        raise ValueError(f"Invalid overlap keyword: {overlap}")
+    assert out.flags.c_contiguous and out.ndim >= 3
+    D, H, W = out.shape[-3:]
+    F = int(np.prod(out.shape[:-3], dtype=np.int64))
+    f64 = out.dtype == np.float64
+    ln = np.ascontiguousarray(lines, dtype=np.float64 if f64 else np.float32).reshape(-1, 7)
+    ix = None if idxs is None else np.ascontiguousarray(idxs, dtype=np.int64 if f64 else np.int32).ravel()
+    if ix is not None and ix.size != ln.shape[0]:
+        raise ValueError("idxs has an invalid size")
+    fn = lib().cbxo_draw_lines3_f64 if f64 else lib().cbxo_draw_lines3_f32
+    rc = fn(_ptr(out), F, D, H, W, _ptr(ln), ln.shape[0], _ptr(ix), float(max_val), KERNELS[kernel],
+            OVERLAPS[overlap])
+    _raise(rc, "oracle draw_lines3 failed")
+    return out
+
+
 def pixel_list(shape, lines: np.ndarray, idxs: Optional[np.ndarray] = None, max_val: float = 1.0,
                kernel: str = 'rectangular', dtype=np.float32):
     """Raw emission list (line, frame, y, x, value, err), one entry per emitted pixel."""
@@ -174,6 +201,34 @@ def accumulate_lines(out: np.ndarray, lines: np.ndarray, terms: np.ndarray, fram
         raise ValueError("Term indices is incompatible with number of lines")
     fn = lib().cbxo_accumulate_lines_f64 if f64 else lib().cbxo_accumulate_lines_f32
     rc = fn(_ptr(out), F, H, W, _ptr(ln), ln.shape[0], _ptr(tm), _ptr(fr), fr.size, float(max_val),
+            KERNELS[kernel], OVERLAPS[in_overlap], OVERLAPS[out_overlap])
+    _raise(rc, "Term index is out of range")
+    return out
+
+
+def accumulate_lines3(out: np.ndarray, lines: np.ndarray, terms: np.ndarray, frames: np.ndarray,
+                      max_val: float = 1.0, kernel: str = 'rectangular', in_overlap: str = 'sum',
+                      out_overlap: str = 'sum') -> np.ndarray:
+    """:func:`accumulate_lines` for 3D lines (rows of 7) into out (..., D, H, W)
+    (accumulate_lines_nd_impl with N = 3, bresenham.cpp:151-252); in place."""
+    if kernel not in KERNELS:
+        raise ValueError(f"kernel is missing for {kernel}")
+    if in_overlap not in OVERLAPS or out_overlap not in OVERLAPS:
+        raise ValueError(f"Invalid in_overlap and out_overlap keywords: {in_overlap} and {out_overlap}")
+    assert out.flags.c_contiguous and out.dtype in (np.float32, np.float64) and out.ndim >= 3
+    D, H, W = out.shape[-3:]
+    F = int(np.prod(out.shape[:-3], dtype=np.int64))
+    f64 = out.dtype == np.float64
+    it = np.int64 if f64 else np.int32
+    ln = np.ascontiguousarray(lines, dtype=out.dtype).reshape(-1, 7)
+    tm = np.ascontiguousarray(terms, dtype=it).ravel()
+    fr = np.ascontiguousarray(frames, dtype=it).ravel()
+    if np.any((fr < 0) | (fr >= F)):
+        raise IndexError("frames is outside of (0, n_frames)")
+    if tm.size != ln.shape[0]:
+        raise ValueError("Term indices is incompatible with number of lines")
+    fn = lib().cbxo_accumulate_lines3_f64 if f64 else lib().cbxo_accumulate_lines3_f32
+    rc = fn(_ptr(out), F, D, H, W, _ptr(ln), ln.shape[0], _ptr(tm), _ptr(fr), fr.size, float(max_val),
             KERNELS[kernel], OVERLAPS[in_overlap], OVERLAPS[out_overlap])
     _raise(rc, "Term index is out of range")
     return out

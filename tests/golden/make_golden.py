@@ -13,7 +13,8 @@ so sums are in ascending line order).  Run from the repository root:
     make -C oracle ref && python tests/golden/make_golden.py [draw label config1 ...]
 
 (no names: every fixture file).  ``moments_cases.npz`` holds the reference's
-region moments and p-values of labelled random masks.  ``config1.npz`` holds the reference's render
+region moments and p-values of labelled random masks; ``draw3_cases.npz`` its
+draw_lines and accumulate_lines on 3D lines (rows of 7).  ``config1.npz`` holds the reference's render
 of the 512x512 / 1000-segment workload (seed 0): the rectangular image whole,
 the triangular image as a SHA-256 of its bytes plus rows 200-263, the SHA-256
 of its raw pixel list (line, y, x as int32), and a LabelResult case.
@@ -125,12 +126,79 @@ def moments_fixtures(lab):
     print(f"wrote moments_cases.npz ({len(cases)} cases)")
 
 
+def draw3_case(rng, shape, n, dtype):
+    """3D lines in volumes ``shape`` (..., D, H, W): random, half-integer end
+    points, axis-aligned, zero length, non-positive and sub-pixel widths,
+    lines leaving the volume."""
+    D, H, W = shape[-3:]
+    p0 = rng.uniform(-4, [W + 4, H + 4, D + 4], (n, 3))
+    p1 = p0 + rng.normal(0, 7, (n, 3))
+    w = rng.uniform(0.3, 4.0, (n, 1))
+    lines = np.concatenate([p0, p1, w], 1)
+    lines[:3, :6] = np.round(lines[:3, :6] * 2) / 2            # half-integer end points
+    lines[3, 3:5] = lines[3, 0:2]                               # along z
+    lines[4, 4:6] = lines[4, 1:3]                               # along x
+    lines[5, 3:6] = lines[5, 0:3]                               # zero length
+    lines[6, 6] = 0.0
+    lines[7, 6] = -1.0
+    lines[8, 6] = 0.25
+    return lines.astype(dtype)
+
+
+def draw3_fixtures(bres):
+    """The reference's bresenham.draw_lines and accumulate_lines on rows of 7
+    (draw_lines_2d_3d / accumulate_lines_2d_3d, bresenham.cpp:126-149,
+    284-309, single thread) on the shapes of its own tests
+    (tests/draw_line_test.py: (4, 3, 16, 22) and (2, 20, 16, 22)).
+    Accumulate cases hold at most 16 lines, where the reference's std::sort of
+    the lines by term keeps their order (insertion sort), so its sums are in
+    ascending line order."""
+    rng = np.random.default_rng(7)
+    fx = {}
+    i = 0
+    for dtype in (np.float32, np.float64):
+        it = np.int64 if dtype == np.float64 else np.int32
+        for k, kernel in enumerate(KERNELS):
+            for overlap in ('sum', 'max'):
+                shape = (4, 3, 16, 22) if (k + i) % 2 else (2, 20, 16, 22)
+                lines = draw3_case(rng, shape, 24, dtype)
+                idxs = rng.integers(0, int(np.prod(shape[:-3])), len(lines)).astype(it)
+                init = np.zeros(shape, dtype)
+                init.flat[rng.integers(0, init.size, 64)] = rng.uniform(-1, 1, 64)
+                out = init.copy()
+                bres.draw_lines(out, lines, idxs, 1.7, kernel, overlap, 1)
+                fx.update({f'd{i}_lines': lines, f'd{i}_idxs': idxs, f'd{i}_init': init, f'd{i}_out': out,
+                           f'd{i}_meta': np.array([k, 0 if overlap == 'sum' else 1], np.int64)})
+                i += 1
+    j = 0
+    for dtype in (np.float32, np.float64):
+        it = np.int64 if dtype == np.float64 else np.int32
+        for m, (iov, oov) in enumerate((('sum', 'sum'), ('max', 'sum'), ('sum', 'max'), ('max', 'max'))):
+            for max_val in (1.3, -0.6):
+                shape = (3, 5, 12, 14)
+                lines = draw3_case(rng, shape, 16, dtype)
+                terms = rng.integers(0, 5, len(lines)).astype(it)
+                frames = rng.integers(0, shape[0], 5).astype(it)
+                init = np.round(rng.uniform(-0.4, 0.6, shape), 1).astype(dtype)
+                out = init.copy()
+                kernel = KERNELS[(m + j) % 5]
+                bres.accumulate_lines(out, lines, terms, frames, max_val, kernel, iov, oov, 1)
+                fx.update({f'a{j}_lines': lines, f'a{j}_terms': terms, f'a{j}_frames': frames, f'a{j}_init': init,
+                           f'a{j}_out': out, f'a{j}_meta': np.array([KERNELS.index(kernel), m], np.int64),
+                           f'a{j}_max_val': np.array(max_val, np.float64)})
+                j += 1
+    np.savez_compressed(os.path.join(HERE, 'draw3_cases.npz'), **fx)
+    print(f"wrote draw3_cases.npz ({i} draw and {j} accumulate cases)")
+
+
 def main():
     bres = O.ref_module('bresenham')
     lab = O.ref_module('label')
     if bres is None or lab is None:
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources exist")
-    names = sys.argv[1:] or ['draw', 'label', 'config1', 'moments']
+    names = sys.argv[1:] or ['draw', 'label', 'config1', 'moments', 'draw3']
+    if 'draw3' in names:
+        draw3_fixtures(bres)
     if 'config1' in names:
         config1_fixtures(bres, lab)
     if 'moments' in names:

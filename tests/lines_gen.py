@@ -47,3 +47,18 @@ def config3(frames=256, H=1024, W=1024, dtype=np.float32, seed=2):
     profile = rng.integers(0, 2, int(counts.sum()))  # 0 = triangular, 1 = gaussian
     it = np.int64 if np.dtype(dtype) == np.float64 else np.int32
     return lines.astype(dtype), idxs.astype(it), profile, counts
+
+
+def segments3(rng, n, shape, length=10.0, wmin=2.0, wmax=2.0, pad=0.0):
+    """3D lines as rows ``[x0, y0, z0, x1, y1, z1, w]`` in a volume ``shape``
+    (..., D, H, W): start uniform in the volume (x along W), random direction,
+    length U[0, length] (the generator of the reference's own draw_lines tests
+    for ndim = 3), widths U[wmin, wmax]."""
+    D, H, W = shape[-3:]
+    lo, hi = -pad, np.array([W, H, D], np.float64) + pad
+    p0 = lo + (hi - lo) * rng.random((n, 3))
+    vec = rng.normal(size=(n, 3))
+    lengths = length * rng.random(n)
+    p1 = p0 + vec * (lengths / np.sqrt(np.sum(vec ** 2, axis=-1)))[:, None]
+    w = rng.uniform(wmin, wmax, (n, 1))
+    return np.concatenate([p0, p1, w], 1)
