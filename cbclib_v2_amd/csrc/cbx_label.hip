@@ -33,7 +33,7 @@ constexpr int LBL_MAX_DIM = 7;
 
 // 0: fastest applicable path; 1: skip the run path; 2: general path only
 static int g_label_path = 0;
-constexpr int LBL_MAX_SHIFTS = 256;
+constexpr int LBL_MAX_SHIFTS = 1 << 16;   // host-side sanity bound (radii 1 in 7D: 3^7 - 1 = 2186 shifts)
 
 struct LabelGeom {
     int ndim;

@@ -14,7 +14,8 @@ so sums are in ascending line order).  Run from the repository root:
 
 (no names: every fixture file).  ``moments_cases.npz`` holds the reference's
 region moments and p-values of labelled random masks; ``draw3_cases.npz`` its
-draw_lines and accumulate_lines on 3D lines (rows of 7).  ``config1.npz`` holds the reference's render
+draw_lines and accumulate_lines on 3D lines (rows of 7); ``label_hd_cases.npz``
+its label on ranks 1 and 4-7.  ``config1.npz`` holds the reference's render
 of the 512x512 / 1000-segment workload (seed 0): the rectangular image whole,
 the triangular image as a SHA-256 of its bytes plus rows 200-263, the SHA-256
 of its raw pixel list (line, y, x as int32), and a LabelResult case.
@@ -191,12 +192,43 @@ def draw3_fixtures(bres):
     print(f"wrote draw3_cases.npz ({i} draw and {j} accumulate cases)")
 
 
+HD_CASES = [((64,), [2], 1), ((5, 4, 6, 7), [1, 1, 1, 1], 1), ((5, 4, 6, 7), [1, 0, 1, 1], 3),
+            ((3, 4, 3, 5, 6), [1, 1, 1, 1, 1], 2), ((3, 3, 4, 3, 5, 4), [1, 1, 1, 1, 1, 1], 6),
+            ((2, 3, 3, 2, 4, 3, 4), [1, 1, 1, 1, 1, 1, 1], 1), ((2, 3, 3, 2, 4, 3, 4), [1, 1, 0, 1, 1, 1, 1], 4),
+            ((2, 2, 3, 2, 3, 3, 4), [1, 1, 1, 1, 1, 1, 1], 7)]
+
+
+def label_hd_fixtures(lab):
+    """The reference's label (label.cpp:109-175, 459-493) on ranks 1 and 4-7,
+    the ranks its Structure and label accept beyond the 2D/3D cases."""
+    rng = np.random.default_rng(31)
+    fx = {}
+    for c, (shape, radii, conn) in enumerate(HD_CASES):
+        for kind in ('bool', 'int64'):
+            if kind == 'bool':
+                inp = rng.random(shape) < 0.4
+            else:
+                inp = (rng.random(shape) < 0.5) * rng.integers(1, 3, shape)
+            for npts in (1, 3):
+                res = lab.label(inp, lab.Structure(radii, conn), npts, 1)
+                n = len(res.regions)
+                out = res.to_array(np.arange(1, n + 1)).astype(np.int32) if n else np.zeros(shape, np.int32)
+                key = f'c{c}_{kind}_{npts}'
+                fx[key + '_inp'] = inp
+                fx[key + '_out'] = out
+                fx[key + '_meta'] = np.array([conn, npts, n] + list(radii), np.int64)
+    np.savez_compressed(os.path.join(HERE, 'label_hd_cases.npz'), **fx)
+    print(f"wrote label_hd_cases.npz ({len(fx) // 3} cases)")
+
+
 def main():
     bres = O.ref_module('bresenham')
     lab = O.ref_module('label')
     if bres is None or lab is None:
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources exist")
-    names = sys.argv[1:] or ['draw', 'label', 'config1', 'moments', 'draw3']
+    names = sys.argv[1:] or ['draw', 'label', 'config1', 'moments', 'draw3', 'label_hd']
+    if 'label_hd' in names:
+        label_hd_fixtures(lab)
     if 'draw3' in names:
         draw3_fixtures(bres)
     if 'config1' in names:
