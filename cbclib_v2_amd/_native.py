@@ -27,7 +27,8 @@ EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32
            'cbx_label_force_path', 'cbx_plan_create_f32', 'cbx_plan_draw_f32', 'cbx_plan_check', 'cbx_plan_bytes',
            'cbx_plan_destroy', 'cbx_label_noalloc', 'cbx_label_noalloc_capacity', 'cbx_region_moments_f32',
            'cbx_region_moments_f64', 'cbx_p_values_f32', 'cbx_p_values_f64', 'cbx_draw_lines3_f32',
-           'cbx_draw_lines3_f64', 'cbx_accumulate_lines3_f32', 'cbx_accumulate_lines3_f64')
+           'cbx_draw_lines3_f64', 'cbx_accumulate_lines3_f32', 'cbx_accumulate_lines3_f64',
+           'cbx_binary_dilation')
 MOMENTS = {'total_mass': 0, 'mean': 1, 'center_of_mass': 2, 'moment_of_inertia': 3, 'covariance_matrix': 4,
            'line_fit': 5}
 
@@ -88,6 +89,8 @@ def _declare(L: ctypes.CDLL) -> None:
         fn = getattr(L, 'cbx_p_values_' + name)
         fn.restype = I32
         fn.argtypes = [P, P, I32, P, P, I64, P, R, R, R, P, P, P]
+    L.cbx_binary_dilation.restype = I32
+    L.cbx_binary_dilation.argtypes = [P, P, I32, P, P, I64, I64, P, P]
     L.cbx_label_noalloc.restype = I32
     L.cbx_label_noalloc.argtypes = [P, I32, I32, P, P, I64, I64, P, ctypes.POINTER(ctypes.c_int64), P]
     L.cbx_label_noalloc_capacity.restype = I64

@@ -24,7 +24,7 @@ from . import _native, device
 from ._arrays import NUMPY, TORCH, dtype_of, namespace, shape_of, stream_ptr, to_device
 
 __all__ = ['Structure', 'Region', 'LabelResult', 'NPLabelResult', 'CPLabelResult', 'label', 'label_threshold',
-           'labels', 'index', 'center_of_mass', 'covariance_matrix', 'ellipse_fit', 'line_fit', 'p_values',
+           'binary_dilation', 'labels', 'index', 'center_of_mass', 'covariance_matrix', 'ellipse_fit', 'line_fit', 'p_values',
            'total_mass', 'mean', 'moment_of_inertia']
 
 
@@ -260,6 +260,38 @@ def label_threshold(image: Any, vmin: float, structure: Structure, npts: int = 1
                                                         shifts.ctypes.data, shifts.shape[0], int(npts),
                                                         t_out.data_ptr(), ctypes.byref(n), stream_ptr()))
     return _finish(t_out, n.value, like, shape)
+
+
+def binary_dilation(inp: Any, structure: Structure, iterations: int = 1, mask: Any = None):
+    """Dilate a boolean array ``iterations`` times by ``structure``
+    (functions.py:304-343 -> label.binary_dilation, label.cpp:8-107).
+
+    Every true element of ``inp`` grows by the structure's shifts, inside the
+    array; with ``mask`` an element joins only where the mask is true.
+    Returns a new bool array (NumPy for NumPy input, a GPU tensor otherwise).
+    ``iterations = 0`` returns a copy, as the reference's CPU module does.
+    Non-boolean input is cast to bool.  A mask of another shape or a
+    structure of another rank raises ValueError.
+    """
+    device.require_rocm('binary_dilation')
+    shape = shape_of(inp)
+    shifts = _shifts_for(structure, len(shape))
+    iterations = int(iterations)
+    if iterations < 0:
+        raise ValueError(f"iterations must be non-negative, got {iterations}")
+    t_in = to_device(inp, torch.bool)
+    t_mask = None
+    if mask is not None:
+        if tuple(shape_of(mask)) != tuple(shape):
+            # label.cpp:62-64
+            raise ValueError("input and mask must have the same shape")
+        t_mask = to_device(mask, torch.bool)
+    t_out = torch.empty(shape, dtype=torch.bool, device=t_in.device)
+    shp = np.asarray(shape, dtype=np.int64)
+    _native.check(_native.lib().cbx_binary_dilation(t_in.data_ptr(), None if t_mask is None else t_mask.data_ptr(),
+                                                    len(shape), shp.ctypes.data, shifts.ctypes.data,
+                                                    shifts.shape[0], iterations, t_out.data_ptr(), stream_ptr()))
+    return t_out.cpu().numpy() if namespace(inp) == NUMPY else t_out
 
 
 def index(labels: Union[LabelResult, CPLabelResult]):

@@ -226,6 +226,21 @@ CBX_API int cbx_label_noalloc(const void *inp, int dtype, int ndim, const int64_
 CBX_API int64_t cbx_label_noalloc_capacity(void);
 
 /*
+ * binary_dilation: out = inp dilated `iterations` times by the structure's
+ * `shifts` (n_shifts x ndim, HOST array, the non-zero shifts), an element
+ * joining only where `mask` is non-zero (mask NULL: everywhere); inp, mask
+ * and out are DEVICE arrays of ndim dims of `shape` with one byte per element
+ * (non-zero = true; out receives 0 / 1; out may alias inp).  iterations = 0
+ * copies the input.  Equals the reference with num_threads = 1 (its
+ * multi-threaded result keeps only the pixels each thread's seeds reach
+ * inside that thread's chunk).  Replaces label.binary_dilation
+ * (label.cpp:8-107, bound at 527-528).
+ */
+CBX_API int cbx_binary_dilation(const uint8_t *inp, const uint8_t *mask, int ndim, const int64_t *shape,
+                                const int64_t *shifts, int64_t n_shifts, int64_t iterations, uint8_t *out,
+                                void *stream);
+
+/*
  * label of (image > vmin) without materialising the mask: the threshold of
  * the detection pipeline (RegionDetector.detect_regions,
  * data_processing.py:630-631) fused into the first labelling pass.

@@ -1440,3 +1440,67 @@ int cbxo_accumulate_lines3_f64(double *out, long n_frames, long D, long H, long 
 {
     ACCUMULATE_ND(double, walk3_f64, acc3_px_f64, int64_t, 7, D)
 }
+
+/* ------------------------------------------------------------------------ */
+/* binary_dilation with one thread (dilate / dilate_with_mask,              */
+/* label.cpp:8-107, Region::dilate with n_iter, label.hpp:464-496, and      */
+/* detail::shift_index, label.hpp:28-45): the output starts as a copy of    */
+/* the input; every true input element seeds a region that grows           */
+/* `iterations` times by the structure's non-zero shifts (staying inside    */
+/* the array), a new element joining only where the mask is true (always   */
+/* without a mask); every element of every region is set.  One breadth-    */
+/* first search per seed, elements stamped with the seed's id.              */
+/* Pinned by: scipy.ndimage.binary_dilation (iterations >= 1, with and      */
+/* without mask), golden fixtures tests/golden/dilation_cases.npz, and the  */
+/* compiled reference label module (num_threads = 1).                       */
+/* ------------------------------------------------------------------------ */
+
+static long shift_flat(long index, const int64_t *shift, int ndim, const int64_t *shape)
+{
+    long out = 0, stride = 1;
+    for (int d = ndim - 1; d >= 0; d--) {
+        long c = index % shape[d] + shift[d];
+        if (c < 0 || c >= shape[d]) return -1;
+        out += c * stride;
+        index /= shape[d];
+        stride *= shape[d];
+    }
+    return out;
+}
+
+int cbxo_binary_dilation(const uint8_t *inp, const uint8_t *mask, int ndim, const int64_t *shape,
+                         const int64_t *shifts, long n_shifts, long iterations, uint8_t *out)
+{
+    if (ndim < 1 || iterations < 0) return CBXO_EVALUE;
+    long P = 1;
+    for (int d = 0; d < ndim; d++) P *= shape[d];
+    for (long i = 0; i < P; i++) out[i] = inp[i] != 0;
+    if (P == 0 || iterations == 0) return CBXO_OK;
+    long *stamp = (long *)malloc(sizeof(long) * (size_t)P);
+    long *cur = (long *)malloc(sizeof(long) * (size_t)P);
+    long *nxt = (long *)malloc(sizeof(long) * (size_t)P);
+    if (!stamp || !cur || !nxt) { free(stamp); free(cur); free(nxt); return CBXO_ERUNTIME; }
+    for (long i = 0; i < P; i++) stamp[i] = -1;
+    for (long seed = 0; seed < P; seed++) {
+        if (!inp[seed]) continue;
+        long nc = 0;
+        cur[nc++] = seed;
+        stamp[seed] = seed;
+        for (long it = 0; it < iterations && nc; it++) {
+            long nn = 0;
+            for (long k = 0; k < nc; k++)
+                for (long s = 0; s < n_shifts; s++) {
+                    long q = shift_flat(cur[k], shifts + s * ndim, ndim, shape);
+                    if (q < 0 || stamp[q] == seed) continue;
+                    if (mask && !mask[q]) continue;
+                    stamp[q] = seed;
+                    out[q] = 1;
+                    nxt[nn++] = q;
+                }
+            long *t = cur; cur = nxt; nxt = t;
+            nc = nn;
+        }
+    }
+    free(stamp); free(cur); free(nxt);
+    return CBXO_OK;
+}

@@ -66,6 +66,7 @@ def lib() -> ctypes.CDLL:
         _lib.cbxo_label_omp.restype = L
         _lib.cbxo_draw_lines3_f32.argtypes = [P, L, L, L, L, P, L, P, F, I, I]
         _lib.cbxo_draw_lines3_f64.argtypes = [P, L, L, L, L, P, L, P, D, I, I]
+        _lib.cbxo_binary_dilation.argtypes = [P, P, I, P, P, L, L, P]
         _lib.cbxo_accumulate_lines3_f32.argtypes = [P, L, L, L, L, P, L, P, P, L, F, I, I, I]
         _lib.cbxo_accumulate_lines3_f64.argtypes = [P, L, L, L, L, P, L, P, P, L, D, I, I, I]
         _lib.cbxo_region_moments_f32.argtypes = [P, P, I, P, P, L, I, P]
@@ -550,3 +551,25 @@ def ref_module(name: str):
     spec.loader.exec_module(mod)
     sys.modules[key] = mod
     return mod
+
+
+def binary_dilation(inp: np.ndarray, shifts: np.ndarray, iterations: int = 1,
+                    mask: Optional[np.ndarray] = None) -> np.ndarray:
+    """Single-threaded restatement of label.binary_dilation (label.cpp:8-107);
+    ``shifts`` are the structure's non-zero shifts (n, ndim).  Returns a new
+    bool array."""
+    a = np.ascontiguousarray(inp, dtype=np.bool_)
+    if iterations < 0:
+        raise ValueError("iterations must be non-negative")
+    m = None
+    if mask is not None:
+        m = np.ascontiguousarray(mask, dtype=np.bool_)
+        if m.shape != a.shape:
+            raise ValueError("input and mask must have the same shape")
+    shape = np.asarray(a.shape, np.int64)
+    sh = np.ascontiguousarray(shifts, dtype=np.int64).reshape(-1, max(a.ndim, 1))
+    out = np.zeros(a.shape, np.uint8)
+    rc = lib().cbxo_binary_dilation(_ptr(a.view(np.uint8)), None if m is None else _ptr(m.view(np.uint8)), a.ndim,
+                                    _ptr(shape), _ptr(sh), sh.shape[0], int(iterations), _ptr(out))
+    _raise(rc, "oracle binary_dilation failed")
+    return out.view(np.bool_)

@@ -15,7 +15,7 @@ so sums are in ascending line order).  Run from the repository root:
 (no names: every fixture file).  ``moments_cases.npz`` holds the reference's
 region moments and p-values of labelled random masks; ``draw3_cases.npz`` its
 draw_lines and accumulate_lines on 3D lines (rows of 7); ``label_hd_cases.npz``
-its label on ranks 1 and 4-7.  ``config1.npz`` holds the reference's render
+its label on ranks 1 and 4-7; ``dilation_cases.npz`` its binary_dilation.  ``config1.npz`` holds the reference's render
 of the 512x512 / 1000-segment workload (seed 0): the rectangular image whole,
 the triangular image as a SHA-256 of its bytes plus rows 200-263, the SHA-256
 of its raw pixel list (line, y, x as int32), and a LabelResult case.
@@ -221,12 +221,39 @@ def label_hd_fixtures(lab):
     print(f"wrote label_hd_cases.npz ({len(fx) // 3} cases)")
 
 
+def dilation_fixtures(lab):
+    """The reference's binary_dilation (label.cpp:8-107, num_threads = 1) with
+    and without a mask, iterations 0-3, on ranks 1-4."""
+    rng = np.random.default_rng(47)
+    fx = {}
+    cases = [((70,), [2], 1), ((33, 41), [1, 1], 1), ((33, 41), [1, 1], 2), ((30, 37), [2, 1], 2),
+             ((5, 19, 23), [0, 1, 1], 2), ((9, 10, 11), [1, 1, 1], 3), ((6, 5, 7, 8), [1, 0, 1, 1], 2),
+             ((20, 130), [0, 1], 1)]
+    c = 0
+    for shape, radii, conn in cases:
+        for it in (0, 1, 2, 3):
+            for masked in (False, True):
+                inp = rng.random(shape) < 0.08
+                mask = rng.random(shape) < 0.75 if masked else None
+                out = np.asarray(lab.binary_dilation(inp, lab.Structure(radii, conn), it, mask, 1))
+                fx[f'c{c}_inp'] = inp
+                if masked:
+                    fx[f'c{c}_mask'] = mask
+                fx[f'c{c}_out'] = out
+                fx[f'c{c}_meta'] = np.array([conn, it] + list(radii), np.int64)
+                c += 1
+    np.savez_compressed(os.path.join(HERE, 'dilation_cases.npz'), **fx)
+    print(f"wrote dilation_cases.npz ({c} cases)")
+
+
 def main():
     bres = O.ref_module('bresenham')
     lab = O.ref_module('label')
     if bres is None or lab is None:
         sys.exit("oracle/_ref is not built: run `make -C oracle ref` where the reference sources exist")
-    names = sys.argv[1:] or ['draw', 'label', 'config1', 'moments', 'draw3', 'label_hd']
+    names = sys.argv[1:] or ['draw', 'label', 'config1', 'moments', 'draw3', 'label_hd', 'dilation']
+    if 'dilation' in names:
+        dilation_fixtures(lab)
     if 'label_hd' in names:
         label_hd_fixtures(lab)
     if 'draw3' in names:
