@@ -407,6 +407,176 @@ def workload_acc(args):
     return res
 
 
+def _kernel_ms(name, fn, steps, warmup):
+    """Average launch time (HIP events on the launch stream) of the library
+    timer `name` over `steps` calls of fn, and the wall-clock ms per call."""
+    from cbclib_v2_amd import _native
+    for _ in range(warmup):
+        fn()
+    _native.timing_reset()
+    _native.timing_enable(True)
+    ms = timed(fn, steps, 0)
+    _native.timing_enable(False)
+    k_ms, k_n = _native.timing_read(name)
+    return ms, (k_ms / k_n if k_n else None)
+
+
+def _hbm(bytes_, ms, kernel):
+    gbs = bytes_ / (ms * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(gbs / HBM_PEAK_GBS, 4), "kernel": kernel, "bytes_per_launch": bytes_,
+            "avg_launch_ms": round(ms, 4)}
+
+
+def workload_rows(args):
+    """The rows beside the hot path, each on a realistic input, with its
+    dominant kernel's roofline (HIP events) and the oracle, timed on a bounded
+    sample, as CPU baseline:
+
+    * region moments (center_of_mass, covariance_matrix, line_fit) and
+      p_values of the C2 label output (64 frames of 2048^2: the labels of
+      render > 0.5, the render as data);
+    * binary_dilation of the C2 mask (render > 0.5, Structure([0, 1, 1], 2),
+      1 iteration, and 4 iterations under a mask);
+    * draw_lines on 3D lines: 16 volumes of 64 x 128 x 128 f32, 50 000
+      segments (lengths U[5, 40], widths U[1, 3], seed 9), gaussian, sum and
+      max."""
+    import numpy as np
+    import torch
+    from cbclib_v2_amd import label as cl, ndimage
+    from oracle import oracle as O
+    from tests.lines_gen import config2, segments3
+    dev = torch.device('cuda', 0)
+    steps, warm = max(3, args.steps // 2), 2
+    res = {}
+
+    # -- region moments / p-values on the C2 label output
+    F, H, W = 64, 2048, 2048
+    lines_np, idxs_np = config2(frames=F, per_frame=2000, H=H, W=W)
+    img = torch.zeros((F, H, W), dtype=torch.float32, device=dev)
+    ndimage.draw_lines(img, torch.from_numpy(lines_np).to(dev), torch.from_numpy(idxs_np).to(dev), 1.0,
+                       'gaussian', 'sum')
+    st = cl.Structure([0, 1, 1], 2)
+    lab = cl.label_threshold(img, 0.5, st)
+    n_reg = int(lab.index.numel())
+    P = F * H * W
+    mom = {}
+    for what in ('center_of_mass', 'covariance_matrix', 'line_fit'):
+        fn = getattr(cl, what)
+        ms, kms = _kernel_ms('region_moments', lambda: fn(lab, img), steps, warm)
+        mom[what] = {"ms_per_call": round(ms, 4), "roofline": _hbm(8.0 * P, kms, "k_region_accumulate")}
+    fit = cl.line_fit(lab, img)
+    ms, kms = _kernel_ms('region_counts', lambda: cl.p_values(lab, fit, img, 0.3, 0.6, 1.5), steps, warm)
+    mom["p_values"] = {"ms_per_call": round(ms, 4), "roofline": _hbm(8.0 * P, kms, "k_region_count")}
+    # parity and CPU baseline on frame 0 (the oracle's single-threaded restatement)
+    f0_lab = lab.labels[0].cpu().numpy()
+    f0_img = img[0].cpu().numpy()
+    ids = np.unique(f0_lab[f0_lab > 0]).astype(np.int64)
+    t0 = time.perf_counter()
+    ref = O.region_moments('line_fit', f0_lab, ids, f0_img)
+    cpu_dt = time.perf_counter() - t0
+    got = cl.line_fit(cl.CPLabelResult(labels=lab.labels[0].contiguous(), index=torch.from_numpy(ids).to(dev)),
+                      img[0].contiguous()).cpu().numpy()
+    scale = max(1.0, float(np.abs(ref).max()) if ref.size else 1.0)
+    res["moments"] = {"n_regions": n_reg, **mom,
+                      "parity_frame0": {"regions": int(ids.size),
+                                        "line_fit_max_abs": float(np.nanmax(np.abs(got - ref))) if ids.size else 0.0,
+                                        "within_1e-4_of_scale": bool(np.allclose(got, ref, atol=1e-4 * scale,
+                                                                                 rtol=1e-4, equal_nan=True))},
+                      "cpu_baseline": {"value": round(H * W / cpu_dt / 1e6, 3), "unit": "Mpx/s", "cores": 1,
+                                       "kind": "port", "sample": "line_fit of frame 0 (2048^2 labels + data)"},
+                      "gpu_Mpx_per_s_line_fit": round(P / (mom['line_fit']['ms_per_call'] * 1e-3) / 1e6, 1),
+                      "config": {"workload": f"region moments of label(render > 0.5) on the C2 render: "
+                                             f"{F} frames of {H}x{W} f32, {n_reg} regions"}}
+
+    # -- binary_dilation of the C2 mask
+    mask = img > 0.5
+    sel = torch.rand((F, H, W), device=dev) < 0.7
+    dil = {}
+    for name, it, m in (("1_iteration", 1, None), ("4_iterations_masked", 4, sel)):
+        ms, kms = _kernel_ms('dilation', lambda: cl.binary_dilation(mask, st, it, m), steps, warm)
+        bytes_ = (3.0 if m is not None else 2.0) * P
+        dil[name] = {"ms_per_call": round(ms, 4), "roofline": _hbm(bytes_, kms, "binary_dilation (pack, rounds, unpack)"),
+                     "Gpx_per_s": round(P / (ms * 1e-3) / 1e9, 2)}
+    m0 = mask[0].cpu().numpy()
+    t0 = time.perf_counter()
+    ref = O.binary_dilation(m0[None], st.shifts(), 1)[0]
+    cpu_dt = time.perf_counter() - t0
+    got = cl.binary_dilation(mask[0:1], st, 1)[0].cpu().numpy()
+    res["dilation"] = {**dil, "parity_frame0_bit_exact": bool(np.array_equal(got, ref)),
+                       "mask_density": round(float(mask.float().mean()), 5),
+                       "cpu_baseline": {"value": round(H * W / cpu_dt / 1e9, 4), "unit": "Gpx/s", "cores": 1,
+                                        "kind": "port", "sample": "frame 0, 1 iteration"},
+                       "config": {"workload": f"binary_dilation(render > 0.5, Structure([0,1,1],2)) on the C2 "
+                                              f"render: {F} frames of {H}x{W} bool"}}
+    del img, mask, sel, lab
+    torch.cuda.empty_cache()
+
+    # -- 3D lines
+    shape = (16, 64, 128, 128)
+    rng = np.random.default_rng(9)
+    n3 = 50000
+    l3 = segments3(rng, n3, shape, 40.0, 1.0, 3.0, 2.0)
+    l3[:, 3:6] = l3[:, 0:3] + (l3[:, 3:6] - l3[:, 0:3]) * (5.0 + 35.0 * rng.random((n3, 1))) / np.maximum(
+        np.linalg.norm(l3[:, 3:6] - l3[:, 0:3], axis=1, keepdims=True), 1e-9)
+    l3 = l3.astype(np.float32)
+    i3 = rng.integers(0, shape[0], n3).astype(np.int32)
+    t_l3, t_i3 = torch.from_numpy(l3).to(dev), torch.from_numpy(i3).to(dev)
+    vol = torch.zeros(shape, dtype=torch.float32, device=dev)
+    V = float(np.prod(shape))
+    d3 = {}
+    for ov in ('sum', 'max'):
+        def call():
+            vol.zero_()
+            ndimage.draw_lines(vol, t_l3, t_i3, 1.0, 'gaussian', ov)
+        ms, kms = _kernel_ms('draw3', call, steps, warm)
+        d3[ov] = {"ms_per_call": round(ms, 4), "Mlines_per_s": round(n3 / (ms * 1e-3) / 1e6, 3),
+                  "roofline": _hbm(8.0 * V + 28.0 * n3, kms, "k_draw3")}
+    sel3 = i3 == 0
+    o3 = np.zeros((1,) + shape[1:], np.float32)
+    t0 = time.perf_counter()
+    O.draw_lines3(o3, l3[sel3], np.zeros(int(sel3.sum()), np.int32), 1.0, 'gaussian', 'max')
+    cpu_dt = time.perf_counter() - t0
+    vol.zero_()
+    ndimage.draw_lines(vol, t_l3, t_i3, 1.0, 'gaussian', 'max')
+    res["draw3"] = {**d3, "parity_volume0_max_bit_exact": bool(np.array_equal(vol[0].cpu().numpy(), o3[0])),
+                    "voxels_drawn_fraction": round(float((vol != 0).float().mean()), 4),
+                    "cpu_baseline": {"value": round(int(sel3.sum()) / cpu_dt / 1e6, 4), "unit": "Mlines/s",
+                                     "cores": 1, "kind": "port",
+                                     "sample": f"volume 0 ({int(sel3.sum())} segments), max"},
+                    "config": {"workload": f"3D draw_lines: {shape[0]} volumes of {shape[1]}x{shape[2]}x{shape[3]} "
+                                           f"f32, {n3} segments (lengths U[5, 40], widths U[1, 3], seed 9), "
+                                           "gaussian"}}
+    del vol
+    torch.cuda.empty_cache()
+    return {"metric": "rows beside the hot path (moments, dilation, 3D lines)", "rows": res,
+            "steps": steps, "warmup": warm}
+
+
+def pmc_summary(src: str) -> str:
+    """Per-kernel mean per dispatch of a rocprofv3 counter CSV
+    (--pmc ... --output-format csv), as the text kept under profiles/."""
+    import collections
+    import csv
+    agg = collections.defaultdict(lambda: collections.defaultdict(float))
+    cnt = collections.Counter()
+    meta = {}
+    for r in csv.DictReader(open(src)):
+        k = r['Kernel_Name']
+        agg[k][r['Counter_Name']] += float(r['Counter_Value'])
+        cnt[(k, r['Counter_Name'])] += 1
+        meta[k] = (r['VGPR_Count'], r['Scratch_Size'], r['LDS_Block_Size'], r['Workgroup_Size'], r['Grid_Size'])
+    lines = []
+    for k, d in sorted(agg.items(), key=lambda kv: -kv[1].get('SQ_BUSY_CYCLES', 0)):
+        n = max(cnt[(k, c)] for c in d)
+        lines.append(f"{k}\n  dispatches {n}  vgpr,scratch,lds,wg,grid {meta[k]}")
+        for c, v in sorted(d.items()):
+            lines.append(f"  {c:26s} {v / cnt[(k, c)]:18.1f}")
+        if d.get('SQ_ACTIVE_INST_LDS'):
+            lines.append(f"  {'conflict/active LDS':26s} {d['SQ_LDS_BANK_CONFLICT'] / d['SQ_ACTIVE_INST_LDS']:18.3f}")
+    return "\n".join(lines)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__.split('\n')[0])
     ap.add_argument('--gpus', type=int, default=1)
@@ -418,22 +588,31 @@ def main() -> None:
     ap.add_argument('--threads', type=int, default=0)
     ap.add_argument('--sweep', action='store_true', help='time the tile kernel over launch shapes and exit')
     ap.add_argument('--no-cpu', action='store_true', help='skip the CPU baseline')
-    ap.add_argument('--workload', choices=['c2', 'c3', 'c4', 'c5', 'acc'], default='c2',
+    ap.add_argument('--workload', choices=['c2', 'c3', 'c4', 'c5', 'acc', 'rows'], default='c2',
                     help='c2 (default, the headline line); c3 batched uneven forward; c4 forward + adjoint; '
-                         'c5 Adam refinement; acc accumulate_lines (in max, out sum)')
+                         'c5 Adam refinement; acc accumulate_lines (in max, out sum); rows region moments, '
+                         'binary_dilation and 3D lines')
     ap.add_argument('--adam-steps', type=int, default=200)
     ap.add_argument('--headline-only', action='store_true',
                     help='the default workload only (no C3 / C4 summaries in the line)')
     ap.add_argument('--calibrate', action='store_true',
                     help='also run an elementwise kernel of known traffic (256 MiB read + 256 MiB written), the '
                          'reference for scaling FETCH_SIZE / WRITE_SIZE counters')
+    ap.add_argument('--deterministic', action='store_true',
+                    help='draw in the deterministic (reference-order) mode; for profiling k_tiles_ordered')
+    ap.add_argument('--pmc-summary', metavar='CSV',
+                    help='print the per-kernel means of a rocprofv3 counter CSV and exit (no GPU use)')
     args = ap.parse_args()
+    if args.pmc_summary:
+        print(pmc_summary(args.pmc_summary))
+        return
     world = int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
         fail_multi(max(args.gpus, world))
 
-    if args.workload in ('c3', 'c4', 'c5', 'acc'):
-        res = {'c3': workload_c3, 'c4': workload_c4, 'c5': workload_c5, 'acc': workload_acc}[args.workload](args)
+    if args.workload in ('c3', 'c4', 'c5', 'acc', 'rows'):
+        res = {'c3': workload_c3, 'c4': workload_c4, 'c5': workload_c5, 'acc': workload_acc,
+               'rows': workload_rows}[args.workload](args)
         res.update({"n_gpus": 1, "higher_is_better": True, "dtype": "f32", "data": "synthetic"})
         print(json.dumps(res))
         return
@@ -458,6 +637,8 @@ def main() -> None:
         kw['seg'] = args.seg
     if args.threads:
         kw['threads'] = args.threads
+    if args.deterministic:
+        kw['deterministic'] = True
     if args.sweep:
         sweep(out, lines, idxs, F, H, W, N)
         return
@@ -506,7 +687,7 @@ def main() -> None:
     ms_step = (t1 - t0) * 1e3 / args.steps
     tile_avg = tiles_ms / max(tiles_n, 1)
     fwd_bytes = 8.0 * F * H * W + 24.0 * N
-    achieved = fwd_bytes / (tile_avg * 1e-3) / 1e9
+    achieved = fwd_bytes / (tile_avg * 1e-3) / 1e9 if tile_avg > 0 else 0.0
     mask_density = float((out > 0.5).float().mean())
     result = {
         "metric": "draw_lines Mlines/s and fwd+bwd frames/s on 1 MI355X; % HBM peak; vs CPU",

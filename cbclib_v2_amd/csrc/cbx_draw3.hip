@@ -12,10 +12,11 @@
 // sqrt), so the emitted pixel set and the per-pixel values are the
 // reference's exactly.
 //
-// One lane walks one line and applies its pixels to `out` with global
-// atomics (add for sum, compare-and-swap for max): sums land in no fixed
-// order (within float rounding of the single-threaded reference), max is
-// exact.  A validation pass runs first, so a call that fails leaves `out`
+// Four lanes walk one line (each replays the main-axis walk and fills one
+// quadrant of every orthogonal plane; the first also emits the main-axis
+// pixels) and apply its pixels to `out` with global atomics (add for sum,
+// compare-and-swap for max): sums land in no fixed order (within float
+// rounding of the single-threaded reference), max is exact.  A validation pass runs first, so a call that fails leaves `out`
 // untouched.
 #include <hip/hip_runtime.h>
 
@@ -189,7 +190,7 @@ __device__ __forceinline__ T it3_err(const Line3<T> &L, const Iter3<T> &it)
 // compile-time constant once the loops are unrolled and the iterator state
 // stays in registers.
 template <int AXIS, typename T, typename Emit>
-__device__ __forceinline__ void walk3_axis(const Line3<T> &L, int *flags, Emit &emit)
+__device__ __forceinline__ void walk3_axis(const Line3<T> &L, int quad, int *flags, Emit &emit)
 {
     Iter3<T> it;
     T d[3];
@@ -214,9 +215,10 @@ __device__ __forceinline__ void walk3_axis(const Line3<T> &L, int *flags, Emit &
             return;
         }
         if (it.next[axis]) {
-            emit(it.cur, it3_err(L, it));
+            if (quad <= 0) emit(it.cur, it3_err(L, it));
 #pragma unroll
             for (int i = 0; i < 4; i++) {
+                if (quad >= 0 && i != quad) continue;
                 Iter3<T> xy = it;
                 xy.step[axis] = 0;                                    // collapse(iter, axis)
                 it3_update(L, xy);
@@ -255,12 +257,14 @@ __device__ __forceinline__ void walk3_axis(const Line3<T> &L, int *flags, Emit &
     }
 }
 
+// quad = -1: the whole walk; 0..3: the main-axis pixels (quad 0 only) and
+// the fills of one quadrant of every plane, so that four lanes share a line
 template <typename T, typename Emit>
-__device__ void walk3(const Line3<T> &L, int *flags, Emit &&emit)
+__device__ void walk3(const Line3<T> &L, int quad, int *flags, Emit &&emit)
 {
-    if (L.axis == 0) walk3_axis<0>(L, flags, emit);
-    else if (L.axis == 1) walk3_axis<1>(L, flags, emit);
-    else walk3_axis<2>(L, flags, emit);
+    if (L.axis == 0) walk3_axis<0>(L, quad, flags, emit);
+    else if (L.axis == 1) walk3_axis<1>(L, quad, flags, emit);
+    else walk3_axis<2>(L, quad, flags, emit);
 }
 
 // *p = std::max(*p, v) (replaced only if *p < v), or *p += v
@@ -317,13 +321,14 @@ __global__ void __launch_bounds__(D3_NT)
 k_draw3(T *__restrict__ out, const T *__restrict__ lines, int64_t n_lines, const I *__restrict__ idxs, Vol3 v, T max_val,
         int *__restrict__ flags)
 {
-    const int64_t i = (int64_t)blockIdx.x * D3_NT + threadIdx.x;
-    if (i >= n_lines) return;
+    const int64_t u = (int64_t)blockIdx.x * D3_NT + threadIdx.x;
+    if (u >= 4 * n_lines) return;
+    const int64_t i = u >> 2;
     Line3<T> L;
     if (line3_setup(lines + 7 * i, L) != 1) return;
     const int64_t f = idxs ? (int64_t)idxs[i] : ((v.n_frames == 1) ? 0 : i);
     T *fo = out + f * v.D * v.H * v.W;
-    walk3(L, flags, [&](const int64_t *p, T err) {
+    walk3(L, (int)(u & 3), flags, [&](const int64_t *p, T err) {
         if (!(err <= T(1)) || p[0] < 0 || p[0] >= v.W || p[1] < 0 || p[1] >= v.H || p[2] < 0 || p[2] >= v.D) return;
         apply_atomic<OV>(fo + (p[2] * v.H + p[1]) * v.W + p[0], pixel_value<K>(err, max_val));
     });
@@ -338,14 +343,15 @@ __global__ void __launch_bounds__(D3_NT)
 k_acc3_draw(T *__restrict__ buf, uint32_t *__restrict__ touched, const T *__restrict__ lines, int64_t n_lines,
             const I *__restrict__ terms, int64_t t0, int64_t nb, Vol3 v, T max_val, int *__restrict__ flags)
 {
-    const int64_t i = (int64_t)blockIdx.x * D3_NT + threadIdx.x;
-    if (i >= n_lines) return;
+    const int64_t u = (int64_t)blockIdx.x * D3_NT + threadIdx.x;
+    if (u >= 4 * n_lines) return;
+    const int64_t i = u >> 2;
     const int64_t b = (int64_t)terms[i] - t0;
     if (b < 0 || b >= nb) return;
     Line3<T> L;
     if (line3_setup(lines + 7 * i, L) != 1) return;
     const int64_t V = v.D * v.H * v.W;
-    walk3(L, flags, [&](const int64_t *p, T err) {
+    walk3(L, (int)(u & 3), flags, [&](const int64_t *p, T err) {
         if (!(err <= T(1)) || p[0] < 0 || p[0] >= v.W || p[1] < 0 || p[1] >= v.H || p[2] < 0 || p[2] >= v.D) return;
         const int64_t k = b * V + (p[2] * v.H + p[1]) * v.W + p[0];
         apply_atomic<IN_OV>(buf + k, pixel_value<K>(err, max_val));
@@ -400,6 +406,7 @@ k_term_frames3(const I *__restrict__ terms, const I *__restrict__ frames, int64_
 static int check_vol(int64_t n_frames, int64_t D, int64_t H, int64_t W, int64_t n_lines, int kernel, int overlap)
 {
     if (n_frames < 0 || D < 0 || H < 0 || W < 0 || n_lines < 0) return fail(CBX_EVALUE, "negative size");
+    if (n_lines >= (int64_t(1) << 34)) return fail(CBX_EVALUE, "too many lines");
     if (kernel < 0 || kernel > 4) return fail(CBX_EVALUE, "kernel is missing");
     if (overlap < 0 || overlap > 1) return fail(CBX_EVALUE, "Invalid overlap keyword");
     return CBX_OK;
@@ -440,7 +447,7 @@ template <typename T, typename I, int OV>
 static void launch_draw3(int kernel, T *out, const T *lines, int64_t n, const I *idxs, const Vol3 &v, T mv,
                          int *flags, hipStream_t st)
 {
-    const unsigned g = (unsigned)grid_for(n, D3_NT);
+    const unsigned g = (unsigned)grid_for(4 * n, D3_NT);   // four lanes per line
     switch (kernel) {
     case K_BIWEIGHT: k_draw3<T, I, K_BIWEIGHT, OV><<<g, D3_NT, 0, st>>>(out, lines, n, idxs, v, mv, flags); break;
     case K_GAUSSIAN: k_draw3<T, I, K_GAUSSIAN, OV><<<g, D3_NT, 0, st>>>(out, lines, n, idxs, v, mv, flags); break;
@@ -505,7 +512,7 @@ static int accumulate3_batches(T *out, const T *lines, int64_t n_lines, const I 
     CBX_HIP(hipMemsetAsync(touched.ptr, 0, sizeof(uint32_t) * words, st));
     for (int64_t t0 = 0; t0 < n_terms; t0 += nb) {
         const int64_t n = std::min(nb, n_terms - t0);
-        k_acc3_draw<T, I, K, IN_OV><<<grid_for(n_lines, D3_NT), D3_NT, 0, st>>>(buf.ptr, touched.ptr, lines, n_lines,
+        k_acc3_draw<T, I, K, IN_OV><<<grid_for(4 * n_lines, D3_NT), D3_NT, 0, st>>>(buf.ptr, touched.ptr, lines, n_lines,
                                                                                terms, t0, n, v, max_val, flags);
         k_acc3_fold<T, I, OUT_OV><<<grid_for(n * V, D3_NT), D3_NT, 0, st>>>(out, buf.ptr, touched.ptr, frames, t0, n, v);
         k_clear_words<<<grid_for((n * V + 31) / 32, D3_NT), D3_NT, 0, st>>>(touched.ptr, (n * V + 31) / 32);

@@ -43,6 +43,8 @@ struct MomGeom {
     int64_t chunks;             // chunks per row
     int32_t maxl;               // largest label in index (0: none)
     int32_t vec;                // rows start 16-byte aligned in the label map
+    int32_t dvec;               // and in the data
+    int32_t small;              // rows * chunks < 2^31: 32-bit index arithmetic
 };
 
 // accumulators per slot, xyz order (x = last axis):
@@ -120,16 +122,64 @@ __device__ __forceinline__ void load_chunk(const int32_t *__restrict__ labels, c
     }
 }
 
+// The data of one lane's chunk, every load issued before any use (the
+// accumulation's atomics would otherwise order each load after them).
+template <typename T>
+__device__ __forceinline__ void load_data(const T *__restrict__ data, const MomGeom &g, int64_t base, int64_t x0,
+                                          int n, T *dv)
+{
+    // 16-byte loads: 4 floats or 2 doubles
+    constexpr int VW = 16 / (int)sizeof(T);
+    typedef T vw __attribute__((ext_vector_type(VW)));
+    if (g.dvec && n == MOM_CHUNK) {
+        const vw *p = reinterpret_cast<const vw *>(data + base + x0);
+#pragma unroll
+        for (int q = 0; q < MOM_CHUNK / VW; q++) {
+            const vw v = __builtin_nontemporal_load(p + q);
+#pragma unroll
+            for (int k = 0; k < VW; k++) dv[VW * q + k] = v[k];
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < MOM_CHUNK; i++) dv[i] = (i < n) ? data[base + x0 + i] : T(0);
+    }
+}
+
+// row and first column of lane t's chunk (32-bit division when it fits)
+__device__ __forceinline__ void chunk_of(const MomGeom &g, int64_t t, int64_t &row, int64_t &x0)
+{
+    if (g.small) {
+        const uint32_t tt = (uint32_t)t, ch = (uint32_t)g.chunks;
+        const uint32_t r = tt / ch;
+        row = r;
+        x0 = (int64_t)(tt - r * ch) * MOM_CHUNK;
+    } else {
+        row = t / g.chunks;
+        x0 = (t - row * g.chunks) * MOM_CHUNK;
+    }
+}
+
 // xyz coordinates 1..N-1 of a row (coordinate 0 is the column)
 template <int N>
 __device__ __forceinline__ void row_coords(const MomGeom &g, int64_t row, int64_t *c)
 {
-    int64_t r = row;
+    if (g.small) {
+        uint32_t r = (uint32_t)row;
 #pragma unroll
-    for (int n = 1; n < N; n++) {
-        const int64_t d = g.shape[N - 1 - n];
-        c[n] = r % d;
-        r /= d;
+        for (int n = 1; n < N; n++) {
+            const uint32_t d = (uint32_t)g.shape[N - 1 - n];
+            const uint32_t q = r / d;
+            c[n] = r - q * d;
+            r = q;
+        }
+    } else {
+        int64_t r = row;
+#pragma unroll
+        for (int n = 1; n < N; n++) {
+            const int64_t d = g.shape[N - 1 - n];
+            c[n] = r % d;
+            r /= d;
+        }
     }
     c[0] = 0;
 }
@@ -141,9 +191,9 @@ k_region_accumulate(const int32_t *__restrict__ labels, const T *__restrict__ da
 {
     using Lay = MomLayout<N>;
     const int64_t t = (int64_t)blockIdx.x * MOM_NT + threadIdx.x;
-    const int64_t row = t / g.chunks;
-    if (row >= g.rows) return;
-    const int64_t x0 = (t - row * g.chunks) * MOM_CHUNK;
+    if (t >= g.rows * g.chunks) return;
+    int64_t row, x0;
+    chunk_of(g, t, row, x0);
     const int n = (int)min((int64_t)MOM_CHUNK, g.W - x0);
     const int64_t base = row * g.W;
     int32_t lab[MOM_CHUNK];
@@ -152,6 +202,8 @@ k_region_accumulate(const int32_t *__restrict__ labels, const T *__restrict__ da
 #pragma unroll
     for (int i = 0; i < MOM_CHUNK; i++) any |= lab[i] > 0;
     if (!any) return;
+    T dv[MOM_CHUNK];
+    load_data(data, g, base, x0, n, dv);
     int64_t c[N];
     row_coords<N>(g, row, c);
 
@@ -192,7 +244,7 @@ k_region_accumulate(const int32_t *__restrict__ labels, const T *__restrict__ da
             s0 = sx = sxx = 0.0;
         }
         if (l > 0) {
-            double v = (double)data[base + x0 + i];
+            double v = (double)dv[i];
             v = (v < 0.0) ? 0.0 : v;   // std::max(value, T()): NaN stays NaN
             const double x = (double)(x0 + i);
             s0 += v;
@@ -364,9 +416,9 @@ k_region_count(const int32_t *__restrict__ labels, const T *__restrict__ data, M
                unsigned long long *__restrict__ counts)
 {
     const int64_t t = (int64_t)blockIdx.x * MOM_NT + threadIdx.x;
-    const int64_t row = t / g.chunks;
-    if (row >= g.rows) return;
-    const int64_t x0 = (t - row * g.chunks) * MOM_CHUNK;
+    if (t >= g.rows * g.chunks) return;
+    int64_t row, x0;
+    chunk_of(g, t, row, x0);
     const int n = (int)min((int64_t)MOM_CHUNK, g.W - x0);
     const int64_t base = row * g.W;
     int32_t lab[MOM_CHUNK];
@@ -375,6 +427,8 @@ k_region_count(const int32_t *__restrict__ labels, const T *__restrict__ data, M
 #pragma unroll
     for (int i = 0; i < MOM_CHUNK; i++) any |= lab[i] > 0;
     if (!any) return;
+    T dv[MOM_CHUNK];
+    load_data(data, g, base, x0, n, dv);
     int64_t p[N];
     row_coords<N>(g, row, p);
     int32_t cur = 0, s = NO_SLOT;
@@ -401,7 +455,7 @@ k_region_count(const int32_t *__restrict__ labels, const T *__restrict__ data, M
         p[0] = x0 + i;
         if (seg_distance<N, T>(ln, p) < xtol) {
             nt++;
-            if (data[base + x0 + i] >= vmin) ns++;
+            if (dv[i] >= vmin) ns++;
         }
     }
     flush();
@@ -469,7 +523,7 @@ __global__ void k_logbinom(const unsigned long long *__restrict__ counts, const 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static int moment_geometry(int ndim, const int64_t *shape, const int32_t *labels, MomGeom &g)
+static int moment_geometry(int ndim, const int64_t *shape, const int32_t *labels, const void *data, MomGeom &g)
 {
     if (ndim < 2 || ndim > MOM_MAX_DIM)
         return fail(CBX_EVALUE, "Unsupported number of dimensions: " + std::to_string(ndim));
@@ -484,6 +538,8 @@ static int moment_geometry(int ndim, const int64_t *shape, const int32_t *labels
     g.rows = g.W ? P / g.W : 0;
     g.chunks = (g.W + MOM_CHUNK - 1) / MOM_CHUNK;
     g.vec = (g.W % 4 == 0 && ((uintptr_t)labels & 15) == 0) ? 1 : 0;
+    g.dvec = (g.W % 4 == 0 && ((uintptr_t)data & 15) == 0) ? 1 : 0;
+    g.small = (g.rows * g.chunks < (int64_t(1) << 31)) ? 1 : 0;
     g.maxl = 0;
     return CBX_OK;
 }
@@ -555,7 +611,7 @@ static int region_moments(const int32_t *labels, const T *data, int ndim, const 
     if (what < 0 || what > 5) return fail(CBX_EVALUE, "unknown region moment");
     if (n_index < 0) return fail(CBX_EVALUE, "negative index size");
     MomGeom g;
-    int rc = moment_geometry(ndim, shape, labels, g);
+    int rc = moment_geometry(ndim, shape, labels, data, g);
     if (rc) return rc;
     if (n_index == 0) return CBX_OK;
     hipStream_t st = (hipStream_t)stream;
@@ -578,7 +634,7 @@ static int p_values(const int32_t *labels, const T *data, int ndim, const int64_
 {
     if (n_index < 0) return fail(CBX_EVALUE, "negative index size");
     MomGeom g;
-    int rc = moment_geometry(ndim, shape, labels, g);
+    int rc = moment_geometry(ndim, shape, labels, data, g);
     if (rc) return rc;
     if (n_index == 0) return CBX_OK;
     hipStream_t st = (hipStream_t)stream;
