@@ -31,6 +31,7 @@ namespace cbx {
 
 constexpr int D3_NT = 128;
 constexpr double MAX_VISITS3 = double(int64_t(1) << 30);
+constexpr int64_t MAX_PLANES = int64_t(1) << 23;   // recorded planes of a two-phase draw (256 / 384 MiB)
 
 template <typename T>
 struct Iter3 {
@@ -189,10 +190,11 @@ __device__ __forceinline__ T it3_err(const Line3<T> &L, const Iter3<T> &it)
 // The long axis is a template argument, so that every array index below is a
 // compile-time constant once the loops are unrolled and the iterator state
 // stays in registers.
-template <int AXIS, typename T, typename Emit>
-__device__ __forceinline__ void walk3_axis(const Line3<T> &L, int quad, int *flags, Emit &emit)
+
+// the main iterator at m0 (LineIterator's constructor)
+template <typename T>
+__device__ __forceinline__ void it3_start(const Line3<T> &L, Iter3<T> &it)
 {
-    Iter3<T> it;
     T d[3];
     for (int i = 0; i < 3; i++) {
         it.step[i] = L.step[i];
@@ -206,65 +208,115 @@ __device__ __forceinline__ void walk3_axis(const Line3<T> &L, int quad, int *fla
         for (int i = 0; i < 3; i++) it.ne[q] += L.nrm[q][i] * d[i];
     }
     it3_update(L, it);
-    constexpr int axis = AXIS, ax1 = (AXIS + 1) % 3, ax2 = (AXIS + 2) % 3;
+}
+
+// The main loop: plane(it) for every iterator state whose long-axis step is
+// taken (the planes draw_line_3d fills).  False if the step cap was hit.
+template <int AXIS, typename T, typename Plane>
+__device__ __forceinline__ bool walk3_main(const Line3<T> &L, int *flags, Plane &plane)
+{
+    Iter3<T> it;
+    it3_start(L, it);
     int64_t main_cap = 64;
     for (int i = 0; i < 3; i++) main_cap += (L.m1[i] - L.m0[i]) * L.step[i];
     for (int64_t n = 0; !(it.cur[0] == L.m1[0] || it.cur[1] == L.m1[1] || it.cur[2] == L.m1[2]); n++) {
         if (n == main_cap) {
             atomicOr(flags, FLAG_STUCK);
-            return;
+            return false;
         }
-        if (it.next[axis]) {
-            if (quad <= 0) emit(it.cur, it3_err(L, it));
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                if (quad >= 0 && i != quad) continue;
-                Iter3<T> xy = it;
-                xy.step[axis] = 0;                                    // collapse(iter, axis)
-                it3_update(L, xy);
-                if (i & 1) xy.step[ax1] = -xy.step[ax1];              // flip
-                if (i >> 1) xy.step[ax2] = -xy.step[ax2];
-                it3_update(L, xy);
-                const int mv = ((i & 1) ^ (i >> 1)) ? ax2 : ax1;     // move(axis(1 + ...))
-                it3_inc(L, xy, xy.step[mv], mv);
-                it3_update(L, xy);
-                for (int64_t r = 0; it3_nerr(L, xy) < L.w2; it3_advance(L, xy), r++) {
-                    if (r == L.ray_cap) {
-                        atomicOr(flags, FLAG_STUCK);
-                        return;
-                    }
-                    emit(xy.cur, it3_err(L, xy));
-#pragma unroll
-                    for (int j = 1; j < 3; j++) {
-                        const int aj = (axis + j) % 3;
-                        if (!xy.next[aj]) continue;
-                        Iter3<T> x = xy;
-                        x.step[aj] = 0;                               // collapse(iter_xy, axis(j))
-                        it3_update(L, x);
-                        it3_advance(L, x);
-                        for (int64_t q = 0; it3_nerr(L, x) < L.w2; it3_advance(L, x), q++) {
-                            if (q == L.ray_cap) {
-                                atomicOr(flags, FLAG_STUCK);
-                                return;
-                            }
-                            emit(x.cur, it3_err(L, x));
-                        }
-                    }
-                }
-            }
-        }
+        if (it.next[AXIS]) plane(it);
         it3_advance(L, it);
     }
+    return true;
 }
 
-// quad = -1: the whole walk; 0..3: the main-axis pixels (quad 0 only) and
-// the fills of one quadrant of every plane, so that four lanes share a line
+// Quadrant i of the plane through the main iterator state `it`: a 2D walk
+// away from the line with single-axis rays from each of its pixels.
+template <int AXIS, typename T, typename Emit>
+__device__ __forceinline__ bool fill3_quadrant(const Line3<T> &L, const Iter3<T> &it, int i, int *flags, Emit &emit)
+{
+    constexpr int axis = AXIS, ax1 = (AXIS + 1) % 3, ax2 = (AXIS + 2) % 3;
+    Iter3<T> xy = it;
+    xy.step[axis] = 0;                                    // collapse(iter, axis)
+    it3_update(L, xy);
+    if (i & 1) xy.step[ax1] = -xy.step[ax1];              // flip
+    if (i >> 1) xy.step[ax2] = -xy.step[ax2];
+    it3_update(L, xy);
+    const int mv = ((i & 1) ^ (i >> 1)) ? ax2 : ax1;     // move(axis(1 + ...))
+    if (mv == ax1) it3_inc(L, xy, xy.step[ax1], ax1);
+    else it3_inc(L, xy, xy.step[ax2], ax2);
+    it3_update(L, xy);
+    for (int64_t r = 0; it3_nerr(L, xy) < L.w2; it3_advance(L, xy), r++) {
+        if (r == L.ray_cap) {
+            atomicOr(flags, FLAG_STUCK);
+            return false;
+        }
+        emit(xy.cur, it3_err(L, xy));
+#pragma unroll
+        for (int j = 1; j < 3; j++) {
+            const int aj = (axis + j) % 3;
+            if (!xy.next[aj]) continue;
+            Iter3<T> x = xy;
+            x.step[aj] = 0;                               // collapse(iter_xy, axis(j))
+            it3_update(L, x);
+            it3_advance(L, x);
+            for (int64_t q = 0; it3_nerr(L, x) < L.w2; it3_advance(L, x), q++) {
+                if (q == L.ray_cap) {
+                    atomicOr(flags, FLAG_STUCK);
+                    return false;
+                }
+                emit(x.cur, it3_err(L, x));
+            }
+        }
+    }
+    return true;
+}
+
+// the whole walk (quad = -1) or the main-axis pixels (quad 0 only) and the
+// fills of one quadrant of every plane (quad = 0..3)
+template <int AXIS, typename T, typename Emit>
+__device__ __forceinline__ void walk3_axis(const Line3<T> &L, int quad, int *flags, Emit &emit)
+{
+    bool ok = true;
+    auto plane = [&](const Iter3<T> &it) {
+        if (!ok) return;
+        if (quad <= 0) emit(it.cur, it3_err(L, it));
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            if (quad >= 0 && i != quad) continue;
+            if (ok) ok = fill3_quadrant<AXIS>(L, it, i, flags, emit);
+        }
+    };
+    walk3_main<AXIS>(L, flags, plane);
+}
+
 template <typename T, typename Emit>
 __device__ void walk3(const Line3<T> &L, int quad, int *flags, Emit &&emit)
 {
     if (L.axis == 0) walk3_axis<0>(L, quad, flags, emit);
     else if (L.axis == 1) walk3_axis<1>(L, quad, flags, emit);
     else walk3_axis<2>(L, quad, flags, emit);
+}
+
+// A recorded plane: the main iterator's errors and position (its steps are
+// the line's, its next flags follow from them).  line < 0: unused slot.
+template <typename T>
+struct PlaneState {
+    T te, ne[3];
+    int32_t cur[3];
+    int32_t line;
+};
+
+template <typename T>
+__device__ __forceinline__ void it3_from(const Line3<T> &L, const PlaneState<T> &ps, Iter3<T> &it)
+{
+    for (int i = 0; i < 3; i++) {
+        it.step[i] = L.step[i];
+        it.cur[i] = ps.cur[i];
+        it.ne[i] = ps.ne[i];
+    }
+    it.te = ps.te;
+    it3_update(L, it);
 }
 
 // *p = std::max(*p, v) (replaced only if *p < v), or *p += v
@@ -332,6 +384,97 @@ k_draw3(T *__restrict__ out, const T *__restrict__ lines, int64_t n_lines, const
         if (!(err <= T(1)) || p[0] < 0 || p[0] >= v.W || p[1] < 0 || p[1] >= v.H || p[2] < 0 || p[2] >= v.D) return;
         apply_atomic<OV>(fo + (p[2] * v.H + p[1]) * v.W + p[0], pixel_value<K>(err, max_val));
     });
+}
+
+// Two-phase draw: planes of every line recorded by one lane per line
+// (k_walk3_main), then one lane per (plane, quadrant) fills (k_fill3), so the
+// lanes of a wave run consecutive planes of one line, whose fills take about
+// as many steps each (one lane per line left most of a wave idle behind its
+// longest and widest line).
+
+// ub[i]: planes of line i at most (the long-axis span), 0 if it draws nothing;
+// ub[n_lines] = 0 (the scan's total)
+template <typename T>
+__global__ void __launch_bounds__(D3_NT)
+k_plane_bound(const T *__restrict__ lines, int64_t n_lines, int64_t *__restrict__ ub)
+{
+    const int64_t i = (int64_t)blockIdx.x * D3_NT + threadIdx.x;
+    if (i > n_lines) return;
+    int64_t n = 0;
+    if (i < n_lines) {
+        Line3<T> L;
+        if (line3_setup(lines + 7 * i, L) == 1) {
+            const int64_t sp = (L.axis == 0) ? L.m1[0] - L.m0[0] : ((L.axis == 1) ? L.m1[1] - L.m0[1] : L.m1[2] - L.m0[2]);
+            n = sp < 0 ? -sp : sp;
+        }
+    }
+    ub[i] = n;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(D3_NT)
+k_walk3_main(const T *__restrict__ lines, int64_t n_lines, const int64_t *__restrict__ off,
+             PlaneState<T> *__restrict__ states, int *__restrict__ flags)
+{
+    const int64_t i = (int64_t)blockIdx.x * D3_NT + threadIdx.x;
+    if (i >= n_lines) return;
+    const int64_t base = off[i], cap = off[i + 1] - base;
+    if (cap == 0) return;
+    Line3<T> L;
+    int64_t k = 0;
+    if (line3_setup(lines + 7 * i, L) == 1) {
+        auto plane = [&](const Iter3<T> &it) {
+            if (k == cap) {   // cannot happen: one plane per long-axis step
+                atomicOr(flags, FLAG_STUCK);
+                return;
+            }
+            PlaneState<T> ps;
+            ps.te = it.te;
+            for (int q = 0; q < 3; q++) {
+                ps.ne[q] = it.ne[q];
+                ps.cur[q] = (int32_t)it.cur[q];
+            }
+            ps.line = (int32_t)i;
+            states[base + k++] = ps;
+        };
+        if (L.axis == 0) walk3_main<0>(L, flags, plane);
+        else if (L.axis == 1) walk3_main<1>(L, flags, plane);
+        else walk3_main<2>(L, flags, plane);
+    }
+    for (; k < cap; k++) states[base + k].line = -1;
+}
+
+template <int AXIS, typename T, typename Emit>
+__device__ __forceinline__ void fill_plane(const Line3<T> &L, const Iter3<T> &it, int quad, int *flags, Emit &emit)
+{
+    if (quad == 0) emit(it.cur, it3_err(L, it));
+    fill3_quadrant<AXIS>(L, it, quad, flags, emit);
+}
+
+template <typename T, typename I, int K, int OV>
+__global__ void __launch_bounds__(D3_NT)
+k_fill3(T *__restrict__ out, const T *__restrict__ lines, const PlaneState<T> *__restrict__ states, int64_t n_states,
+        int64_t n_lines, const I *__restrict__ idxs, Vol3 v, T max_val, int *__restrict__ flags)
+{
+    const int64_t u = (int64_t)blockIdx.x * D3_NT + threadIdx.x;
+    if (u >= 4 * n_states) return;
+    const PlaneState<T> ps = states[u >> 2];
+    if (ps.line < 0) return;
+    const int64_t i = ps.line;
+    Line3<T> L;
+    if (line3_setup(lines + 7 * i, L) != 1) return;
+    Iter3<T> it;
+    it3_from(L, ps, it);
+    const int64_t f = idxs ? (int64_t)idxs[i] : ((v.n_frames == 1) ? 0 : i);
+    T *fo = out + f * v.D * v.H * v.W;
+    auto emit = [&](const int64_t *p, T err) {
+        if (!(err <= T(1)) || p[0] < 0 || p[0] >= v.W || p[1] < 0 || p[1] >= v.H || p[2] < 0 || p[2] >= v.D) return;
+        apply_atomic<OV>(fo + (p[2] * v.H + p[1]) * v.W + p[0], pixel_value<K>(err, max_val));
+    };
+    const int quad = (int)(u & 3);
+    if (L.axis == 0) fill_plane<0>(L, it, quad, flags, emit);
+    else if (L.axis == 1) fill_plane<1>(L, it, quad, flags, emit);
+    else fill_plane<2>(L, it, quad, flags, emit);
 }
 
 // accumulate_lines, mixed overlaps: the lines of terms [t0, t0 + nb) drawn
@@ -459,7 +602,25 @@ static void launch_draw3(int kernel, T *out, const T *lines, int64_t n, const I 
     }
 }
 
-// validation, the draw, then its step-budget flag (inside an open Workspace)
+template <typename T, typename I, int OV>
+static void launch_fill3(int kernel, T *out, const T *lines, const PlaneState<T> *states, int64_t n_states,
+                         int64_t n_lines, const I *idxs, const Vol3 &v, T mv, int *flags, hipStream_t st)
+{
+    const unsigned g = (unsigned)grid_for(4 * n_states, D3_NT);   // four lanes per plane
+#define CBX_FILL3(KK) k_fill3<T, I, KK, OV><<<g, D3_NT, 0, st>>>(out, lines, states, n_states, n_lines, idxs, v, mv, flags)
+    switch (kernel) {
+    case K_BIWEIGHT: CBX_FILL3(K_BIWEIGHT); break;
+    case K_GAUSSIAN: CBX_FILL3(K_GAUSSIAN); break;
+    case K_PARABOLIC: CBX_FILL3(K_PARABOLIC); break;
+    case K_RECTANGULAR: CBX_FILL3(K_RECTANGULAR); break;
+    default: CBX_FILL3(K_TRIANGULAR); break;
+    }
+#undef CBX_FILL3
+}
+
+// validation, the draw, then its step-budget flag (inside an open Workspace).
+// Above MAX_PLANES recorded planes the single-kernel walk (four lanes per
+// line) runs instead of the two phases.
 template <typename T, typename I>
 static int draw3_run(T *out, const Vol3 &v, const T *lines, int64_t n_lines, const I *idxs, T max_val, int kernel,
                      int overlap, hipStream_t st)
@@ -468,10 +629,40 @@ static int draw3_run(T *out, const Vol3 &v, const T *lines, int64_t n_lines, con
     CBX_HIP(flags.alloc(1, st));
     int rc = validate3(lines, n_lines, idxs, v, flags.ptr, st);
     if (rc) return rc;
+    // planes of every line: offsets by a scan of their bounds
+    DevBuf<int64_t> ub, off;
+    CBX_HIP(ub.alloc((size_t)n_lines + 1, st));
+    CBX_HIP(off.alloc((size_t)n_lines + 1, st));
+    k_plane_bound<T><<<grid_for(n_lines + 1, D3_NT), D3_NT, 0, st>>>(lines, n_lines, ub.ptr);
+    CBX_HIP(hipGetLastError());
+    CBX_HIP(exclusive_scan_i64(ub.ptr, off.ptr, n_lines + 1, st));
+    uint64_t total = 0;
+    {
+        const void *src[1] = {off.ptr + n_lines};
+        const int nb[1] = {8};
+        CBX_HIP(read_device_values(st, 1, src, nb, &total));
+    }
+    DevBuf<PlaneState<T>> states;
+    const bool two_phase = total <= (uint64_t)MAX_PLANES;
+    if (two_phase && total > 0) CBX_HIP(states.alloc((size_t)total, st));
     {
         KernelTimer timer("draw3", st);
-        if (overlap == 0) launch_draw3<T, I, 0>(kernel, out, lines, n_lines, idxs, v, max_val, flags.ptr, st);
-        else launch_draw3<T, I, 1>(kernel, out, lines, n_lines, idxs, v, max_val, flags.ptr, st);
+        if (two_phase) {
+            if (total > 0) {
+                k_walk3_main<T><<<grid_for(n_lines, D3_NT), D3_NT, 0, st>>>(lines, n_lines, off.ptr, states.ptr,
+                                                                           flags.ptr);
+                if (overlap == 0)
+                    launch_fill3<T, I, 0>(kernel, out, lines, states.ptr, (int64_t)total, n_lines, idxs, v, max_val,
+                                          flags.ptr, st);
+                else
+                    launch_fill3<T, I, 1>(kernel, out, lines, states.ptr, (int64_t)total, n_lines, idxs, v, max_val,
+                                          flags.ptr, st);
+            }
+        } else if (overlap == 0) {
+            launch_draw3<T, I, 0>(kernel, out, lines, n_lines, idxs, v, max_val, flags.ptr, st);
+        } else {
+            launch_draw3<T, I, 1>(kernel, out, lines, n_lines, idxs, v, max_val, flags.ptr, st);
+        }
         CBX_HIP(hipGetLastError());
     }
     int hv = 0;
