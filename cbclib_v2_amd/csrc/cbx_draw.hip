@@ -58,6 +58,7 @@ struct Geometry {
     int32_t H, W;
     int32_t tile, seg;
     int32_t ntx, nty;
+    int32_t skip_neg = 0;   // 1: a line with a negative frame draws nothing (accumulate_lines term batches)
 };
 
 // ---------------------------------------------------------------------------
@@ -78,7 +79,10 @@ __device__ __forceinline__ bool setup_line(const T *__restrict__ lines, const I 
         L.valid = 0;
     }
     int64_t f = idxs ? (int64_t)idxs[l] : ((g.n_frames == 1) ? 0 : l);
-    if (f < 0 || f >= g.n_frames) {
+    if (f < 0 && g.skip_neg) {
+        L.valid = 0;
+        f = 0;
+    } else if (f < 0 || f >= g.n_frames) {
         atomicOr(flags, FLAG_INDEX);
         L.valid = 0;
     }
@@ -1645,9 +1649,11 @@ static int flags_to_status(int flags)
 // Builds segments (and, with bin, the per-tile item lists).  One host sync.
 template <typename T, typename I>
 static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int kernel, int64_t n_lines,
-                      int64_t n_frames, int64_t H, int64_t W, int tile, int seg, bool bin, hipStream_t st)
+                      int64_t n_frames, int64_t H, int64_t W, int tile, int seg, bool bin, hipStream_t st,
+                      bool skip_neg = false)
 {
     Geometry &g = P.g;
+    g.skip_neg = skip_neg ? 1 : 0;
     g.n_frames = n_frames;
     g.H = (int32_t)H;
     g.W = (int32_t)W;
@@ -2175,6 +2181,25 @@ static void launch_accumulate(int kernel, T *out, const SegmentPlan<T> &P, T max
 // term buffer, and its touched pixels are combined into frame frames[term]
 // with atomics (bresenham.cpp:230-248).  Terms reach a frame in no fixed
 // order, as under the reference's OpenMP loop over terms.
+// (term, tile) pairs of one mixed-overlap accumulate plan at most
+constexpr int64_t MAX_TERM_TILES = int64_t(1) << 24;
+
+// term of each line within the batch [t0, t1), or -1 (a line of another batch)
+template <typename I>
+__global__ void k_term_batch(const I *__restrict__ terms, int64_t n_lines, int64_t t0, int64_t t1,
+                             I *__restrict__ out)
+{
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n_lines) return;
+    const int64_t t = (int64_t)terms[i];
+    out[i] = (t >= t0 && t < t1) ? (I)(t - t0) : (I)(-1);
+}
+
+template <typename T, typename I>
+static int accumulate_batch(T *out, const T *lines, int64_t n_lines, const I *bidx, int64_t n_terms, bool skip,
+                            const int32_t *frames32, int64_t H, int64_t W, T max_val, int kernel, int in_overlap,
+                            hipStream_t st);
+
 template <typename T, typename I>
 static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const T *lines, int64_t n_lines,
                                  const I *terms, const I *frames, int64_t n_terms, T max_val, int kernel,
@@ -2218,8 +2243,41 @@ static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W,
                                            (in_overlap == 1 && max_val < T(0)) ? T(0) : max_val, kernel,
                                            in_overlap, opts, stream);
 
+    // The plan's frames are terms: (term, tile) lists scale with n_terms.
+    // Above MAX_TERM_TILES pairs the terms go in batches, each with a plan of
+    // its own (the lines of other terms skipped), in a nested workspace whose
+    // memory the next batch reuses.
+    const int64_t per_frame = ((H + 31) / 32) * ((W + 31) / 32);
+    const int64_t batch = std::max<int64_t>(1, std::min<int64_t>(n_terms, MAX_TERM_TILES / per_frame));
+    DevBuf<I> bterms;
+    if (batch < n_terms) CBX_HIP(bterms.alloc(n_lines, st));
+    for (int64_t t0 = 0; t0 < n_terms; t0 += batch) {
+        const int64_t t1 = std::min(n_terms, t0 + batch);
+        Workspace inner(st);
+        const I *bidx = terms;
+        if (batch < n_terms) {
+            k_term_batch<I><<<grid_for(n_lines, 256), 256, 0, st>>>(terms, n_lines, t0, t1, bterms.ptr);
+            CBX_HIP(hipGetLastError());
+            bidx = bterms.ptr;
+        }
+        if ((rc = accumulate_batch<T, I>(out, lines, n_lines, bidx, t1 - t0, batch < n_terms, frames32.ptr + t0, H,
+                                         W, max_val, kernel, in_overlap, st)))
+            return rc;
+    }
+    return CBX_OK;
+}
+
+// One batch of terms of a mixed-overlap accumulate_lines: plan over the
+// batch's (term, tile) pairs, tile sort, ordered render in accumulate mode.
+// bidx: term of each line within the batch (negative: another batch's line).
+template <typename T, typename I>
+static int accumulate_batch(T *out, const T *lines, int64_t n_lines, const I *bidx, int64_t n_terms, bool skip,
+                            const int32_t *frames32, int64_t H, int64_t W, T max_val, int kernel, int in_overlap,
+                            hipStream_t st)
+{
+    int rc;
     SegmentPlan<T> P;
-    if ((rc = build_plan<T, I>(P, lines, terms, kernel, n_lines, n_terms, H, W, 32, 16, true, st))) return rc;
+    if ((rc = build_plan<T, I>(P, lines, bidx, kernel, n_lines, n_terms, H, W, 32, 16, true, st, skip))) return rc;
     if (P.n_segs == 0) return CBX_OK;
     // (term, tile) pairs are mostly empty: launch over the non-empty ones only
     DevBuf<int> list, n_list;
@@ -2242,8 +2300,8 @@ static int accumulate_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W,
     CBX_HIP(hipGetLastError());
     {
         KernelTimer timer("accumulate_tiles", st);
-        if (in_overlap == 0) launch_accumulate<T, 0, 1>(kernel, out, P, max_val, frames32.ptr, list.ptr, nl, st);
-        else launch_accumulate<T, 1, 0>(kernel, out, P, max_val, frames32.ptr, list.ptr, nl, st);
+        if (in_overlap == 0) launch_accumulate<T, 0, 1>(kernel, out, P, max_val, frames32, list.ptr, nl, st);
+        else launch_accumulate<T, 1, 0>(kernel, out, P, max_val, frames32, list.ptr, nl, st);
     }
     CBX_HIP(hipGetLastError());
     int fl = 0;

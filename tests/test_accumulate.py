@@ -173,6 +173,26 @@ def test_accumulate_many_terms(oracle, cuda, modes, size):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('modes', [('max', 'sum'), ('sum', 'max')])
+def test_accumulate_term_batches(oracle, cuda, modes):
+    """More (term, tile) pairs than one plan holds (40 000 terms over 2 frames
+    of 1024^2: 41 M pairs, three batches of 16 384 terms): the lines of each
+    batch are drawn with the others skipped, and the result equals the
+    oracle's as for one batch."""
+    from cbclib_v2_amd import ndimage
+    rng = np.random.default_rng(21)
+    F, H, W, T = 2, 1024, 1024, 40000
+    n = 60000
+    lines = random_segments(rng, n, H, W, 2.0, 20.0, 0.5, 2.0).astype(np.float32)
+    terms = rng.integers(0, T, n).astype(np.int32)
+    frames = rng.integers(0, F, T).astype(np.int32)
+    ref = oracle.accumulate_lines(np.zeros((F, H, W), np.float32), lines, terms, frames, 1.0, 'parabolic', *modes)
+    got = ndimage.accumulate_lines(np.zeros((F, H, W), np.float32), lines, terms, frames, 1.0, 'parabolic', *modes)
+    assert np.array_equal(got != 0, ref != 0)
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6)
+
+
+@pytest.mark.gpu
 def test_accumulate_errors_and_empty(cuda):
     import torch
     from cbclib_v2_amd import ndimage
