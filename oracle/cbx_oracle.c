@@ -1,9 +1,10 @@
 /*
  * ORACLE — test infrastructure only.
  *
- * CPU restatement of cbclib_v2's streak rasterizer (draw_lines) and of its
- * connected-component labelling (label), written from the reference's
- * behaviour, used as the checker for the HIP product path.  Only tests/,
+ * CPU restatement of cbclib_v2's streak rasterizer (draw_lines and
+ * accumulate_lines, 2D and 3D lines), of its connected-component labelling
+ * (label), region moments, p-values and binary_dilation, written from the
+ * reference's behaviour, used as the checker for the HIP product path.  Only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this
  * library; the product (cbclib_v2_amd) never does.
  *
@@ -52,8 +53,8 @@ void cbxo_plant(int mistake) { g_plant = mistake; }
 /* Profiles: include.hpp:446-468 (kernel_traits).  Float versions reproduce  */
 /* the promotions of the C++ expressions (std::pow(float, int) -> double,    */
 /* std::max<T> -> conversion back to T).                                     */
-/* Pinned by: closed forms (tests/test_oracle_profiles.py) and by the        */
-/* compiled reference through draw_lines images.                             */
+/* Pinned by: closed forms (tests/test_oracle_cpu.py) and by the compiled    */
+/* reference through draw_lines images and the golden fixtures.              */
 /* ------------------------------------------------------------------------ */
 
 static float prof_f32(int k, float x)
@@ -128,8 +129,8 @@ float cbxo_profile_f32(int k, float x) { return prof_f32(k, x); }
 /* Emits every (x, y, err) the reference hands to its draw_pixel callback,   */
 /* in the same order (bresenham.cpp:80-88).                                  */
 /* Pinned by: the compiled reference module (oracle/_ref) on random inputs   */
-/* (tests/test_oracle_vs_ref.py) and the committed golden fixtures; and by   */
-/* a brute-force {dist <= w} footprint away from the boundary.               */
+/* and the committed golden fixtures (tests/test_oracle_cpu.py); and by a    */
+/* brute-force {dist <= w} footprint away from the boundary.                 */
 /* ------------------------------------------------------------------------ */
 
 typedef void (*emit_f32_t)(void *ctx, long x, long y, float err);
@@ -409,7 +410,7 @@ long cbxo_pixel_list_f64(long n_frames, long H, long W, const double *lines, lon
 /* dist(p, segment) <= w, dist from the clamped projection of                */
 /* LineND::project_to_streak (geometry.hpp:319-339); the same formula is the */
 /* reference test's image_numpy (tests/draw_line_test.py:154-167).           */
-/* Pinned by: analytic special cases (tests/test_oracle_closed_form.py).     */
+/* Pinned by: analytic special cases (tests/test_oracle_cpu.py).             */
 /* ------------------------------------------------------------------------ */
 
 static double seg_dist2(double px, double py, double x0, double y0, double x1, double y1)
@@ -461,7 +462,7 @@ int cbxo_render_closed_f64(double *out, long n_frames, long H, long W, const dou
 /* theorem), du/dw = -2u/w.  The pixel set is the float walk's (f32 lines)   */
 /* or the double walk's (f64 lines), so it is the set the forward uses.      */
 /* Pinned by: FP64 central finite differences of cbxo_render_closed_f64 and  */
-/* analytic special cases (tests/test_oracle_adjoint.py).                    */
+/* analytic special cases (tests/test_oracle_cpu.py).                        */
 /* ------------------------------------------------------------------------ */
 
 typedef struct {
@@ -1226,7 +1227,9 @@ int cbxo_p_values_f64(const int32_t *labels, const double *data, int ndim, const
 /* in four quadrants by a 2D walk with single-axis rays.  Same arithmetic   */
 /* contract as the 2D walk (T increments, pow squares in double).           */
 /* Pinned by: the compiled reference bresenham module on 3D inputs (golden  */
-/* fixtures tests/golden/draw3_cases.npz and random inputs).                 */
+/* fixtures tests/golden/draw3_cases.npz and random inputs) and the brute-   */
+/* force distance-to-segment image of the reference's own test               */
+/* (tests/test_draw3.py).                                                    */
 /* ------------------------------------------------------------------------ */
 
 #define W3_BODY(REAL, ROUND, CEIL, FABS, SQRT)                                                          \
@@ -1450,9 +1453,10 @@ int cbxo_accumulate_lines3_f64(double *out, long n_frames, long D, long H, long 
 /* the array), a new element joining only where the mask is true (always   */
 /* without a mask); every element of every region is set.  One breadth-    */
 /* first search per seed, elements stamped with the seed's id.              */
-/* Pinned by: scipy.ndimage.binary_dilation (iterations >= 1, with and      */
-/* without mask), golden fixtures tests/golden/dilation_cases.npz, and the  */
-/* compiled reference label module (num_threads = 1).                       */
+/* Pinned by: scipy.ndimage.binary_dilation applied one masked step per     */
+/* iteration (its own multi-iteration masked mode is another operation),   */
+/* golden fixtures tests/golden/dilation_cases.npz, and the compiled        */
+/* reference label module (num_threads = 1) (tests/test_dilation.py).      */
 /* ------------------------------------------------------------------------ */
 
 static long shift_flat(long index, const int64_t *shift, int ndim, const int64_t *shape)

@@ -16,7 +16,15 @@ The functions mirror the reference call signatures they restate:
 * :func:`label` — ``label.label`` + ``LabelResult.to_array``
   (label.cpp:109-175, 459-493) returned as an int32 map and a count;
 * :func:`render_closed_f64` / :func:`adjoint_f64` — the FP64 closed form and
-  its gradient (no reference counterpart for the gradient).
+  its gradient (no reference counterpart for the gradient);
+* :func:`draw_lines3` / :func:`accumulate_lines3` — the same two on 3D lines
+  (rows of 7, draw_line_3d, bresenham.hpp:389-430);
+* :func:`region_moments` / :func:`p_values` — the per-region functions and
+  p_values of the label module (label.cpp:176-255, 534-583);
+* :func:`binary_dilation` — ``label.binary_dilation`` with ``num_threads=1``
+  (label.cpp:8-107);
+* :func:`draw_lines_omp` / :func:`label_omp` — the reference's multi-threaded
+  CPU procedures restated (the CPU baseline).
 """
 from __future__ import annotations
 
