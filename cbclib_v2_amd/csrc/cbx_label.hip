@@ -526,19 +526,23 @@ constexpr int RUN_ROWS = RUN_NT / 64; // rows per workgroup, one per wave
 
 __device__ __forceinline__ uint64_t le_mask(int b) { return (b >= 63) ? ~0ull : ((2ull << b) - 1ull); }
 
+// sums and exclusive scans over groups of WIDTH lanes (64: the wave; 32:
+// each half-wave on its own row); lane = the lane's index in its group
+template <int WIDTH = 64>
 __device__ __forceinline__ int wave_sum(int v)
 {
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    for (int o = WIDTH / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WIDTH);
     return v;
 }
 
+template <int WIDTH = 64>
 __device__ __forceinline__ int wave_excl_scan(int v, int lane)
 {
     int x = v;
 #pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const int y = __shfl_up(x, d, 64);
+    for (int d = 1; d < WIDTH; d <<= 1) {
+        const int y = __shfl_up(x, d, WIDTH);
         if (lane >= d) x += y;
     }
     return x - v;
@@ -659,14 +663,14 @@ __device__ __forceinline__ RowChunk load_chunk(const uint64_t *__restrict__ cur,
 
 // Contacts of one chunk (see run_contacts); ccarry / pcarry count the runs of
 // rows y and y-1 in the chunks before it.
-template <typename Join>
+template <int WIDTH, typename Join>
 __device__ __forceinline__ void chunk_contacts(const RunGeom &g, const RowChunk &r, int32_t cbase, int32_t pbase,
                                                int &ccarry, int &pcarry, int lane, Join join)
 {
     const uint64_t cw = r.cw, pw = r.pw, pwl = r.pwl, pwr = r.pwr;
     const uint64_t cst = run_starts(cw, r.cwl), pst = run_starts(pw, pwl);
     const int cn = __popcll(cst), pn = __popcll(pst);
-    const int cpre = ccarry + wave_excl_scan(cn, lane), ppre = pcarry + wave_excl_scan(pn, lane);
+    const int cpre = ccarry + wave_excl_scan<WIDTH>(cn, lane), ppre = pcarry + wave_excl_scan<WIDTH>(pn, lane);
 #pragma unroll
     for (int d = -1; d <= 1; d++) {
         if (!(g.prev & (d < 0 ? 2 : (d == 0 ? 4 : 8)))) continue;
@@ -695,11 +699,13 @@ __device__ __forceinline__ void chunk_contacts(const RunGeom &g, const RowChunk 
             join(a, pbase + nb - 1);
         }
     }
-    ccarry += wave_sum(cn);
-    pcarry += wave_sum(pn);
+    ccarry += wave_sum<WIDTH>(cn);
+    pcarry += wave_sum<WIDTH>(pn);
 }
 
-template <typename Join>
+// Contacts of row `row` with the row above, over a group of WIDTH lanes
+// (lane: index in the group).
+template <int WIDTH = 64, typename Join>
 __device__ __forceinline__ void run_contacts(const RunGeom &g, const uint64_t *__restrict__ bits,
                                              const int32_t *__restrict__ roff, int64_t row, int lane, Join join)
 {
@@ -707,8 +713,8 @@ __device__ __forceinline__ void run_contacts(const RunGeom &g, const uint64_t *_
     const uint64_t *cur = bits + row * nw, *up = cur - nw;
     const int32_t cbase = roff[row], pbase = roff[row - 1];
     int ccarry = 0, pcarry = 0;
-    for (int c0 = 0; c0 < nw; c0 += 64)
-        chunk_contacts(g, load_chunk(cur, up, nw, c0 + lane), cbase, pbase, ccarry, pcarry, lane, join);
+    for (int c0 = 0; c0 < nw; c0 += WIDTH)
+        chunk_contacts<WIDTH>(g, load_chunk(cur, up, nw, c0 + lane), cbase, pbase, ccarry, pcarry, lane, join);
 }
 
 // One workgroup per band of RUN_BAND rows of one frame.  The band's runs are
@@ -721,11 +727,14 @@ __device__ __forceinline__ void run_contacts(const RunGeom &g, const uint64_t *_
 constexpr int RUN_BAND = 32;
 constexpr int RUN_LCAP = 4096;
 
+// WIDTH 32 (rows of at most 32 words): each half-wave takes a row of its own.
+template <int WIDTH>
 __global__ void __launch_bounds__(RUN_NT)
 k_run_band(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff, int32_t *par)
 {
     __shared__ int32_t lpar[RUN_LCAP];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int GROUPS = RUN_NT / WIDTH;   // rows in flight per workgroup
+    const int lane = threadIdx.x % WIDTH, grp = threadIdx.x / WIDTH;
     const int bpf = (g.H + RUN_BAND - 1) / RUN_BAND;
     const int64_t f = blockIdx.x / bpf;
     const int y0 = (int)(blockIdx.x - f * bpf) * RUN_BAND;
@@ -735,8 +744,9 @@ k_run_band(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restri
     if (nb <= RUN_LCAP) {
         for (int k = threadIdx.x; k < nb; k += RUN_NT) lpar[k] = k;
         __syncthreads();
-        for (int y = 1 + wave; y < rows; y += RUN_ROWS)
-            run_contacts(g, bits, roff, row0 + y, lane, [&](int32_t a, int32_t b) { lds_union(lpar, a - r0, b - r0); });
+        for (int y = 1 + grp; y < rows; y += GROUPS)
+            run_contacts<WIDTH>(g, bits, roff, row0 + y, lane,
+                                [&](int32_t a, int32_t b) { lds_union(lpar, a - r0, b - r0); });
         __syncthreads();
         for (int k = threadIdx.x; k < nb; k += RUN_NT) par[r0 + k] = r0 + lds_find(lpar, k);
     } else {
@@ -744,24 +754,25 @@ k_run_band(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restri
             __hip_atomic_store(&par[r0 + k], r0 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         __threadfence();
         __syncthreads();
-        for (int y = 1 + wave; y < rows; y += RUN_ROWS)
-            run_contacts(g, bits, roff, row0 + y, lane, [&](int32_t a, int32_t b) { run_union(par, a, b); });
+        for (int y = 1 + grp; y < rows; y += GROUPS)
+            run_contacts<WIDTH>(g, bits, roff, row0 + y, lane, [&](int32_t a, int32_t b) { run_union(par, a, b); });
     }
 }
 
 // Contacts between the first row of a band and the row above it (same frame).
+template <int WIDTH>
 __global__ void __launch_bounds__(RUN_NT)
 k_run_border(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__restrict__ roff, int64_t n_bands,
              int32_t *par)
 {
-    const int lane = threadIdx.x & 63;
-    const int64_t band = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    const int lane = threadIdx.x % WIDTH;
+    const int64_t band = (int64_t)blockIdx.x * (RUN_NT / WIDTH) + threadIdx.x / WIDTH;
     if (band >= n_bands) return;
     const int bpf = (g.H + RUN_BAND - 1) / RUN_BAND;
     const int64_t f = band / bpf;
     const int y0 = (int)(band - f * bpf) * RUN_BAND;
     if (y0 == 0) return;
-    run_contacts(g, bits, roff, f * g.H + y0, lane, [&](int32_t a, int32_t b) { run_union(par, a, b); });
+    run_contacts<WIDTH>(g, bits, roff, f * g.H + y0, lane, [&](int32_t a, int32_t b) { run_union(par, a, b); });
 }
 
 template <bool COUNT>
@@ -955,9 +966,16 @@ static int label_runs(In in, int ndim, const int64_t *shape, int prev, int64_t n
     if (sizes) k_run_init<true><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, size.ptr);
     if (prev & 14) {
         const int64_t n_bands = (int64_t)(R / g.H) * ((g.H + RUN_BAND - 1) / RUN_BAND);
-        k_run_band<<<(unsigned)n_bands, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr);
-        k_run_border<<<(unsigned)((n_bands + RUN_ROWS - 1) / RUN_ROWS), RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr,
-                                                                                         n_bands, par.ptr);
+        if (g.nw <= 32) {
+            // rows of at most 2048 pixels: one row per half-wave
+            k_run_band<32><<<(unsigned)n_bands, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr);
+            k_run_border<32><<<(unsigned)((n_bands + 2 * RUN_ROWS - 1) / (2 * RUN_ROWS)), RUN_NT, 0, st>>>(
+                g, bits.ptr, roff.ptr, n_bands, par.ptr);
+        } else {
+            k_run_band<64><<<(unsigned)n_bands, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr);
+            k_run_border<64><<<(unsigned)((n_bands + RUN_ROWS - 1) / RUN_ROWS), RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr,
+                                                                                                 n_bands, par.ptr);
+        }
     } else if (!sizes) {
         k_run_init<false><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, nullptr);
     }
