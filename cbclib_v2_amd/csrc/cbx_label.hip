@@ -775,6 +775,39 @@ k_run_border(RunGeom g, const uint64_t *__restrict__ bits, const int32_t *__rest
     run_contacts<WIDTH>(g, bits, roff, f * g.H + y0, lane, [&](int32_t a, int32_t b) { run_union(par, a, b); });
 }
 
+// par[r] = root of run r, one lane per run (the root walks of a row's few
+// runs would leave most lanes of a row's wave idle); a grid-stride loop over
+// the *n_runs runs, so the launch needs no host read of the count
+__global__ void __launch_bounds__(RUN_NT)
+k_run_resolve_flat(const int32_t *__restrict__ n_runs, int32_t *par)
+{
+    const int64_t n = *n_runs;
+    for (int64_t r = (int64_t)blockIdx.x * RUN_NT + threadIdx.x; r < n; r += (int64_t)gridDim.x * RUN_NT) {
+        // read-only walk: a halving store here could replace another run's
+        // freshly written root by a mere ancestor
+        int32_t root = (int32_t)r, p = __hip_atomic_load(&par[r], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (p != root) {
+            root = p;
+            p = __hip_atomic_load(&par[root], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (root != (int32_t)r) __hip_atomic_store(&par[r], root, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// roots per row (after the resolve: a run is a root iff par[r] == r)
+__global__ void __launch_bounds__(RUN_NT)
+k_run_roots(RunGeom g, const int32_t *__restrict__ roff, const int32_t *__restrict__ par, int32_t *__restrict__ kcnt)
+{
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * RUN_ROWS + (threadIdx.x >> 6);
+    if (row >= g.rows) return;
+    const int32_t off = roff[row], n = roff[row + 1] - off;
+    int roots = 0;
+    for (int k = lane; k < n; k += 64) roots += par[off + k] == off + k;
+    roots = wave_sum(roots);
+    if (lane == 0) kcnt[row] = roots;
+}
+
 template <bool COUNT>
 __global__ void __launch_bounds__(RUN_NT)
 k_run_resolve(RunGeom g, const int32_t *__restrict__ roff, int32_t *par, int32_t *__restrict__ kcnt)
@@ -984,7 +1017,10 @@ static int label_runs(In in, int ndim, const int64_t *shape, int prev, int64_t n
         k_run_sizes<<<grid, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr, size.ptr);
         k_run_count<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, size.ptr, npts, kcnt.ptr);
     } else {
-        k_run_resolve<true><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, kcnt.ptr);
+        // resolve one lane per run (the run total is the scan's last entry),
+        // 8 workgroups per CU of 256
+        k_run_resolve_flat<<<2048, RUN_NT, 0, st>>>(roff.ptr + R, par.ptr);
+        k_run_roots<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, kcnt.ptr);
     }
     CBX_HIP(exclusive_scan_i32(kcnt.ptr, koff.ptr, R + 1, st));
     k_run_rank<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, koff.ptr, par.ptr, sizes ? size.ptr : nullptr, npts);
