@@ -142,9 +142,8 @@ def test_label_structures(path, oracle, upper):
 
 
 def test_label_large_binary(path, oracle):
-    """A C2-sized frame pair (2 x 2048 x 2048, long runs, mostly background)."""
-    if path == 2:
-        pytest.skip("element union-find on 8 Mpx is covered by the smaller cases")
+    """A C2-sized frame pair (2 x 2048 x 2048, long runs, mostly background),
+    under every label algorithm (path 2: the element union-find on 8 Mpx)."""
     rng = np.random.default_rng(12)
     img = np.zeros((2, 2048, 2048), np.float32)
     for f in range(2):
