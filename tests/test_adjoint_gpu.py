@@ -1,5 +1,8 @@
 """Adjoint of draw_lines on the GPU against the oracle's FP64 gradient over
-the same pixel set, and against FP64 central finite differences (h = 1e-3)."""
+the same pixel set (rtol 1e-5, atol 1e-7 of the gradient scale), and against
+FP64 central finite differences (h = 1e-3) of the closed-form render: that
+render and the walk's f32 render differ on boundary pixels, hence rtol 2e-2
+there (the C4 test checks the FD of the closed-form residual at 1e-5 scale)."""
 import numpy as np
 import pytest
 import torch
@@ -25,7 +28,11 @@ def test_adjoint_vs_oracle(cuda, oracle, kernel):
                            1.3, kernel).cpu().numpy()
     ref = oracle.adjoint_f64(g.astype(np.float64), lines, idxs, 1.3, kernel, walk='f32')
     scale = np.abs(ref).max()
-    np.testing.assert_allclose(got, ref, rtol=1e-4, atol=1e-5 * scale)
+    big = np.abs(ref) > 1e-3 * scale
+    print(f"{kernel}: max-rel {float((np.abs(got - ref)[big] / np.abs(ref[big])).max()):.3e} over components "
+          f"above 1e-3 of the scale", flush=True)
+    # the bar of the north star (1e-5 relative), with atol 1e-7 of the scale for cancelled sums
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-7 * scale)
 
 
 def test_adjoint_rectangular_is_zero(cuda):
