@@ -35,11 +35,12 @@ MOMENTS = {'total_mass': 0, 'mean': 1, 'center_of_mass': 2, 'moment_of_inertia':
 
 class DrawOpts(ctypes.Structure):
     """``cbx_draw_opts``: tile side, columns per walk checkpoint, deterministic
-    flag, tile-kernel workgroup size, exact-value flag, items per phase, and
-    per-line profile ids (device pointer or None)."""
+    flag, tile-kernel workgroup size, exact-value flag, items per phase,
+    per-line profile ids (device pointer or None) and the event `out` waits
+    for (hipEvent_t or None)."""
     _fields_ = [('tile', ctypes.c_int32), ('seg', ctypes.c_int32), ('deterministic', ctypes.c_int32),
                 ('threads', ctypes.c_int32), ('exact_values', ctypes.c_int32), ('chunk', ctypes.c_int32),
-                ('kernels', ctypes.c_void_p)]
+                ('kernels', ctypes.c_void_p), ('out_ready', ctypes.c_void_p)]
 
 
 class CapacityError(RuntimeError):

@@ -2079,14 +2079,19 @@ static int draw_lines_impl(T *out, int64_t n_frames, int64_t H, int64_t W, const
         seg = 16;
     }
     SegmentPlan<T> P;
+    // `out` may become ready on another stream: wait for it only before the
+    // first kernel that touches it
+    const hipEvent_t out_ready = opts ? (hipEvent_t)opts->out_ready : nullptr;
     if (det) {
         if ((rc = build_plan<T, I>(P, lines, idxs, kernel, n_lines, n_frames, H, W, tile, seg, true, st))) return rc;
+        if (out_ready) CBX_HIP(hipStreamWaitEvent(st, out_ready, 0));
         if (P.n_segs == 0) return CBX_OK;
         return ordered_pipeline<T>(P, kernel, overlap, max_val, out, nullptr, nullptr, nullptr, nullptr, nullptr, 0,
                                    nullptr, st);
     }
     if ((rc = build_plan_unordered<T, I>(P, lines, idxs, kernels, kernel, n_lines, n_frames, H, W, tile, seg, st)))
         return rc;
+    if (out_ready) CBX_HIP(hipStreamWaitEvent(st, out_ready, 0));
     if (P.n_segs == 0) return CBX_OK;
     {
         KernelTimer timer("draw_tiles", st);

@@ -72,24 +72,41 @@ def _check_idxs(idxs: Any, n_lines: int, n_frames: int) -> None:
         raise ValueError(f"idxs has an invalid size ({n} != {n_lines})")
 
 
+def _event_handle(ev: Any) -> Optional[int]:
+    """hipEvent_t of a torch.cuda.Event (or a raw handle), None for None."""
+    if ev is None:
+        return None
+    if isinstance(ev, int):
+        return ev
+    handle = getattr(ev, 'cuda_event', None)
+    if not handle:
+        raise ValueError("out_ready must be a recorded torch.cuda.Event or a hipEvent_t handle")
+    return int(handle)
+
+
 def _opts(tile: Optional[int], seg: Optional[int], deterministic: bool, threads: Optional[int] = None,
-          exact_values: bool = False, chunk: Optional[int] = None, kernels_ptr: Optional[int] = None):
+          exact_values: bool = False, chunk: Optional[int] = None, kernels_ptr: Optional[int] = None,
+          out_ready: Any = None):
     return _native.DrawOpts(int(tile or 0), int(seg or 0), 1 if deterministic else 0, int(threads or 0),
-                            1 if exact_values else 0, int(chunk or 0), kernels_ptr)
+                            1 if exact_values else 0, int(chunk or 0), kernels_ptr, _event_handle(out_ready))
 
 
 def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, kernel: str = 'rectangular',
                overlap: str = 'sum', *, tile: Optional[int] = None, seg: Optional[int] = None,
                deterministic: bool = False, threads: Optional[int] = None, exact_values: bool = False,
-               chunk: Optional[int] = None, kernels: Any = None):
+               chunk: Optional[int] = None, kernels: Any = None, out_ready: Any = None):
     """Draw thick lines with a profile into ``out`` (in place) and return ``out``.
 
     Extra keyword-only arguments (no reference counterpart): ``tile`` (32 or
     64, pixel tile per workgroup), ``seg`` (8 or 16, walk checkpoint spacing),
     ``threads`` (64, 128 or 256, tile workgroup size), ``deterministic``
     (reference summation order: bit-exact images), ``exact_values``
-    (reference value arithmetic in the tile path) and ``kernels`` (one profile
-    per line, names or ids; ``kernel`` is then ignored; tile path only).
+    (reference value arithmetic in the tile path), ``kernels`` (one profile
+    per line, names or ids; ``kernel`` is then ignored; tile path only) and
+    ``out_ready`` (a torch.cuda.Event recorded where ``out`` becomes ready on
+    another stream: the draw waits for it only before it touches ``out``, so
+    its line setup, walk and binning overlap that work; 2D lines, GPU
+    tensors).
     """
     device.require_rocm('draw_lines')
     kid = _native.kernel_id(kernel)
@@ -122,7 +139,7 @@ def draw_lines(out: Any, lines: Any, idxs: Any = None, max_val: float = 1.0, ker
         t_kern = to_device(_native.kernel_ids(kernels), torch.uint8).reshape(-1)
         if t_kern.numel() != n_lines:
             raise ValueError(f"kernels has an invalid size ({t_kern.numel()} != {n_lines})")
-    opts = _opts(tile, seg, deterministic, threads, exact_values, chunk, ptr(t_kern))
+    opts = _opts(tile, seg, deterministic, threads, exact_values, chunk, ptr(t_kern), out_ready)
     _native.check(fn(t_out.data_ptr(), n_frames, H, W, t_lines.data_ptr(), n_lines, ptr(t_idxs), float(max_val),
                      kid, oid, ctypes.byref(opts), stream_ptr()))
     return finish()

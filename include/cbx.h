@@ -83,6 +83,10 @@ typedef struct cbx_draw_opts {
                                drawn with its own profile (tile path only; the `kernel` argument is then
                                ignored).  No reference counterpart: the reference takes one profile per
                                call, so mixed-profile frames need one call per profile there. */
+    void *out_ready;        /* NULL, or a hipEvent_t recorded where `out` becomes ready (e.g. a fill on
+                               another stream): draw_lines waits for it just before its first access to
+                               `out`, so the line setup, walk and binning overlap that work.  `lines` and
+                               `idxs` must be ready on `stream` as usual.  No reference counterpart. */
 } cbx_draw_opts;
 
 /*

@@ -243,3 +243,30 @@ def test_repeatable(cuda):
     a = gpu_image(shape, lines, idxs, 'gaussian', 'max', np.float32)
     b = gpu_image(shape, lines, idxs, 'gaussian', 'max', np.float32)
     assert np.array_equal(a, b)
+
+
+def test_out_ready_event(cuda, oracle):
+    """`out` filled on a second stream (slowly: several passes over 256 MiB
+    first), the draw told to wait for it with out_ready: the image is the
+    fill plus the render, as if the two ran one after the other."""
+    shape, lines, idxs = case(11)
+    F, H, W = shape
+    ref = oracle.draw_lines(np.full(shape, 0.25, np.float32), lines, idxs, 1.7, 'gaussian', 'sum')
+    main = torch.cuda.current_stream()
+    side = torch.cuda.Stream()
+    big = torch.empty(64 * 1024 * 1024, device='cuda')
+    t_lines, t_idxs = torch.from_numpy(lines).cuda(), torch.from_numpy(idxs).cuda()
+    for rep in range(3):
+        out = torch.zeros(shape, dtype=torch.float32, device='cuda')
+        ready = torch.cuda.Event()
+        side.wait_stream(main)
+        with torch.cuda.stream(side):
+            for _ in range(4):
+                big.fill_(float(rep))
+            out.fill_(0.25)
+            ready.record(side)
+        ndimage.draw_lines(out, t_lines, t_idxs, 1.7, 'gaussian', 'sum', out_ready=ready)
+        got = out.cpu().numpy()
+        assert np.array_equal(got != 0.25, ref != 0.25), f"rep {rep}: support"
+        np.testing.assert_allclose(got, ref, rtol=RTOL, atol=ATOL * 1.7, err_msg=f"rep {rep}")
+    torch.cuda.synchronize()
