@@ -9,10 +9,11 @@
 // One pass over the label map (k_region_accumulate): each lane takes a chunk
 // of MOM_CHUNK pixels of one row, folds consecutive pixels of equal label
 // into run sums (sum v, sum v x, sum v x^2 along the row) and, per run,
-// adds the region's raw moments in double (one atomic per component: the
-// other coordinates are constant along a row) and lowers the region's
-// origin (its smallest flat index, integer atomicMin).  Background chunks
-// end after the label load.  k_region_finish then turns the raw sums into the
+// adds the region's raw moments in double (the other coordinates are
+// constant along a row) and lowers the region's origin (its smallest flat
+// index, integer atomicMin) -- into an LDS table of its workgroup's patch
+// of rows, which then adds each region's sums to the global accumulators
+// once.  Background chunks end after the label load.  k_region_finish then turns the raw sums into the
 // reference's quantities, in double, written in the data's type.
 //
 // The reference accumulates in the data's type relative to the region's
@@ -45,6 +46,7 @@ struct MomGeom {
     int32_t vec;                // rows start 16-byte aligned in the label map
     int32_t dvec;               // and in the data
     int32_t small;              // rows * chunks < 2^31: 32-bit index arithmetic
+    int64_t n_pc;               // patch columns of the moment pass (chunks / MOM_PC, rounded up)
 };
 
 // accumulators per slot, xyz order (x = last axis):
@@ -184,75 +186,139 @@ __device__ __forceinline__ void row_coords(const MomGeom &g, int64_t row, int64_
     c[0] = 0;
 }
 
+// Patches of MOM_PR rows x (MOM_NT / MOM_PR) chunks: a workgroup sums the
+// runs of its patch per region in an LDS table (MOM_TBL entries, open
+// addressing on the slot) and adds each region's sums to the global
+// accumulators once, so a region that spans several rows of the patch costs
+// one set of global atomics instead of one per run (the global double
+// atomics took three quarters of the pass).  A run whose region finds no
+// table entry within MOM_PROBE probes goes straight to the global sums.
+constexpr int MOM_PR = 8;
+constexpr int MOM_PC = MOM_NT / MOM_PR;
+constexpr int MOM_TBL = 128;
+constexpr int MOM_PROBE = 16;
+
+__device__ __forceinline__ int mom_entry(int32_t *tkey, int32_t s)
+{
+    const uint32_t h = ((uint32_t)s * 2654435761u) >> 25;   // 7 bits: MOM_TBL
+    for (int p = 0; p < MOM_PROBE; p++) {
+        const int e = (int)((h + p) & (MOM_TBL - 1));
+        const int32_t k = atomicCAS(&tkey[e], -1, s);
+        if (k == -1 || k == s) return e;
+    }
+    return -1;
+}
+
 template <int N, typename T>
 __global__ void __launch_bounds__(MOM_NT)
 k_region_accumulate(const int32_t *__restrict__ labels, const T *__restrict__ data, MomGeom g,
                     const int32_t *__restrict__ slot, double *__restrict__ acc, unsigned long long *__restrict__ origin)
 {
     using Lay = MomLayout<N>;
-    const int64_t t = (int64_t)blockIdx.x * MOM_NT + threadIdx.x;
-    if (t >= g.rows * g.chunks) return;
-    int64_t row, x0;
-    chunk_of(g, t, row, x0);
-    const int n = (int)min((int64_t)MOM_CHUNK, g.W - x0);
-    const int64_t base = row * g.W;
-    int32_t lab[MOM_CHUNK];
-    load_chunk(labels, g, base, x0, n, lab);
-    bool any = false;
-#pragma unroll
-    for (int i = 0; i < MOM_CHUNK; i++) any |= lab[i] > 0;
-    if (!any) return;
-    T dv[MOM_CHUNK];
-    load_data(data, g, base, x0, n, dv);
-    int64_t c[N];
-    row_coords<N>(g, row, c);
+    constexpr int S = Lay::STRIDE;
+    __shared__ int32_t tkey[MOM_TBL];
+    __shared__ unsigned long long torg[MOM_TBL];
+    __shared__ double tval[MOM_TBL * S];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < MOM_TBL; e += MOM_NT) {
+        tkey[e] = -1;
+        torg[e] = ~0ull;
+    }
+    for (int i = tid; i < MOM_TBL * S; i += MOM_NT) tval[i] = 0.0;
+    __syncthreads();
 
-    int32_t cur = 0;
-    int64_t start = 0;
-    double s0 = 0.0, sx = 0.0, sxx = 0.0;
-    auto flush = [&]() {
-        if (cur <= 0 || cur > g.maxl) return;
-        const int32_t s = slot[cur];
-        if (s == NO_SLOT) return;
-        atomicMin(&origin[s], (unsigned long long)(base + start));
-        if (s0 == 0.0) return;   // every value <= 0 (NaN sums go on)
-        double *a = acc + (int64_t)s * Lay::STRIDE;
-        atomicAdd(&a[0], s0);
-        atomicAdd(&a[1], sx);
-        atomicAdd(&a[1 + N], sxx);
+    // this lane's chunk: patch (pr, pc), row pr * MOM_PR + tid / MOM_PC
+    const int64_t pr = (int64_t)blockIdx.x / g.n_pc, pc = (int64_t)blockIdx.x - pr * g.n_pc;
+    const int64_t row = pr * MOM_PR + tid / MOM_PC, ch = pc * MOM_PC + tid % MOM_PC;
+    bool any = false;
+    int32_t lab[MOM_CHUNK];
+    T dv[MOM_CHUNK];
+    int64_t x0 = 0, base = 0;
+    int n = 0;
+    if (row < g.rows && ch < g.chunks) {
+        x0 = ch * MOM_CHUNK;
+        n = (int)min((int64_t)MOM_CHUNK, g.W - x0);
+        base = row * g.W;
+        load_chunk(labels, g, base, x0, n, lab);
 #pragma unroll
-        for (int m = 1; m < N; m++) {
-            const double cm = (double)c[m];
-            atomicAdd(&a[1 + m], cm * s0);
-            atomicAdd(&a[1 + N + m], cm * cm * s0);
-        }
+        for (int i = 0; i < MOM_CHUNK; i++) any |= lab[i] > 0;
+    }
+    if (any) {
+        load_data(data, g, base, x0, n, dv);
+        int64_t c[N];
+        row_coords<N>(g, row, c);
+        int32_t cur = 0;
+        int64_t start = 0;
+        double s0 = 0.0, sx = 0.0, sxx = 0.0;
+        auto flush = [&]() {
+            if (cur <= 0 || cur > g.maxl) return;
+            const int32_t sl = slot[cur];
+            if (sl == NO_SLOT) return;
+            // the run's contributions to the region's components
+            double v[S];
+            v[0] = s0;
+            v[1] = sx;
+            v[1 + N] = sxx;
 #pragma unroll
-        for (int p = 0; p < Lay::NP; p++) {
-            int i, j;
-            pair_of<N>(p, i, j);
-            atomicAdd(&a[1 + 2 * N + p], (i == 0) ? (double)c[j] * sx : (double)c[i] * (double)c[j] * s0);
-        }
-    };
+            for (int m = 1; m < N; m++) {
+                const double cm = (double)c[m];
+                v[1 + m] = cm * s0;
+                v[1 + N + m] = cm * cm * s0;
+            }
 #pragma unroll
-    for (int i = 0; i < MOM_CHUNK; i++) {
-        if (i >= n) break;
-        const int32_t l = lab[i];
-        if (l != cur) {
-            flush();
-            cur = l;
-            start = x0 + i;
-            s0 = sx = sxx = 0.0;
+            for (int p = 0; p < Lay::NP; p++) {
+                int i, j;
+                pair_of<N>(p, i, j);
+                v[1 + 2 * N + p] = (i == 0) ? (double)c[j] * sx : (double)c[i] * (double)c[j] * s0;
+            }
+            const int e = mom_entry(tkey, sl);
+            if (e >= 0) {
+                atomicMin(&torg[e], (unsigned long long)(base + start));
+                if (s0 == 0.0) return;   // every value <= 0 (NaN sums go on)
+#pragma unroll
+                for (int j = 0; j < S; j++) atomicAdd(&tval[e * S + j], v[j]);
+            } else {
+                atomicMin(&origin[sl], (unsigned long long)(base + start));
+                if (s0 == 0.0) return;
+                double *a = acc + (int64_t)sl * S;
+#pragma unroll
+                for (int j = 0; j < S; j++) atomicAdd(&a[j], v[j]);
+            }
+        };
+#pragma unroll
+        for (int i = 0; i < MOM_CHUNK; i++) {
+            if (i >= n) break;
+            const int32_t l = lab[i];
+            if (l != cur) {
+                flush();
+                cur = l;
+                start = x0 + i;
+                s0 = sx = sxx = 0.0;
+            }
+            if (l > 0) {
+                double v = (double)dv[i];
+                v = (v < 0.0) ? 0.0 : v;   // std::max(value, T()): NaN stays NaN
+                const double x = (double)(x0 + i);
+                s0 += v;
+                sx += v * x;
+                sxx += v * x * x;
+            }
         }
-        if (l > 0) {
-            double v = (double)dv[i];
-            v = (v < 0.0) ? 0.0 : v;   // std::max(value, T()): NaN stays NaN
-            const double x = (double)(x0 + i);
-            s0 += v;
-            sx += v * x;
-            sxx += v * x * x;
+        flush();
+    }
+    __syncthreads();
+    // the table to the global accumulators (zero components skipped: adding
+    // +0 changes nothing; NaN sums are added)
+    for (int e = tid; e < MOM_TBL; e += MOM_NT) {
+        const int32_t sl = tkey[e];
+        if (sl < 0) continue;
+        atomicMin(&origin[sl], torg[e]);
+        double *a = acc + (int64_t)sl * S;
+        for (int j = 0; j < S; j++) {
+            const double v = tval[e * S + j];
+            if (v != 0.0) atomicAdd(&a[j], v);
         }
     }
-    flush();
 }
 
 // The reference's outputs from the raw sums (label.hpp:554-722; what as
@@ -540,6 +606,7 @@ static int moment_geometry(int ndim, const int64_t *shape, const int32_t *labels
     g.vec = (g.W % 4 == 0 && ((uintptr_t)labels & 15) == 0) ? 1 : 0;
     g.dvec = (g.W % 4 == 0 && ((uintptr_t)data & 15) == 0) ? 1 : 0;
     g.small = (g.rows * g.chunks < (int64_t(1) << 31)) ? 1 : 0;
+    g.n_pc = 0;
     g.maxl = 0;
     return CBX_OK;
 }
@@ -579,10 +646,14 @@ static int moments_n(const int32_t *labels, const T *data, const MomGeom &g, con
     CBX_HIP(org.alloc((size_t)n_slots, st));
     CBX_HIP(hipMemsetAsync(acc.ptr, 0, sizeof(double) * n_slots * Lay::STRIDE, st));
     CBX_HIP(hipMemsetAsync(org.ptr, 0xFF, sizeof(unsigned long long) * n_slots, st));
-    const int64_t units = g.rows * g.chunks;
-    if (units > 0) {
+    if (g.rows * g.chunks > 0) {
+        // one workgroup per patch of MOM_PR rows x MOM_PC chunks
+        MomGeom gp = g;
+        gp.n_pc = (g.chunks + MOM_PC - 1) / MOM_PC;
+        const int64_t patches = ((g.rows + MOM_PR - 1) / MOM_PR) * gp.n_pc;
+        if (patches >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "label map too large");
         KernelTimer timer("region_moments", st);
-        k_region_accumulate<N, T><<<grid_for(units, MOM_NT), MOM_NT, 0, st>>>(labels, data, g, slot, acc.ptr, org.ptr);
+        k_region_accumulate<N, T><<<(unsigned)patches, MOM_NT, 0, st>>>(labels, data, gp, slot, acc.ptr, org.ptr);
     }
     k_region_finish<N, T><<<grid_for(n_index, MOM_NT), MOM_NT, 0, st>>>(acc.ptr, org.ptr, slot, index, n_index, g,
                                                                         what, out);
