@@ -474,57 +474,84 @@ __device__ __forceinline__ T seg_distance(const T *ln, const int64_t *p)
 }
 
 // p_values counts: pixels of region j within xtol of lines[j] (n_total) and
-// those of them reaching vmin (n_signal), per run then atomically per slot.
+// those of them reaching vmin (n_signal), per run, pre-reduced per patch of
+// MOM_PR rows x MOM_PC chunks in a workgroup table (as k_region_accumulate),
+// then atomically per slot.
 template <int N, typename T>
 __global__ void __launch_bounds__(MOM_NT)
 k_region_count(const int32_t *__restrict__ labels, const T *__restrict__ data, MomGeom g,
                const int32_t *__restrict__ slot, const T *__restrict__ lines, T vmin, T xtol,
                unsigned long long *__restrict__ counts)
 {
-    const int64_t t = (int64_t)blockIdx.x * MOM_NT + threadIdx.x;
-    if (t >= g.rows * g.chunks) return;
-    int64_t row, x0;
-    chunk_of(g, t, row, x0);
-    const int n = (int)min((int64_t)MOM_CHUNK, g.W - x0);
-    const int64_t base = row * g.W;
-    int32_t lab[MOM_CHUNK];
-    load_chunk(labels, g, base, x0, n, lab);
-    bool any = false;
-#pragma unroll
-    for (int i = 0; i < MOM_CHUNK; i++) any |= lab[i] > 0;
-    if (!any) return;
-    T dv[MOM_CHUNK];
-    load_data(data, g, base, x0, n, dv);
-    int64_t p[N];
-    row_coords<N>(g, row, p);
-    int32_t cur = 0, s = NO_SLOT;
-    unsigned long long nt = 0, ns = 0;
-    T ln[2 * N];
-    auto flush = [&]() {
-        if (s == NO_SLOT) return;
-        if (nt) atomicAdd(&counts[2 * (int64_t)s], nt);
-        if (ns) atomicAdd(&counts[2 * (int64_t)s + 1], ns);
-    };
-#pragma unroll
-    for (int i = 0; i < MOM_CHUNK; i++) {
-        if (i >= n) break;
-        const int32_t l = lab[i];
-        if (l != cur) {
-            flush();
-            cur = l;
-            nt = ns = 0;
-            s = (l > 0 && l <= g.maxl) ? slot[l] : NO_SLOT;
-            if (s != NO_SLOT)
-                for (int m = 0; m < 2 * N; m++) ln[m] = lines[2 * N * (int64_t)s + m];
-        }
-        if (s == NO_SLOT) continue;
-        p[0] = x0 + i;
-        if (seg_distance<N, T>(ln, p) < xtol) {
-            nt++;
-            if (dv[i] >= vmin) ns++;
-        }
+    __shared__ int32_t tkey[MOM_TBL];
+    __shared__ uint32_t tcnt[2 * MOM_TBL];
+    const int tid = threadIdx.x;
+    for (int e = tid; e < MOM_TBL; e += MOM_NT) {
+        tkey[e] = -1;
+        tcnt[2 * e] = tcnt[2 * e + 1] = 0;
     }
-    flush();
+    __syncthreads();
+    const int64_t pr = (int64_t)blockIdx.x / g.n_pc, pc = (int64_t)blockIdx.x - pr * g.n_pc;
+    const int64_t row = pr * MOM_PR + tid / MOM_PC, ch = pc * MOM_PC + tid % MOM_PC;
+    bool any = false;
+    int32_t lab[MOM_CHUNK];
+    int64_t x0 = 0, base = 0;
+    int n = 0;
+    if (row < g.rows && ch < g.chunks) {
+        x0 = ch * MOM_CHUNK;
+        n = (int)min((int64_t)MOM_CHUNK, g.W - x0);
+        base = row * g.W;
+        load_chunk(labels, g, base, x0, n, lab);
+#pragma unroll
+        for (int i = 0; i < MOM_CHUNK; i++) any |= lab[i] > 0;
+    }
+    if (any) {
+        T dv[MOM_CHUNK];
+        load_data(data, g, base, x0, n, dv);
+        int64_t p[N];
+        row_coords<N>(g, row, p);
+        int32_t cur = 0, s = NO_SLOT;
+        uint32_t nt = 0, ns = 0;
+        T ln[2 * N];
+        auto flush = [&]() {
+            if (s == NO_SLOT || nt == 0) return;   // ns <= nt
+            const int e = mom_entry(tkey, s);
+            if (e >= 0) {
+                atomicAdd(&tcnt[2 * e], nt);
+                if (ns) atomicAdd(&tcnt[2 * e + 1], ns);
+            } else {
+                atomicAdd(&counts[2 * (int64_t)s], (unsigned long long)nt);
+                if (ns) atomicAdd(&counts[2 * (int64_t)s + 1], (unsigned long long)ns);
+            }
+        };
+#pragma unroll
+        for (int i = 0; i < MOM_CHUNK; i++) {
+            if (i >= n) break;
+            const int32_t l = lab[i];
+            if (l != cur) {
+                flush();
+                cur = l;
+                nt = ns = 0;
+                s = (l > 0 && l <= g.maxl) ? slot[l] : NO_SLOT;
+                if (s != NO_SLOT)
+                    for (int m = 0; m < 2 * N; m++) ln[m] = lines[2 * N * (int64_t)s + m];
+            }
+            if (s == NO_SLOT) continue;
+            p[0] = x0 + i;
+            if (seg_distance<N, T>(ln, p) < xtol) {
+                nt++;
+                if (dv[i] >= vmin) ns++;
+            }
+        }
+        flush();
+    }
+    __syncthreads();
+    for (int e = tid; e < MOM_TBL; e += MOM_NT) {
+        const int32_t sl = tkey[e];
+        if (sl < 0) continue;
+        if (tcnt[2 * e]) atomicAdd(&counts[2 * (int64_t)sl], (unsigned long long)tcnt[2 * e]);
+        if (tcnt[2 * e + 1]) atomicAdd(&counts[2 * (int64_t)sl + 1], (unsigned long long)tcnt[2 * e + 1]);
+    }
 }
 
 __device__ __forceinline__ double dev_logaddexp(double a, double b)
@@ -665,11 +692,14 @@ template <int N, typename T>
 static int count_n(const int32_t *labels, const T *data, const MomGeom &g, const int32_t *slot, const T *lines,
                    T vmin, T xtol, unsigned long long *counts, hipStream_t st)
 {
-    const int64_t units = g.rows * g.chunks;
-    if (units > 0) {
+    if (g.rows * g.chunks > 0) {
+        MomGeom gp = g;
+        gp.n_pc = (g.chunks + MOM_PC - 1) / MOM_PC;
+        const int64_t patches = ((g.rows + MOM_PR - 1) / MOM_PR) * gp.n_pc;
+        if (patches >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "label map too large");
         KernelTimer timer("region_counts", st);
-        k_region_count<N, T><<<grid_for(units, MOM_NT), MOM_NT, 0, st>>>(labels, data, g, slot, lines, vmin, xtol,
-                                                                         counts);
+        k_region_count<N, T><<<(unsigned)patches, MOM_NT, 0, st>>>(labels, data, gp, slot, lines, vmin, xtol,
+                                                                   counts);
     }
     CBX_HIP(hipGetLastError());
     return CBX_OK;
