@@ -28,7 +28,6 @@
 // reference's emission order, a stable radix sort by pixel and an ordered
 // segmented reduction.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <climits>
@@ -417,12 +416,6 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
     }
     if (nseg > ub) atomicOr(flags, FLAG_STUCK); // cannot happen: the bound holds for every walk
     for (int k = nseg; k < ub; k++) segs[base + k].ncols = 0;
-}
-
-__global__ void k_tile_total(int64_t n, const int *__restrict__ a, const int *__restrict__ b, int *__restrict__ sum)
-{
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) sum[i] = a[i] + b[i];
 }
 
 // Clears a plan's counters before a draw (a kernel, not a memset: a draw may
@@ -1774,9 +1767,8 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     P.n_segs = (int64_t)slots;
     if (P.n_segs == 0) return CBX_OK;
     CBX_HIP(P.segs.alloc(P.n_segs, st));
-    DevBuf<int> cnt2, total;
+    DevBuf<int> cnt2;
     CBX_HIP(cnt2.alloc(P.n_tiles + 1, st));
-    CBX_HIP(total.alloc(P.n_tiles + 1, st));
     CBX_HIP(hipMemsetAsync(cnt2.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
     {
         KernelTimer timer("segments_walk", st);
@@ -1790,8 +1782,8 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
                                                                        P.flags.ptr, line_base);
     }
     CBX_HIP(hipGetLastError());
-    k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, cnt2.ptr, total.ptr);
-    CBX_HIP(exclusive_scan_i32(total.ptr, P.tile_off.ptr, P.n_tiles + 1, st));
+    // tile lists: offsets of cnt1 + cnt2 (one scan kernel)
+    CBX_HIP(exclusive_scan_total_i32(P.tile_cnt.ptr, cnt2.ptr, P.tile_off.ptr, P.n_tiles, st));
     {
         const void *src[2] = {P.tile_off.ptr + P.n_tiles, P.flags.ptr};
         const int nb[2] = {4, 4};
@@ -2402,9 +2394,8 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
 struct cbx_plan {
     cbx::SegmentPlan<float> P; // capacities in P.n_segs (slots) and P.n_items
     cbx::DevBuf<unsigned long long> counters;
-    cbx::DevBuf<int> cnt2, total;
-    cbx::DevBuf<uint8_t> scan_tmp;
-    size_t scan_bytes = 0;
+    cbx::DevBuf<int> cnt2;
+    cbx::DevBuf<unsigned long long> scan_state; // the tile-offset scan's look-back state
     int64_t n_lines_max = 0;
     int64_t bytes = 0;
     cbx_draw_opts opts{};
@@ -2476,17 +2467,16 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
     take(P.pairs.alloc(item_cap, st));
     take(pl->counters.alloc(2, st));
     take(pl->cnt2.alloc(P.n_tiles + 1, st));
-    take(pl->total.alloc(P.n_tiles + 1, st));
-    if (e == hipSuccess)
-        e = hipcub::DeviceScan::ExclusiveSum(nullptr, pl->scan_bytes, pl->total.ptr, P.tile_off.ptr,
-                                             (int)(P.n_tiles + 1), st);
-    take(pl->scan_tmp.alloc(pl->scan_bytes ? pl->scan_bytes : 1, st));
+    const size_t scan_words = scan_state_words(P.n_tiles + 1);
+    take(pl->scan_state.alloc(scan_words, st));
+    if (e == hipSuccess) e = clear_scan_state(pl->scan_state.ptr, scan_words, st);
     if (e != hipSuccess) {
         delete pl;
         return fail(CBX_ERUNTIME, std::string("plan allocation: ") + hipGetErrorString(e));
     }
     pl->bytes = (int64_t)(sizeof(LineSetup<float>) * n_lines + sizeof(Segment<float>) * slot_cap +
-                          sizeof(int2) * item_cap + 4 * sizeof(int) * (P.n_tiles + 1) + pl->scan_bytes);
+                          sizeof(int2) * item_cap + 3 * sizeof(int) * (P.n_tiles + 1) +
+                          sizeof(unsigned long long) * scan_words);
     *out_plan = pl;
     return CBX_OK;
 }
@@ -2514,11 +2504,9 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
         k_walk_lines<float, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
                                                                        pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
                                                                        P.flags.ptr, nullptr);
-    k_tile_total<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles + 1, P.tile_cnt.ptr, pl->cnt2.ptr,
-                                                              pl->total.ptr);
-    size_t bytes = pl->scan_bytes;
-    CBX_HIP(hipcub::DeviceScan::ExclusiveSum(pl->scan_tmp.ptr, bytes, pl->total.ptr, P.tile_off.ptr,
-                                             (int)(P.n_tiles + 1), st));
+    // the plan's own scan state (zeroed at creation, left zeroed by each scan)
+    CBX_HIP(scan_i32_with_state(P.tile_cnt.ptr, pl->cnt2.ptr, P.tile_off.ptr, P.n_tiles, P.n_tiles + 1,
+                                pl->scan_state.ptr, st));
     k_bin_slots<float><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr,
                                                                 P.tile_off.ptr, P.tile_cnt.ptr, P.tile_cur.ptr,
                                                                 P.pairs.ptr, pl->counters.ptr, P.n_tiles, P.n_items,

@@ -1,6 +1,6 @@
 // Host-side helpers shared by the C-ABI entry points: status codes, the
-// thread-local error message, stream-ordered scratch buffers and the hipcub
-// scans.
+// thread-local error message, stream-ordered scratch buffers and the
+// device-wide scans.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -53,8 +53,13 @@ class Workspace {
     bool overflow() const { return overflow_; }
     size_t requested() const { return requested_; }
     static Workspace *current();
+    // zeroed look-back state of at least `words` words for one exclusive scan
+    // on this workspace's stream (kept per cache; carved and cleared in a
+    // fixed workspace)
+    hipError_t scan_state(size_t words, unsigned long long **p);
 
   private:
+    hipStream_t stream_ = nullptr;
     struct Cache *cache_ = nullptr;
     Workspace *outer_ = nullptr;
     size_t block_ = 0, off_ = 0;
@@ -134,8 +139,21 @@ struct KernelTimer {
 // src[i]: device address, bytes[i]: 4 or 8; out[i]: the value, zero-extended.
 hipError_t read_device_values(hipStream_t s, int n, const void *const *src, const int *bytes, uint64_t *out);
 
+// Exclusive prefix sums (one kernel, single pass), ordered on stream s:
+// out[i] = in[0] + ... + in[i - 1] for i < n.  The _total forms read n
+// entries of a (+ b when given) and write n + 1 (out[n] = the total).
+// int32 sums wrap (a negative total flags overflow); int64 totals must stay
+// within +-2^61.
 hipError_t exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s);
 hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s);
+hipError_t exclusive_scan_total_i32(const int *a, const int *b, int *out, int64_t n, hipStream_t s);
+hipError_t exclusive_scan_total_i64(const int64_t *a, int64_t *out, int64_t n, hipStream_t s);
+// The same scan on caller-held state: scan_state_words(n_out) words, zero
+// before the first scan (clear_scan_state) and left zero by each scan.
+size_t scan_state_words(int64_t n_out);
+hipError_t clear_scan_state(unsigned long long *state, size_t words, hipStream_t s);
+hipError_t scan_i32_with_state(const int *a, const int *b, int *out, int64_t n_in, int64_t n_out,
+                               unsigned long long *state, hipStream_t s);
 
 inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
 

@@ -984,18 +984,16 @@ static int label_runs(In in, int ndim, const int64_t *shape, int prev, int64_t n
     DevBuf<uint64_t> bits;
     DevBuf<int32_t> rcnt, roff, par, kcnt, koff, size;
     CBX_HIP(bits.alloc(R * g.nw, st));
-    CBX_HIP(rcnt.alloc(R + 1, st));
+    CBX_HIP(rcnt.alloc(R, st));
     CBX_HIP(roff.alloc(R + 1, st));
     CBX_HIP(par.alloc(max_runs, st));
-    CBX_HIP(kcnt.alloc(R + 1, st));
+    CBX_HIP(kcnt.alloc(R, st));
     CBX_HIP(koff.alloc(R + 1, st));
     if (sizes) CBX_HIP(size.alloc(max_runs, st));
-    CBX_HIP(hipMemsetAsync(rcnt.ptr + R, 0, sizeof(int32_t), st));
-    CBX_HIP(hipMemsetAsync(kcnt.ptr + R, 0, sizeof(int32_t), st));
     const unsigned grid = (unsigned)((R + RUN_ROWS - 1) / RUN_ROWS);
     KernelTimer timer("label_total", st);
     k_run_pack<In><<<grid, RUN_NT, 0, st>>>(in, g, bits.ptr, rcnt.ptr);
-    CBX_HIP(exclusive_scan_i32(rcnt.ptr, roff.ptr, R + 1, st));
+    CBX_HIP(exclusive_scan_total_i32(rcnt.ptr, nullptr, roff.ptr, R, st));
     if (sizes) k_run_init<true><<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, size.ptr);
     if (prev & 14) {
         const int64_t n_bands = (int64_t)(R / g.H) * ((g.H + RUN_BAND - 1) / RUN_BAND);
@@ -1022,7 +1020,7 @@ static int label_runs(In in, int ndim, const int64_t *shape, int prev, int64_t n
         k_run_resolve_flat<<<2048, RUN_NT, 0, st>>>(roff.ptr + R, par.ptr);
         k_run_roots<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, par.ptr, kcnt.ptr);
     }
-    CBX_HIP(exclusive_scan_i32(kcnt.ptr, koff.ptr, R + 1, st));
+    CBX_HIP(exclusive_scan_total_i32(kcnt.ptr, nullptr, koff.ptr, R, st));
     k_run_rank<<<grid, RUN_NT, 0, st>>>(g, roff.ptr, koff.ptr, par.ptr, sizes ? size.ptr : nullptr, npts);
     const int vec = (g.W % 4 == 0 && ((uintptr_t)out & 15) == 0) ? 1 : 0;
     k_run_final<<<grid, RUN_NT, 0, st>>>(g, bits.ptr, roff.ptr, par.ptr, out, vec);

@@ -1,13 +1,14 @@
 // Host utilities of the C ABI: the per-thread error message, version,
-// profile names and the device-wide scans (hipcub / rocPRIM).
+// profile names, workspace caches, device-to-host reads and the device-wide
+// exclusive scans.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <algorithm>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "cbx_host.hpp"
@@ -73,6 +74,8 @@ struct Cache {
     hipStream_t stream;
     bool busy;
     std::vector<std::pair<void *, size_t>> blocks;
+    unsigned long long *scan_state = nullptr;   // zero between scans
+    size_t scan_words = 0;
 };
 
 static std::mutex g_ws_mu;
@@ -83,7 +86,7 @@ static const size_t WS_MIN_BLOCK = size_t(64) << 20;
 
 Workspace *Workspace::current() { return g_ws_current; }
 
-Workspace::Workspace(hipStream_t s)
+Workspace::Workspace(hipStream_t s) : stream_(s)
 {
     int dev = 0;
     status_ = hipGetDevice(&dev);
@@ -118,8 +121,8 @@ Workspace::Workspace(hipStream_t s)
     g_ws_current = this;
 }
 
-Workspace::Workspace(hipStream_t, void *base, size_t capacity)
-    : fixed_base_(static_cast<char *>(base)), fixed_cap_(capacity)
+Workspace::Workspace(hipStream_t s, void *base, size_t capacity)
+    : stream_(s), fixed_base_(static_cast<char *>(base)), fixed_cap_(capacity)
 {
     if (!base) status_ = hipErrorInvalidValue;
     outer_ = g_ws_current;
@@ -177,6 +180,36 @@ hipError_t Workspace::take(void **p, size_t bytes)
     return hipSuccess;
 }
 
+hipError_t Workspace::scan_state(size_t words, unsigned long long **p)
+{
+    *p = nullptr;
+    if (status_ != hipSuccess) return status_;
+    if (fixed_base_) {
+        void *q = nullptr;
+        hipError_t e = take(&q, sizeof(unsigned long long) * words);
+        if (e != hipSuccess) return e;
+        if ((e = clear_scan_state(static_cast<unsigned long long *>(q), words, stream_)) != hipSuccess) return e;
+        *p = static_cast<unsigned long long *>(q);
+        return hipSuccess;
+    }
+    Cache *c = cache_;
+    if (c->scan_words < words) {
+        // hipFree waits for the device, so no scan still uses the old state
+        if (c->scan_state) (void)hipFree(c->scan_state);
+        c->scan_state = nullptr;
+        c->scan_words = 0;
+        const size_t w = std::max(words, size_t(4096));
+        void *q = nullptr;
+        hipError_t e = hipMalloc(&q, sizeof(unsigned long long) * w);
+        if (e != hipSuccess) return e;
+        c->scan_state = static_cast<unsigned long long *>(q);
+        c->scan_words = w;
+        if ((e = clear_scan_state(c->scan_state, w, stream_)) != hipSuccess) return e;
+    }
+    *p = c->scan_state;
+    return hipSuccess;
+}
+
 static size_t workspace_release()
 {
     std::lock_guard<std::mutex> g(g_ws_mu);
@@ -195,6 +228,7 @@ static size_t workspace_release()
             (void)hipFree(b.first);
             freed += b.second;
         }
+        if (c->scan_state) (void)hipFree(c->scan_state);
         delete c;
         it = g_ws.erase(it);
     }
@@ -287,23 +321,228 @@ hipError_t read_device_values(hipStream_t s, int n, const void *const *src, cons
     return hipSuccess;
 }
 
-template <typename V>
-static hipError_t exclusive_scan(const V *in, V *out, int64_t n, hipStream_t s)
+// ---------------------------------------------------------------------------
+// exclusive prefix sums: one kernel, single pass with decoupled look-back
+// ---------------------------------------------------------------------------
+// Workgroups take tiles in the order they start (a ticket counter), so every
+// tile a workgroup waits for belongs to a workgroup that is already running
+// and waits only on tiles before its own: every wait ends.  Each tile
+// publishes its sum (AGG) as soon as it has it and its inclusive prefix (INC)
+// once its look-back is done; wave 0 reads 64 predecessors per step and stops
+// at the nearest INC.  Status and value share one 64-bit word (status in bits
+// 62-63, the value's low 62 bits, sign-extended when read), so no ordering
+// between separate words is needed.  The state (ticket, done counter, one
+// word per tile) must be zero when a scan starts; the workgroup that finishes
+// last (done counter) clears it for the next scan, after every workgroup's
+// look-back has ended.
+constexpr int SCAN_NT = 256;
+constexpr int SCAN_IPT = 8;
+constexpr int SCAN_TILE = SCAN_NT * SCAN_IPT;
+constexpr uint64_t SCAN_AGG = 1ull << 62;
+constexpr uint64_t SCAN_INC = 2ull << 62;
+constexpr uint64_t SCAN_VAL = SCAN_AGG - 1;
+constexpr unsigned SCAN_MAX_POLLS = 1u << 20;   // a bound only: every wait ends
+
+size_t scan_state_words(int64_t n_out) { return 2 + (size_t)((n_out + SCAN_TILE - 1) / SCAN_TILE); }
+
+template <typename U>
+__device__ __forceinline__ U scan_wave_incl(U v, int lane)
 {
-    if (n <= 0) return hipSuccess;
-    size_t bytes = 0;
-    hipError_t e = hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, (int)n, s);
-    if (e != hipSuccess) return e;
-    DevBuf<uint8_t> tmp;
-    e = tmp.alloc(bytes ? bytes : 1, s);
-    if (e != hipSuccess) return e;
-    return hipcub::DeviceScan::ExclusiveSum(tmp.ptr, bytes, in, out, (int)n, s);
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const U o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
 }
 
-hipError_t exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s) { return exclusive_scan(in, out, n, s); }
+template <typename V>
+__device__ __forceinline__ uint64_t scan_pack(V v)
+{
+    return (uint64_t)(int64_t)v & SCAN_VAL;
+}
+
+template <typename V>
+__device__ __forceinline__ V scan_unpack(uint64_t w)
+{
+    return (V)(((int64_t)(w << 2)) >> 2);
+}
+
+// out[i] = sum of a[j] (+ b[j]) over j < i, for i < n_out; entries i >= n_in
+// read as zero (n_out = n_in + 1 gives the total in out[n_in]).  Sums wrap
+// like V's unsigned counterpart (callers detect int32 overflow as a negative
+// total); int64 totals must stay within +-2^61.
+template <typename V>
+__global__ void __launch_bounds__(SCAN_NT)
+k_scan(const V *__restrict__ a, const V *__restrict__ b, V *__restrict__ out, int64_t n_in, int64_t n_out,
+       unsigned long long *__restrict__ state)
+{
+    using U = typename std::make_unsigned<V>::type;
+    __shared__ U buf[SCAN_TILE];
+    __shared__ U wsum[SCAN_NT / 64];
+    __shared__ U s_prefix;
+    __shared__ int s_tile;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nblk = (int)gridDim.x;
+    if (tid == 0) s_tile = (int)atomicAdd(&state[0], 1ull);
+    __syncthreads();
+    const int tile = s_tile;
+    const int64_t base = (int64_t)tile * SCAN_TILE;
+    // coalesced load, then each thread takes SCAN_IPT consecutive entries
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const int64_t i = base + k * SCAN_NT + tid;
+        U v = 0;
+        if (i < n_in) {
+            v = (U)a[i];
+            if (b) v += (U)b[i];
+        }
+        buf[k * SCAN_NT + tid] = v;
+    }
+    __syncthreads();
+    U loc[SCAN_IPT], ts = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        loc[k] = buf[tid * SCAN_IPT + k];
+        ts += loc[k];
+    }
+    const U incl = scan_wave_incl<U>(ts, lane);
+    if (lane == 63) wsum[wave] = incl;
+    __syncthreads();
+    U wpre = 0, agg = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_NT / 64; w++) {
+        wpre += (w < wave) ? wsum[w] : U(0);
+        agg += wsum[w];
+    }
+    if (wave == 0) {
+        U prefix = 0;
+        if (tile == 0) {
+            if (lane == 0) __hip_atomic_store(&state[2], SCAN_INC | scan_pack<V>((V)agg), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            if (lane == 0) __hip_atomic_store(&state[2 + tile], SCAN_AGG | scan_pack<V>((V)agg), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+            int j = tile - 1;
+            unsigned polls = 0;
+            while (true) {
+                const int k = j - lane;
+                const uint64_t w = (k >= 0) ? __hip_atomic_load(&state[2 + k], __ATOMIC_RELAXED,
+                                                                __HIP_MEMORY_SCOPE_AGENT)
+                                            : SCAN_INC;
+                const uint64_t inc = __ballot((w >> 62) == 2);
+                const uint64_t inv = __ballot((w >> 62) == 0);
+                const int first = inc ? __builtin_ctzll(inc) : 64;
+                const uint64_t need = (first >= 63) ? ~0ull : ((2ull << first) - 1);
+                if ((inv & need) && ++polls < SCAN_MAX_POLLS) {
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                U part = (lane <= first) ? (U)scan_unpack<V>(w) : U(0);
+#pragma unroll
+                for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
+                prefix += part;
+                if (inc || polls >= SCAN_MAX_POLLS) break;
+                j -= 64;
+            }
+            if (lane == 0) __hip_atomic_store(&state[2 + tile], SCAN_INC | scan_pack<V>((V)(prefix + agg)),
+                                              __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    U run = s_prefix + wpre + incl - ts;
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        buf[tid * SCAN_IPT + k] = run;
+        run += loc[k];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < SCAN_IPT; k++) {
+        const int64_t i = base + k * SCAN_NT + tid;
+        if (i < n_out) out[i] = (V)buf[k * SCAN_NT + tid];
+    }
+    if (wave == 0) {
+        // count the finished look-backs (release: this tile's INC before the
+        // count; acquire: the clear after every other's); the last one clears
+        // the state for the next scan
+        unsigned long long d = 0;
+        if (lane == 0) d = __hip_atomic_fetch_add(&state[1], 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        d = __shfl(d, 0, 64);
+        if (d == (unsigned long long)(nblk - 1))
+            for (int i = lane; i < nblk + 2; i += 64) state[i] = 0ull;
+    }
+}
+
+__global__ void k_scan_clear(unsigned long long *__restrict__ state, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) state[i] = 0ull;
+}
+
+template <typename V>
+static hipError_t launch_scan(const V *a, const V *b, V *out, int64_t n_in, int64_t n_out,
+                              unsigned long long *state, hipStream_t s)
+{
+    if (n_out <= 0) return hipSuccess;
+    if (n_in < 0 || n_in > n_out || n_out >= (int64_t(1) << 31)) return hipErrorInvalidValue;
+    const int64_t nblk = (n_out + SCAN_TILE - 1) / SCAN_TILE;
+    k_scan<V><<<(unsigned)nblk, SCAN_NT, 0, s>>>(a, b, out, n_in, n_out, state);
+    return hipGetLastError();
+}
+
+hipError_t clear_scan_state(unsigned long long *state, size_t words, hipStream_t s)
+{
+    if (words == 0) return hipSuccess;
+    k_scan_clear<<<grid_for((int64_t)words, 256), 256, 0, s>>>(state, (int64_t)words);
+    return hipGetLastError();
+}
+
+hipError_t scan_i32_with_state(const int *a, const int *b, int *out, int64_t n_in, int64_t n_out,
+                               unsigned long long *state, hipStream_t s)
+{
+    return launch_scan<int>(a, b, out, n_in, n_out, state, s);
+}
+
+// The look-back state of scans on this thread's open workspace: kept per
+// (device, stream) cache and left zeroed by every scan; in a fixed workspace
+// (nothing may be allocated) or with no workspace open, carved or allocated
+// for the call and cleared by a kernel first.
+template <typename V>
+static hipError_t exclusive_scan(const V *a, const V *b, V *out, int64_t n_in, int64_t n_out, hipStream_t s)
+{
+    if (n_out <= 0) return hipSuccess;
+    const size_t words = scan_state_words(n_out);
+    Workspace *ws = Workspace::current();
+    unsigned long long *state = nullptr;
+    hipError_t e = ws ? ws->scan_state(words, &state) : hipSuccess;
+    if (e != hipSuccess) return e;
+    if (state) return launch_scan<V>(a, b, out, n_in, n_out, state, s);
+    DevBuf<unsigned long long> tmp;
+    if ((e = tmp.alloc(words, s)) != hipSuccess) return e;
+    if ((e = clear_scan_state(tmp.ptr, words, s)) != hipSuccess) return e;
+    return launch_scan<V>(a, b, out, n_in, n_out, tmp.ptr, s);
+}
+
+hipError_t exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s)
+{
+    return exclusive_scan<int>(in, nullptr, out, n, n, s);
+}
+
 hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStream_t s)
 {
-    return exclusive_scan(in, out, n, s);
+    return exclusive_scan<int64_t>(in, nullptr, out, n, n, s);
+}
+
+hipError_t exclusive_scan_total_i32(const int *a, const int *b, int *out, int64_t n, hipStream_t s)
+{
+    return exclusive_scan<int>(a, b, out, n, n + 1, s);
+}
+
+hipError_t exclusive_scan_total_i64(const int64_t *a, int64_t *out, int64_t n, hipStream_t s)
+{
+    return exclusive_scan<int64_t>(a, nullptr, out, n, n + 1, s);
 }
 
 } // namespace cbx
