@@ -421,9 +421,11 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
 // Clears a plan's counters before a draw (a kernel, not a memset: a draw may
 // be captured in a HIP graph).
 __global__ void k_plan_reset(int64_t n_tiles, unsigned long long *__restrict__ counters, int *__restrict__ flags,
-                             int *__restrict__ cnt1, int *__restrict__ cnt2, int *__restrict__ cur)
+                             int *__restrict__ cnt1, int *__restrict__ cnt2, int *__restrict__ cur,
+                             unsigned long long *__restrict__ scan_state, int64_t scan_words)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < scan_words) scan_state[i] = 0ull;
     if (i == 0) {
         counters[0] = 0ull;
         counters[1] = 0ull;
@@ -2492,8 +2494,10 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
     if (n_lines == 0) return CBX_OK;
     const int blk = 256;
     const Geometry &g = P.g;
-    k_plan_reset<<<grid_for(P.n_tiles + 1, blk), blk, 0, st>>>(P.n_tiles, pl->counters.ptr, P.flags.ptr,
-                                                               P.tile_cnt.ptr, pl->cnt2.ptr, P.tile_cur.ptr);
+    const int64_t reset_n = std::max<int64_t>(P.n_tiles + 1, (int64_t)pl->scan_state.n);
+    k_plan_reset<<<grid_for(reset_n, blk), blk, 0, st>>>(P.n_tiles, pl->counters.ptr, P.flags.ptr, P.tile_cnt.ptr,
+                                                         pl->cnt2.ptr, P.tile_cur.ptr, pl->scan_state.ptr,
+                                                         (int64_t)pl->scan_state.n);
     k_setup_lines<float, int32_t><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                          P.setup.ptr, pl->counters.ptr, P.flags.ptr);
     if (n_lines <= WALK_RING_LINES)
@@ -2504,7 +2508,7 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
         k_walk_lines<float, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
                                                                        pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
                                                                        P.flags.ptr, nullptr);
-    // the plan's own scan state (zeroed at creation, left zeroed by each scan)
+    // the plan's own scan state (cleared by k_plan_reset above)
     CBX_HIP(scan_i32_with_state(P.tile_cnt.ptr, pl->cnt2.ptr, P.tile_off.ptr, P.n_tiles, P.n_tiles + 1,
                                 pl->scan_state.ptr, st));
     k_bin_slots<float><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr,

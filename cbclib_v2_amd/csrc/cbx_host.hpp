@@ -40,6 +40,12 @@ int fail(int code, const std::string &msg);
 // does not fit fails with hipErrorOutOfMemory and sets overflow().  While a
 // fixed Workspace is open, read_device_values and KernelTimer allocate
 // nothing either (no pinned mailbox, no events).
+struct ScanState {
+    unsigned long long *state = nullptr;   // zero
+    unsigned long long *clear = nullptr;   // for the scan to clear
+    size_t n_clear = 0;
+};
+
 class Workspace {
   public:
     explicit Workspace(hipStream_t s);
@@ -54,9 +60,10 @@ class Workspace {
     size_t requested() const { return requested_; }
     static Workspace *current();
     // zeroed look-back state of at least `words` words for one exclusive scan
-    // on this workspace's stream (kept per cache; carved and cleared in a
-    // fixed workspace)
-    hipError_t scan_state(size_t words, unsigned long long **p);
+    // on this workspace's stream, and the words the scan is to clear (two
+    // states per cache, used in turn; carved and cleared in a fixed workspace)
+    hipError_t scan_state(size_t words, struct ScanState *st);
+    void scan_failed();
 
   private:
     hipStream_t stream_ = nullptr;
@@ -149,7 +156,8 @@ hipError_t exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, hipStr
 hipError_t exclusive_scan_total_i32(const int *a, const int *b, int *out, int64_t n, hipStream_t s);
 hipError_t exclusive_scan_total_i64(const int64_t *a, int64_t *out, int64_t n, hipStream_t s);
 // The same scan on caller-held state: scan_state_words(n_out) words, zero
-// before the first scan (clear_scan_state) and left zero by each scan.
+// when the scan starts (clear_scan_state or the caller's own kernel) and left
+// set by it.
 size_t scan_state_words(int64_t n_out);
 hipError_t clear_scan_state(unsigned long long *state, size_t words, hipStream_t s);
 hipError_t scan_i32_with_state(const int *a, const int *b, int *out, int64_t n_in, int64_t n_out,

@@ -74,8 +74,10 @@ struct Cache {
     hipStream_t stream;
     bool busy;
     std::vector<std::pair<void *, size_t>> blocks;
-    unsigned long long *scan_state = nullptr;   // zero between scans
+    unsigned long long *scan_base = nullptr;   // two look-back states of scan_words each
     size_t scan_words = 0;
+    int scan_next = 0;        // the state the next scan uses (zero)
+    size_t scan_dirty = 0;    // words the previous scan left set in the other
 };
 
 static std::mutex g_ws_mu;
@@ -180,34 +182,52 @@ hipError_t Workspace::take(void **p, size_t bytes)
     return hipSuccess;
 }
 
-hipError_t Workspace::scan_state(size_t words, unsigned long long **p)
+hipError_t Workspace::scan_state(size_t words, ScanState *st)
 {
-    *p = nullptr;
+    *st = ScanState{};
     if (status_ != hipSuccess) return status_;
     if (fixed_base_) {
         void *q = nullptr;
         hipError_t e = take(&q, sizeof(unsigned long long) * words);
         if (e != hipSuccess) return e;
         if ((e = clear_scan_state(static_cast<unsigned long long *>(q), words, stream_)) != hipSuccess) return e;
-        *p = static_cast<unsigned long long *>(q);
+        st->state = static_cast<unsigned long long *>(q);
         return hipSuccess;
     }
     Cache *c = cache_;
     if (c->scan_words < words) {
-        // hipFree waits for the device, so no scan still uses the old state
-        if (c->scan_state) (void)hipFree(c->scan_state);
-        c->scan_state = nullptr;
+        // hipFree waits for the device, so no scan still uses the old states
+        if (c->scan_base) (void)hipFree(c->scan_base);
+        c->scan_base = nullptr;
         c->scan_words = 0;
         const size_t w = std::max(words, size_t(4096));
         void *q = nullptr;
-        hipError_t e = hipMalloc(&q, sizeof(unsigned long long) * w);
+        hipError_t e = hipMalloc(&q, 2 * sizeof(unsigned long long) * w);
         if (e != hipSuccess) return e;
-        c->scan_state = static_cast<unsigned long long *>(q);
+        c->scan_base = static_cast<unsigned long long *>(q);
+        if ((e = clear_scan_state(c->scan_base, 2 * w, stream_)) != hipSuccess) {
+            (void)hipFree(q);
+            c->scan_base = nullptr;
+            return e;
+        }
         c->scan_words = w;
-        if ((e = clear_scan_state(c->scan_state, w, stream_)) != hipSuccess) return e;
+        c->scan_next = 0;
+        c->scan_dirty = 0;
     }
-    *p = c->scan_state;
+    // this scan uses state scan_next and clears the other, which the previous
+    // scan on this stream (finished before this one starts) left dirty
+    st->state = c->scan_base + c->scan_next * c->scan_words;
+    st->clear = c->scan_base + (1 - c->scan_next) * c->scan_words;
+    st->n_clear = c->scan_dirty;
+    c->scan_dirty = words;
+    c->scan_next = 1 - c->scan_next;
     return hipSuccess;
+}
+
+void Workspace::scan_failed()
+{
+    // the launch did not run: neither state is known to be clear any more
+    if (cache_) cache_->scan_words = 0;
 }
 
 static size_t workspace_release()
@@ -228,7 +248,7 @@ static size_t workspace_release()
             (void)hipFree(b.first);
             freed += b.second;
         }
-        if (c->scan_state) (void)hipFree(c->scan_state);
+        if (c->scan_base) (void)hipFree(c->scan_base);
         delete c;
         it = g_ws.erase(it);
     }
@@ -328,22 +348,28 @@ hipError_t read_device_values(hipStream_t s, int n, const void *const *src, cons
 // tile a workgroup waits for belongs to a workgroup that is already running
 // and waits only on tiles before its own: every wait ends.  Each tile
 // publishes its sum (AGG) as soon as it has it and its inclusive prefix (INC)
-// once its look-back is done; wave 0 reads 64 predecessors per step and stops
-// at the nearest INC.  Status and value share one 64-bit word (status in bits
-// 62-63, the value's low 62 bits, sign-extended when read), so no ordering
-// between separate words is needed.  The state (ticket, done counter, one
-// word per tile) must be zero when a scan starts; the workgroup that finishes
-// last (done counter) clears it for the next scan, after every workgroup's
-// look-back has ended.
+// once its look-back is done; wave 0 reads 256 predecessors per step (four
+// per lane, one round trip: the state words are device-coherent atomics that
+// travel past the per-XCD L2 caches) and stops at the nearest INC.  Status
+// and value share one 64-bit word (status in bits 62-63, the value's low 62
+// bits, sign-extended when read), so no ordering between separate words is
+// needed.  The state (ticket, one word per tile) must be zero when a scan
+// starts; a scan may clear another, finished scan's state on the way
+// (`clear`), which is how consecutive scans on one stream alternate between
+// two states without a separate clearing launch.
+// Tiles of 2048 entries (8 per thread) up to SCAN_BIG entries: the shortest
+// workgroups for the small scans, whose time is a few dependent round trips;
+// 4096 above, halving the look-back of large scans.
 constexpr int SCAN_NT = 256;
-constexpr int SCAN_IPT = 8;
-constexpr int SCAN_TILE = SCAN_NT * SCAN_IPT;
+constexpr int SCAN_TILE = SCAN_NT * 8;           // the state is sized for the smaller tile
+constexpr int64_t SCAN_BIG = int64_t(1) << 19;
+constexpr int SCAN_LB = 4;                       // predecessors per lane per look-back step
 constexpr uint64_t SCAN_AGG = 1ull << 62;
 constexpr uint64_t SCAN_INC = 2ull << 62;
 constexpr uint64_t SCAN_VAL = SCAN_AGG - 1;
 constexpr unsigned SCAN_MAX_POLLS = 1u << 20;   // a bound only: every wait ends
 
-size_t scan_state_words(int64_t n_out) { return 2 + (size_t)((n_out + SCAN_TILE - 1) / SCAN_TILE); }
+size_t scan_state_words(int64_t n_out) { return 1 + (size_t)((n_out + SCAN_TILE - 1) / SCAN_TILE); }
 
 template <typename U>
 __device__ __forceinline__ U scan_wave_incl(U v, int lane)
@@ -372,22 +398,24 @@ __device__ __forceinline__ V scan_unpack(uint64_t w)
 // read as zero (n_out = n_in + 1 gives the total in out[n_in]).  Sums wrap
 // like V's unsigned counterpart (callers detect int32 overflow as a negative
 // total); int64 totals must stay within +-2^61.
-template <typename V>
+template <typename V, int SCAN_IPT>
 __global__ void __launch_bounds__(SCAN_NT)
 k_scan(const V *__restrict__ a, const V *__restrict__ b, V *__restrict__ out, int64_t n_in, int64_t n_out,
-       unsigned long long *__restrict__ state)
+       unsigned long long *__restrict__ state, unsigned long long *__restrict__ clear, int64_t n_clear)
 {
     using U = typename std::make_unsigned<V>::type;
-    __shared__ U buf[SCAN_TILE];
+    constexpr int TILE = SCAN_NT * SCAN_IPT;
+    __shared__ U buf[TILE];
     __shared__ U wsum[SCAN_NT / 64];
     __shared__ U s_prefix;
     __shared__ int s_tile;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int nblk = (int)gridDim.x;
     if (tid == 0) s_tile = (int)atomicAdd(&state[0], 1ull);
+    for (int64_t i = (int64_t)blockIdx.x * SCAN_NT + tid; i < n_clear; i += (int64_t)gridDim.x * SCAN_NT)
+        clear[i] = 0ull;
     __syncthreads();
     const int tile = s_tile;
-    const int64_t base = (int64_t)tile * SCAN_TILE;
+    const int64_t base = (int64_t)tile * TILE;
     // coalesced load, then each thread takes SCAN_IPT consecutive entries
 #pragma unroll
     for (int k = 0; k < SCAN_IPT; k++) {
@@ -418,34 +446,53 @@ k_scan(const V *__restrict__ a, const V *__restrict__ b, V *__restrict__ out, in
     if (wave == 0) {
         U prefix = 0;
         if (tile == 0) {
-            if (lane == 0) __hip_atomic_store(&state[2], SCAN_INC | scan_pack<V>((V)agg), __ATOMIC_RELAXED,
+            if (lane == 0) __hip_atomic_store(&state[1], SCAN_INC | scan_pack<V>((V)agg), __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            if (lane == 0) __hip_atomic_store(&state[2 + tile], SCAN_AGG | scan_pack<V>((V)agg), __ATOMIC_RELAXED,
+            if (lane == 0) __hip_atomic_store(&state[1 + tile], SCAN_AGG | scan_pack<V>((V)agg), __ATOMIC_RELAXED,
                                               __HIP_MEMORY_SCOPE_AGENT);
-            int j = tile - 1;
+            int j = tile - 1;   // the nearest predecessor not yet summed
             unsigned polls = 0;
             while (true) {
-                const int k = j - lane;
-                const uint64_t w = (k >= 0) ? __hip_atomic_load(&state[2 + k], __ATOMIC_RELAXED,
-                                                                __HIP_MEMORY_SCOPE_AGENT)
-                                            : SCAN_INC;
-                const uint64_t inc = __ballot((w >> 62) == 2);
-                const uint64_t inv = __ballot((w >> 62) == 0);
-                const int first = inc ? __builtin_ctzll(inc) : 64;
-                const uint64_t need = (first >= 63) ? ~0ull : ((2ull << first) - 1);
-                if ((inv & need) && ++polls < SCAN_MAX_POLLS) {
+                // predecessor at distance d = lane * SCAN_LB + q from j
+                uint64_t w[SCAN_LB];
+#pragma unroll
+                for (int q = 0; q < SCAN_LB; q++) {
+                    const int k = j - (lane * SCAN_LB + q);
+                    w[q] = (k >= 0) ? __hip_atomic_load(&state[1 + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                    : SCAN_INC;   // before tile 0: an inclusive zero
+                }
+                int first = 64 * SCAN_LB;   // distance of the nearest INC
+#pragma unroll
+                for (int q = 0; q < SCAN_LB; q++) {
+                    const uint64_t inc = __ballot((w[q] >> 62) == 2);
+                    if (inc) first = min(first, __builtin_ctzll(inc) * SCAN_LB + q);
+                }
+                bool wait = false;   // a predecessor up to the nearest INC has not published
+#pragma unroll
+                for (int q = 0; q < SCAN_LB; q++) {
+                    const uint64_t inv = __ballot((w[q] >> 62) == 0);
+                    if (first >= q) {
+                        const int lmax = (first - q) / SCAN_LB;
+                        const uint64_t need = (lmax >= 63) ? ~0ull : ((2ull << lmax) - 1);
+                        wait |= (inv & need) != 0;
+                    }
+                }
+                if (wait && ++polls < SCAN_MAX_POLLS) {
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                U part = (lane <= first) ? (U)scan_unpack<V>(w) : U(0);
+                U part = 0;
+#pragma unroll
+                for (int q = 0; q < SCAN_LB; q++)
+                    if (lane * SCAN_LB + q <= first) part += (U)scan_unpack<V>(w[q]);
 #pragma unroll
                 for (int d = 32; d > 0; d >>= 1) part += __shfl_xor(part, d, 64);
                 prefix += part;
-                if (inc || polls >= SCAN_MAX_POLLS) break;
-                j -= 64;
+                if (first < 64 * SCAN_LB || polls >= SCAN_MAX_POLLS) break;
+                j -= 64 * SCAN_LB;
             }
-            if (lane == 0) __hip_atomic_store(&state[2 + tile], SCAN_INC | scan_pack<V>((V)(prefix + agg)),
+            if (lane == 0) __hip_atomic_store(&state[1 + tile], SCAN_INC | scan_pack<V>((V)(prefix + agg)),
                                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         if (lane == 0) s_prefix = prefix;
@@ -463,16 +510,6 @@ k_scan(const V *__restrict__ a, const V *__restrict__ b, V *__restrict__ out, in
         const int64_t i = base + k * SCAN_NT + tid;
         if (i < n_out) out[i] = (V)buf[k * SCAN_NT + tid];
     }
-    if (wave == 0) {
-        // count the finished look-backs (release: this tile's INC before the
-        // count; acquire: the clear after every other's); the last one clears
-        // the state for the next scan
-        unsigned long long d = 0;
-        if (lane == 0) d = __hip_atomic_fetch_add(&state[1], 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        d = __shfl(d, 0, 64);
-        if (d == (unsigned long long)(nblk - 1))
-            for (int i = lane; i < nblk + 2; i += 64) state[i] = 0ull;
-    }
 }
 
 __global__ void k_scan_clear(unsigned long long *__restrict__ state, int64_t n)
@@ -483,12 +520,17 @@ __global__ void k_scan_clear(unsigned long long *__restrict__ state, int64_t n)
 
 template <typename V>
 static hipError_t launch_scan(const V *a, const V *b, V *out, int64_t n_in, int64_t n_out,
-                              unsigned long long *state, hipStream_t s)
+                              unsigned long long *state, unsigned long long *clear, int64_t n_clear, hipStream_t s)
 {
     if (n_out <= 0) return hipSuccess;
     if (n_in < 0 || n_in > n_out || n_out >= (int64_t(1) << 31)) return hipErrorInvalidValue;
-    const int64_t nblk = (n_out + SCAN_TILE - 1) / SCAN_TILE;
-    k_scan<V><<<(unsigned)nblk, SCAN_NT, 0, s>>>(a, b, out, n_in, n_out, state);
+    if (n_out <= SCAN_BIG) {
+        const int64_t nblk = (n_out + SCAN_NT * 8 - 1) / (SCAN_NT * 8);
+        k_scan<V, 8><<<(unsigned)nblk, SCAN_NT, 0, s>>>(a, b, out, n_in, n_out, state, clear, n_clear);
+    } else {
+        const int64_t nblk = (n_out + SCAN_NT * 16 - 1) / (SCAN_NT * 16);
+        k_scan<V, 16><<<(unsigned)nblk, SCAN_NT, 0, s>>>(a, b, out, n_in, n_out, state, clear, n_clear);
+    }
     return hipGetLastError();
 }
 
@@ -502,27 +544,32 @@ hipError_t clear_scan_state(unsigned long long *state, size_t words, hipStream_t
 hipError_t scan_i32_with_state(const int *a, const int *b, int *out, int64_t n_in, int64_t n_out,
                                unsigned long long *state, hipStream_t s)
 {
-    return launch_scan<int>(a, b, out, n_in, n_out, state, s);
+    return launch_scan<int>(a, b, out, n_in, n_out, state, nullptr, 0, s);
 }
 
-// The look-back state of scans on this thread's open workspace: kept per
-// (device, stream) cache and left zeroed by every scan; in a fixed workspace
-// (nothing may be allocated) or with no workspace open, carved or allocated
-// for the call and cleared by a kernel first.
+// The look-back state of scans on this thread's open workspace: two states
+// per (device, stream) cache, used in turn, each scan clearing the one the
+// previous scan used; in a fixed workspace (nothing may be allocated) or with
+// no workspace open, carved or allocated for the call and cleared first.
 template <typename V>
 static hipError_t exclusive_scan(const V *a, const V *b, V *out, int64_t n_in, int64_t n_out, hipStream_t s)
 {
     if (n_out <= 0) return hipSuccess;
     const size_t words = scan_state_words(n_out);
     Workspace *ws = Workspace::current();
-    unsigned long long *state = nullptr;
-    hipError_t e = ws ? ws->scan_state(words, &state) : hipSuccess;
-    if (e != hipSuccess) return e;
-    if (state) return launch_scan<V>(a, b, out, n_in, n_out, state, s);
+    if (ws) {
+        ScanState st;
+        hipError_t e = ws->scan_state(words, &st);
+        if (e != hipSuccess) return e;
+        e = launch_scan<V>(a, b, out, n_in, n_out, st.state, st.clear, (int64_t)st.n_clear, s);
+        if (e != hipSuccess) ws->scan_failed();
+        return e;
+    }
     DevBuf<unsigned long long> tmp;
-    if ((e = tmp.alloc(words, s)) != hipSuccess) return e;
+    hipError_t e = tmp.alloc(words, s);
+    if (e != hipSuccess) return e;
     if ((e = clear_scan_state(tmp.ptr, words, s)) != hipSuccess) return e;
-    return launch_scan<V>(a, b, out, n_in, n_out, tmp.ptr, s);
+    return launch_scan<V>(a, b, out, n_in, n_out, tmp.ptr, nullptr, 0, s);
 }
 
 hipError_t exclusive_scan_i32(const int *in, int *out, int64_t n, hipStream_t s)
