@@ -1743,14 +1743,20 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     if (P.n_tiles >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "too many tiles");
 
     DevBuf<unsigned long long> counters;
+    DevBuf<int> cnt2;
     CBX_HIP(P.setup.alloc(n_lines, st));
     CBX_HIP(P.flags.alloc(1, st));
     CBX_HIP(counters.alloc(2, st));
     CBX_HIP(P.tile_cnt.alloc(P.n_tiles + 1, st));
     CBX_HIP(P.tile_off.alloc(P.n_tiles + 1, st));
-    CBX_HIP(hipMemsetAsync(P.flags.ptr, 0, sizeof(int), st));
-    CBX_HIP(hipMemsetAsync(counters.ptr, 0, 2 * sizeof(unsigned long long), st));
-    CBX_HIP(hipMemsetAsync(P.tile_cnt.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
+    CBX_HIP(cnt2.alloc(P.n_tiles + 1, st));
+    CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
+    {
+        // every counter of the pass, cleared by one kernel
+        void *const zp[5] = {P.flags.ptr, counters.ptr, P.tile_cnt.ptr, cnt2.ptr, P.tile_cur.ptr};
+        const int64_t zw[5] = {1, 4, P.n_tiles + 1, P.n_tiles + 1, P.n_tiles};
+        CBX_HIP(zero_words(st, 5, zp, zw));
+    }
     const int blk = 256;
     k_setup_lines<T, I><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                P.setup.ptr, counters.ptr, P.flags.ptr);
@@ -1769,9 +1775,6 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     P.n_segs = (int64_t)slots;
     if (P.n_segs == 0) return CBX_OK;
     CBX_HIP(P.segs.alloc(P.n_segs, st));
-    DevBuf<int> cnt2;
-    CBX_HIP(cnt2.alloc(P.n_tiles + 1, st));
-    CBX_HIP(hipMemsetAsync(cnt2.ptr, 0, sizeof(int) * (P.n_tiles + 1), st));
     {
         KernelTimer timer("segments_walk", st);
         if (n_lines <= WALK_RING_LINES)
@@ -1797,8 +1800,6 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     if (n_items < 0) return fail(CBX_ERUNTIME, "tile item count overflow");
     P.n_items = n_items;
     CBX_HIP(P.pairs.alloc(P.n_items, st));
-    CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
-    CBX_HIP(hipMemsetAsync(P.tile_cur.ptr, 0, sizeof(int) * P.n_tiles, st));
     k_bin_slots<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr, P.tile_off.ptr,
                                                             P.tile_cnt.ptr, P.tile_cur.ptr, P.pairs.ptr, counters.ptr,
                                                             P.n_tiles, P.n_items, P.flags.ptr);

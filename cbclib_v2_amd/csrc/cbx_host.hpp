@@ -163,6 +163,11 @@ hipError_t clear_scan_state(unsigned long long *state, size_t words, hipStream_t
 hipError_t scan_i32_with_state(const int *a, const int *b, int *out, int64_t n_in, int64_t n_out,
                                unsigned long long *state, hipStream_t s);
 
+// Clears n (at most ZERO_MAX) buffers of words[i] 4-byte words with one
+// kernel on stream s (a kernel, not memsets: usable under graph capture).
+constexpr int ZERO_MAX = 6;
+hipError_t zero_words(hipStream_t s, int n, void *const *p, const int64_t *words);
+
 inline int grid_for(int64_t n, int block) { return (int)((n + block - 1) / block); }
 
 } // namespace cbx

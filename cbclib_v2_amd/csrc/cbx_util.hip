@@ -342,6 +342,44 @@ hipError_t read_device_values(hipStream_t s, int n, const void *const *src, cons
 }
 
 // ---------------------------------------------------------------------------
+// clearing several small buffers with one launch
+// ---------------------------------------------------------------------------
+struct ZeroList {
+    uint32_t *p[ZERO_MAX];
+    int64_t end[ZERO_MAX];   // running totals of the 4-byte word counts
+    int n;
+};
+
+__global__ void k_zero_words(ZeroList z)
+{
+    const int64_t total = z.end[z.n - 1];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int r = 0;
+        while (i >= z.end[r]) r++;
+        z.p[r][i - (r ? z.end[r - 1] : 0)] = 0u;
+    }
+}
+
+hipError_t zero_words(hipStream_t s, int n, void *const *p, const int64_t *words)
+{
+    if (n < 1 || n > ZERO_MAX) return hipErrorInvalidValue;
+    ZeroList z{};
+    int64_t t = 0;
+    for (int i = 0; i < n; i++) {
+        if (words[i] < 0 || (words[i] > 0 && !p[i])) return hipErrorInvalidValue;
+        z.p[i] = static_cast<uint32_t *>(p[i]);
+        t += words[i];
+        z.end[i] = t;
+    }
+    z.n = n;
+    if (t == 0) return hipSuccess;
+    const int64_t blocks = std::min<int64_t>((t + 255) / 256, 2048);
+    k_zero_words<<<(unsigned)blocks, 256, 0, s>>>(z);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // exclusive prefix sums: one kernel, single pass with decoupled look-back
 // ---------------------------------------------------------------------------
 // Workgroups take tiles in the order they start (a ticket counter), so every
