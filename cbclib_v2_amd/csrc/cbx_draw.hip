@@ -49,7 +49,6 @@ struct Segment {
     int32_t ncols;  // long-axis columns covered from c[axis] in the walk direction
     T Te, Ne;       // walk errors at that position
     int32_t lo, hi; // short-axis pixel range the fills can reach, clipped to the frame
-    int32_t rank[2]; // unordered plan: places in the item lists of its first two tiles
 };
 
 struct Geometry {
@@ -275,8 +274,9 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int kernel, 
 //   k_walk_lines    each wave takes slots for its lines' bounds with one
 //                   atomic (counters[1]), walks, writes the segments into
 //                   them (unused slots get ncols = 0) and counts tile items
-//                   (the count atomics also give each item its place);
-//   k_bin_slots     one lane per slot: writes the per-tile item lists.
+//                   (atomics the walk does not wait for);
+//   k_bin_slots     one lane per slot: writes the per-tile item lists, one
+//                   cursor atomic per (wave, tile).
 template <typename T, typename I>
 __global__ void __launch_bounds__(256)
 k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, const uint8_t *__restrict__ kernels,
@@ -312,7 +312,7 @@ template <typename T, int WALK_RING>
 __global__ void __launch_bounds__(256)
 k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
              int64_t slot_cap, unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt,
-             int *__restrict__ tile_cnt2, int *__restrict__ flags, int64_t *__restrict__ line_base)
+             int *__restrict__ flags, int64_t *__restrict__ line_base)
 {
     const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -359,21 +359,14 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
     // are written to memory WALK_RING at a time, all loads before all
     // stores: a store's data registers cannot be reused before the store
     // completes, so writing each segment from registers at its close would
-    // stall every close on the previous store.  A record's places (the
-    // returns of its count atomics) are filled in at the next close, so the
-    // walk runs on while the atomics are in flight.
+    // stall every close on the previous store.  The tile counts are atomics
+    // whose results the walk does not use (k_bin_slots places the items).
     // (ring words [slot][word][thread]: a record moves as 32-bit words with
     // constant indices, so it stays in registers)
     constexpr int NW = (int)(sizeof(Segment<T>) / 4);
     static_assert(sizeof(Segment<T>) % 8 == 0, "records move as 64-bit pairs");
     __shared__ uint32_t ring[WALK_RING][NW][256];
     const int t = threadIdx.x;
-    int r0 = 0, r1 = 0;
-    auto put_ranks = [&](int slot) {
-        constexpr int RW = (int)(offsetof(Segment<T>, rank) / 4);
-        ring[slot][RW][t] = (uint32_t)r0;
-        ring[slot][RW + 1][t] = (uint32_t)r1;
-    };
     auto flush = [&](int first, int cnt) {   // records first .. first + cnt - 1, ring slots 0 .. cnt - 1
         uint32_t w[WALK_RING][NW];
 #pragma unroll
@@ -391,27 +384,18 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
         }
     };
     walk_segments(L, l, g, flags, [&](const Segment<T> &cur, int tl) {
-        if (nseg > 0) {
-            put_ranks((nseg - 1) % WALK_RING);
-            if (nseg % WALK_RING == 0) flush(nseg - WALK_RING, WALK_RING);
-        }
         uint32_t cw[NW];
         __builtin_memcpy(cw, &cur, sizeof(cw));
         const int slot = nseg % WALK_RING;
 #pragma unroll
         for (int k = 0; k < NW; k++) ring[slot][k][t] = cw[k];
-        // the counting atomics hand out each piece's place in its tile's list:
-        // the first two tiles of a segment in class 1 (kept in the segment),
-        // further ones in class 2 (placed by k_bin_slots)
         const int tb0 = cur.lo / g.tile, tb1 = cur.hi / g.tile;
-        r0 = atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb0)], 1);
-        if (tb1 > tb0) r1 = atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb0 + 1)], 1);
-        for (int tb = tb0 + 2; tb <= tb1; tb++) atomicAdd(&tile_cnt2[tile_index(g, L.frame, L.axis, tl, tb)], 1);
+        for (int tb = tb0; tb <= tb1; tb++) atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb)], 1);
         nseg++;
+        if (nseg % WALK_RING == 0) flush(nseg - WALK_RING, WALK_RING);
     });
-    if (nseg > 0) {
-        put_ranks((nseg - 1) % WALK_RING);
-        const int done = ((nseg - 1) / WALK_RING) * WALK_RING;   // records flushed at closes
+    if (nseg % WALK_RING) {
+        const int done = (nseg / WALK_RING) * WALK_RING;   // records flushed at closes
         flush(done, nseg - done);
     }
     if (nseg > ub) atomicOr(flags, FLAG_STUCK); // cannot happen: the bound holds for every walk
@@ -421,8 +405,8 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
 // Clears a plan's counters before a draw (a kernel, not a memset: a draw may
 // be captured in a HIP graph).
 __global__ void k_plan_reset(int64_t n_tiles, unsigned long long *__restrict__ counters, int *__restrict__ flags,
-                             int *__restrict__ cnt1, int *__restrict__ cnt2, int *__restrict__ cur,
-                             unsigned long long *__restrict__ scan_state, int64_t scan_words)
+                             int *__restrict__ cnt, int *__restrict__ cur, unsigned long long *__restrict__ scan_state,
+                             int64_t scan_words)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < scan_words) scan_state[i] = 0ull;
@@ -431,42 +415,67 @@ __global__ void k_plan_reset(int64_t n_tiles, unsigned long long *__restrict__ c
         counters[1] = 0ull;
         *flags = 0;
     }
-    if (i <= n_tiles) {
-        cnt1[i] = 0;
-        cnt2[i] = 0;
+    if (i < n_tiles) {
+        cnt[i] = 0;
+        cur[i] = 0;
     }
-    if (i < n_tiles) cur[i] = 0;
 }
 
-// Tile t's list: class-1 pieces at [0, cnt1[t]) by their ranks, then class 2.
+// Tile t's list: the lanes of a wave (64 consecutive slots) whose segments
+// reach one tile take their places with one cursor atomic, in lane order, so
+// a list holds runs of consecutive slots (the tile kernels' segment loads
+// coalesce) and same-address atomics are few.
 template <typename T>
 __global__ void __launch_bounds__(256)
 k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup, const Segment<T> *__restrict__ segs,
-            const int *__restrict__ tile_off, const int *__restrict__ cnt1, int *__restrict__ cur2,
-            int2 *__restrict__ pairs, const unsigned long long *__restrict__ counters, int64_t n_tiles,
-            int64_t item_cap, int *__restrict__ flags)
+            const int *__restrict__ tile_off, int *__restrict__ cur, int2 *__restrict__ pairs,
+            const unsigned long long *__restrict__ counters, int64_t n_tiles, int64_t item_cap,
+            int *__restrict__ flags)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
     // after a slot overflow some reserved slots were never written (stale
     // segments): bin nothing; otherwise the slots taken by the walk, and the
     // item total against the list capacity
     if (*flags & FLAG_OVERFLOW) return;
-    if (i >= n_slots || i >= (int64_t)counters[1]) return;
     if ((int64_t)tile_off[n_tiles] > item_cap) {
         if (i == 0) atomicOr(flags, FLAG_OVERFLOW);
         return;
     }
-    const Segment<T> sg = segs[i];
-    if (sg.ncols <= 0) return;
-    const int axis = setup[sg.line].axis, frame = setup[sg.line].frame;
-    const int tl = pick(sg.c, axis) / g.tile;
-    const int tb0 = sg.lo / g.tile, tb1 = sg.hi / g.tile;
-    const int2 item = make_int2((int)i, sg.line);
-    pairs[tile_off[tile_index(g, frame, axis, tl, tb0)] + sg.rank[0]] = item;
-    if (tb1 > tb0) pairs[tile_off[tile_index(g, frame, axis, tl, tb0 + 1)] + sg.rank[1]] = item;
-    for (int tb = tb0 + 2; tb <= tb1; tb++) {
-        const int64_t tid = tile_index(g, frame, axis, tl, tb);
-        pairs[tile_off[tid] + cnt1[tid] + atomicAdd(&cur2[tid], 1)] = item;
+    int np = 0, axis = 0, frame = 0, tl = 0, tb0 = 0;
+    int2 item = make_int2(0, 0);
+    if (i < n_slots && i < (int64_t)counters[1]) {
+        const Segment<T> sg = segs[i];
+        if (sg.ncols > 0) {
+            axis = setup[sg.line].axis;
+            frame = setup[sg.line].frame;
+            tl = pick(sg.c, axis) / g.tile;
+            tb0 = sg.lo / g.tile;
+            np = sg.hi / g.tile - tb0 + 1;
+            item = make_int2((int)i, sg.line);
+        }
+    }
+    const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    for (int p = 0; __ballot(p < np) != 0ull; p++) {
+        const bool act = p < np;
+        const int key = act ? (int)tile_index(g, frame, axis, tl, tb0 + p) : -1;
+        // group the active lanes by tile: one leader (lowest lane) per group
+        uint64_t rem = __ballot(act), grp = 0ull;
+        int leader = 0;
+        while (rem) {
+            const int l = __builtin_ctzll(rem);
+            const int k = __shfl(key, l, 64);
+            const uint64_t m = __ballot(key == k);
+            if (key == k) {
+                grp = m;
+                leader = l;
+            }
+            rem &= ~m;
+        }
+        int b = 0;
+        if (act && lane == leader) b = atomicAdd(&cur[key], __popcll(grp));
+        b = __shfl(b, leader, 64);
+        if (act) pairs[tile_off[key] + b + __popcll(grp & below)] = item;
     }
 }
 
@@ -1743,19 +1752,17 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     if (P.n_tiles >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "too many tiles");
 
     DevBuf<unsigned long long> counters;
-    DevBuf<int> cnt2;
     CBX_HIP(P.setup.alloc(n_lines, st));
     CBX_HIP(P.flags.alloc(1, st));
     CBX_HIP(counters.alloc(2, st));
-    CBX_HIP(P.tile_cnt.alloc(P.n_tiles + 1, st));
+    CBX_HIP(P.tile_cnt.alloc(P.n_tiles, st));
     CBX_HIP(P.tile_off.alloc(P.n_tiles + 1, st));
-    CBX_HIP(cnt2.alloc(P.n_tiles + 1, st));
     CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
     {
         // every counter of the pass, cleared by one kernel
-        void *const zp[5] = {P.flags.ptr, counters.ptr, P.tile_cnt.ptr, cnt2.ptr, P.tile_cur.ptr};
-        const int64_t zw[5] = {1, 4, P.n_tiles + 1, P.n_tiles + 1, P.n_tiles};
-        CBX_HIP(zero_words(st, 5, zp, zw));
+        void *const zp[4] = {P.flags.ptr, counters.ptr, P.tile_cnt.ptr, P.tile_cur.ptr};
+        const int64_t zw[4] = {1, 4, P.n_tiles, P.n_tiles};
+        CBX_HIP(zero_words(st, 4, zp, zw));
     }
     const int blk = 256;
     k_setup_lines<T, I><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
@@ -1778,17 +1785,15 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     {
         KernelTimer timer("segments_walk", st);
         if (n_lines <= WALK_RING_LINES)
-            k_walk_lines<T, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                       counters.ptr, P.tile_cnt.ptr, cnt2.ptr,
-                                                                       P.flags.ptr, line_base);
+            k_walk_lines<T, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(
+                n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
         else
-            k_walk_lines<T, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                       counters.ptr, P.tile_cnt.ptr, cnt2.ptr,
-                                                                       P.flags.ptr, line_base);
+            k_walk_lines<T, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(
+                n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
     }
     CBX_HIP(hipGetLastError());
-    // tile lists: offsets of cnt1 + cnt2 (one scan kernel)
-    CBX_HIP(exclusive_scan_total_i32(P.tile_cnt.ptr, cnt2.ptr, P.tile_off.ptr, P.n_tiles, st));
+    // tile list offsets (tile_off[n_tiles]: the item total)
+    CBX_HIP(exclusive_scan_total_i32(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, P.n_tiles, st));
     {
         const void *src[2] = {P.tile_off.ptr + P.n_tiles, P.flags.ptr};
         const int nb[2] = {4, 4};
@@ -1801,8 +1806,8 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     P.n_items = n_items;
     CBX_HIP(P.pairs.alloc(P.n_items, st));
     k_bin_slots<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr, P.tile_off.ptr,
-                                                            P.tile_cnt.ptr, P.tile_cur.ptr, P.pairs.ptr, counters.ptr,
-                                                            P.n_tiles, P.n_items, P.flags.ptr);
+                                                            P.tile_cur.ptr, P.pairs.ptr, counters.ptr, P.n_tiles,
+                                                            P.n_items, P.flags.ptr);
     CBX_HIP(hipGetLastError());
     return CBX_OK;
 }
@@ -2397,7 +2402,6 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
 struct cbx_plan {
     cbx::SegmentPlan<float> P; // capacities in P.n_segs (slots) and P.n_items
     cbx::DevBuf<unsigned long long> counters;
-    cbx::DevBuf<int> cnt2;
     cbx::DevBuf<unsigned long long> scan_state; // the tile-offset scan's look-back state
     int64_t n_lines_max = 0;
     int64_t bytes = 0;
@@ -2464,12 +2468,11 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
     take(P.setup.alloc(n_lines, st));
     take(P.flags.alloc(1, st));
     take(P.segs.alloc(slot_cap, st));
-    take(P.tile_cnt.alloc(P.n_tiles + 1, st));
+    take(P.tile_cnt.alloc(P.n_tiles, st));
     take(P.tile_off.alloc(P.n_tiles + 1, st));
     take(P.tile_cur.alloc(P.n_tiles, st));
     take(P.pairs.alloc(item_cap, st));
     take(pl->counters.alloc(2, st));
-    take(pl->cnt2.alloc(P.n_tiles + 1, st));
     const size_t scan_words = scan_state_words(P.n_tiles + 1);
     take(pl->scan_state.alloc(scan_words, st));
     if (e == hipSuccess) e = clear_scan_state(pl->scan_state.ptr, scan_words, st);
@@ -2478,7 +2481,7 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
         return fail(CBX_ERUNTIME, std::string("plan allocation: ") + hipGetErrorString(e));
     }
     pl->bytes = (int64_t)(sizeof(LineSetup<float>) * n_lines + sizeof(Segment<float>) * slot_cap +
-                          sizeof(int2) * item_cap + 3 * sizeof(int) * (P.n_tiles + 1) +
+                          sizeof(int2) * item_cap + 3 * sizeof(int) * P.n_tiles + sizeof(int) +
                           sizeof(unsigned long long) * scan_words);
     *out_plan = pl;
     return CBX_OK;
@@ -2497,25 +2500,22 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
     const Geometry &g = P.g;
     const int64_t reset_n = std::max<int64_t>(P.n_tiles + 1, (int64_t)pl->scan_state.n);
     k_plan_reset<<<grid_for(reset_n, blk), blk, 0, st>>>(P.n_tiles, pl->counters.ptr, P.flags.ptr, P.tile_cnt.ptr,
-                                                         pl->cnt2.ptr, P.tile_cur.ptr, pl->scan_state.ptr,
+                                                         P.tile_cur.ptr, pl->scan_state.ptr,
                                                          (int64_t)pl->scan_state.n);
     k_setup_lines<float, int32_t><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                          P.setup.ptr, pl->counters.ptr, P.flags.ptr);
     if (n_lines <= WALK_RING_LINES)
-        k_walk_lines<float, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                       pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
-                                                                       P.flags.ptr, nullptr);
+        k_walk_lines<float, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(
+            n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr, nullptr);
     else
-        k_walk_lines<float, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                       pl->counters.ptr, P.tile_cnt.ptr, pl->cnt2.ptr,
-                                                                       P.flags.ptr, nullptr);
+        k_walk_lines<float, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(
+            n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr, nullptr);
     // the plan's own scan state (cleared by k_plan_reset above)
-    CBX_HIP(scan_i32_with_state(P.tile_cnt.ptr, pl->cnt2.ptr, P.tile_off.ptr, P.n_tiles, P.n_tiles + 1,
+    CBX_HIP(scan_i32_with_state(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, P.n_tiles, P.n_tiles + 1,
                                 pl->scan_state.ptr, st));
     k_bin_slots<float><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr,
-                                                                P.tile_off.ptr, P.tile_cnt.ptr, P.tile_cur.ptr,
-                                                                P.pairs.ptr, pl->counters.ptr, P.n_tiles, P.n_items,
-                                                                P.flags.ptr);
+                                                                P.tile_off.ptr, P.tile_cur.ptr, P.pairs.ptr,
+                                                                pl->counters.ptr, P.n_tiles, P.n_items, P.flags.ptr);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(st, &cap);
     if (cap == hipStreamCaptureStatusNone) {
