@@ -188,32 +188,32 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
         cur.Te = s.Te;
         cur.Ne = s.Ne;
         cur.ncols = (sa > 0) ? (b_hi - c0) : (c0 - b_lo + 1);
+        // one exit test per step (the block [b_lo, b_hi) lies inside the
+        // frame, so leaving it covers leaving the frame); why the block
+        // ended is sorted out after the loop, in the order the checks have
+        // in a plain loop: the budget, the end of the walk, the frame edge,
+        // else the next block.  Branch-free updates keep the step short.
         for (;; it++) {
-            if (it == budget) {
-                atomicOr(flags, FLAG_STUCK);
-                stop = true;
-                break;
-            }
-            if (walk_done(L, s)) {
-                stop = true;
-                break;
-            }
             const int c = pick(s.c, a);
-            if ((sa > 0) ? (c >= da) : (c < 0)) {
-                stop = true;
-                break;
-            }
-            if (c < b_lo || c >= b_hi) break; // the next block: close, then open there
+            if ((it == budget) | walk_done(L, s) | (c < b_lo) | (c >= b_hi)) break;
             bool nx, ny;
             walk_next(L, s.Ne, nx, ny);
-            if ((a == 0) ? nx : ny) {
-                nem++;
-                const int sb = pick(s.c, b);
-                smin = min(smin, sb);
-                smax = max(smax, sb);
-                maxN = fmax(maxN, fabs((double)s.Ne));
-            }
+            const bool em = (a == 0) ? nx : ny;
+            const int sb = pick(s.c, b);
+            nem += em ? 1 : 0;
+            smin = em ? min(smin, sb) : smin;
+            smax = em ? max(smax, sb) : smax;
+            maxN = em ? fmax(maxN, fabs((double)s.Ne)) : maxN;
             walk_step(L, s, nx, ny);
+        }
+        if (it == budget) {
+            atomicOr(flags, FLAG_STUCK);
+            stop = true;
+        } else if (walk_done(L, s)) {
+            stop = true;
+        } else {
+            const int c = pick(s.c, a);
+            if ((sa > 0) ? (c >= da) : (c < 0)) stop = true;
         }
         close();
     }
