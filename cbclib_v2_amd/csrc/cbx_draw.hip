@@ -126,13 +126,13 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
     s.Ne = L.e0[1];
 
     int b_lo = 0, b_hi = 0, tl = 0, nem = 0, smin = 0, smax = 0;
-    double maxN = 0.0;
+    T maxN = T(0);   // the largest |N| at an emitted column (exact in T)
     Segment<T> cur;
 
     auto close = [&]() {
         if (nem == 0) return;
         // fills stop once |N| / len >= w; |N| grows by |t_a| per short step
-        int R = (int)((wl + maxN) / ta) + 2;
+        int R = (int)((wl + (double)maxN) / ta) + 2;
         int lo = max(smin - R, 0), hi = min(smax + R, db - 1);
         if (lo > hi) return;
         cur.lo = lo;
@@ -179,7 +179,7 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
         b_hi = min(b_lo + g.seg, da);
         tl = c0 / g.tile;
         nem = 0;
-        maxN = 0.0;
+        maxN = T(0);
         smin = INT_MAX;
         smax = INT_MIN;
         cur.line = (int32_t)l;
@@ -188,14 +188,20 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
         cur.Te = s.Te;
         cur.Ne = s.Ne;
         cur.ncols = (sa > 0) ? (b_hi - c0) : (c0 - b_lo + 1);
-        // one exit test per step (the block [b_lo, b_hi) lies inside the
-        // frame, so leaving it covers leaving the frame); why the block
-        // ended is sorted out after the loop, in the order the checks have
-        // in a plain loop: the budget, the end of the walk, the frame edge,
-        // else the next block.  Branch-free updates keep the step short.
-        for (;; it++) {
-            const int c = pick(s.c, a);
-            if ((it == budget) | walk_done(L, s) | (c < b_lo) | (c >= b_hi)) break;
+        // One exit test per step: the long-axis coordinate moves by 0 or sa
+        // per step and starts inside the block [b_lo, b_hi), which lies
+        // inside the frame, so the block (and the frame) is left exactly at
+        // c_end.  Steps are counted per block in 32 bits (at most two per
+        // column for a valid walk: kmax is never reached) and added to the
+        // walk's budget after it.  Why the block ended is sorted out after
+        // the loop, in the order the checks have in a plain loop: the step
+        // budget, the end of the walk, the frame edge, else the next block.
+        // Branch-free updates keep the step short.
+        const int c_end = (sa > 0) ? b_hi : b_lo - 1;
+        const int kmax = 4 * g.seg + 8;
+        int k = 0;
+        for (;; k++) {
+            if ((k == kmax) | (pick(s.c, a) == c_end) | walk_done(L, s)) break;
             bool nx, ny;
             walk_next(L, s.Ne, nx, ny);
             const bool em = (a == 0) ? nx : ny;
@@ -203,10 +209,11 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
             nem += em ? 1 : 0;
             smin = em ? min(smin, sb) : smin;
             smax = em ? max(smax, sb) : smax;
-            maxN = em ? fmax(maxN, fabs((double)s.Ne)) : maxN;
+            maxN = em ? fmax(maxN, m_abs(s.Ne)) : maxN;
             walk_step(L, s, nx, ny);
         }
-        if (it == budget) {
+        it += k;
+        if (k == kmax || it >= budget) {
             atomicOr(flags, FLAG_STUCK);
             stop = true;
         } else if (walk_done(L, s)) {
