@@ -258,7 +258,7 @@ def workload_c4(args):
     target rendered from end points perturbed by N(0, 0.5)."""
     import numpy as np
     import torch
-    from cbclib_v2_amd import _native, autograd
+    from cbclib_v2_amd import _native, autograd, batch
     from tests.lines_gen import random_segments
     dev = torch.device('cuda', 0)
     F, H, W, per = 32, 1024, 1024, 5000
@@ -271,16 +271,33 @@ def workload_c4(args):
     target = autograd.render(torch.tensor(pert, dtype=torch.float32, device=dev), (F, H, W), idxs,
                              kernel='biweight').detach()
 
+    # the step through a plan (sized once on the lines, headroom for their
+    # drift): no host synchronisation, and the backward reuses the forward's
+    # segments and tile lists; the eager step (draw_lines + adjoint, each
+    # walking the lines) is timed beside it
+    plan = batch.Plan((F, H, W), lines.detach(), idxs, kernel='biweight', headroom=1.25)
+
     def step():
+        lines.grad = None
+        img = autograd.render_plan(plan, lines, idxs, kernel='biweight')
+        loss = ((img - target) ** 2).sum()
+        loss.backward()
+
+    def step_eager():
         lines.grad = None
         img = autograd.render(lines, (F, H, W), idxs, kernel='biweight')
         loss = ((img - target) ** 2).sum()
         loss.backward()
 
+    eager_ms = timed(step_eager, args.steps, args.warmup)
+    g_eager = lines.grad.clone()
     _native.timing_reset()
     _native.timing_enable(True)
     ms = timed(step, args.steps, args.warmup)
     _native.timing_enable(False)
+    plan.check()
+    same_grad = bool(torch.equal(lines.grad, g_eager))
+    plan.close()
     adj_ms, adj_n = _native.timing_read('adjoint_tiles')
     N = F * per
     step_bytes = 12.0 * F * H * W + 68.0 * N
@@ -295,7 +312,10 @@ def workload_c4(args):
             "config": {"workload": f"forward + adjoint: {F} frames of {H}x{W} f32, {N} segments ({per}/frame, "
                                    "C2 distributions, seed 3), biweight, loss sum((render - target)^2)"},
             "step_GBs": round(step_bytes / (ms * 1e-3) / 1e9, 2), "step_bytes": step_bytes,
-            "adjoint_tiles_ms": round(adj_ms / max(adj_n, 1), 4)}
+            "adjoint_tiles_ms": round(adj_ms / max(adj_n, 1), 4), "path": "batch.Plan + autograd.render_plan",
+            "eager": {"ms_per_step": round(eager_ms, 4), "frames_per_s": round(F / (eager_ms * 1e-3), 2),
+                      "path": "autograd.render (draw_lines + draw_lines_adjoint)"},
+            "grad_equals_eager": same_grad}
 
 
 def workload_c5(args):
@@ -740,7 +760,8 @@ def main() -> None:
         c4 = workload_c4(args)
         result["c4"] = {"fwd_adjoint_frames_per_s": c4["value"],
                         **{k: c4[k] for k in ("ms_per_step", "Mlines_per_s", "step_GBs", "step_bytes",
-                                              "adjoint_tiles_ms", "adjoint_roofline", "config")}}
+                                              "adjoint_tiles_ms", "adjoint_roofline", "config", "path", "eager",
+                                              "grad_equals_eager")}}
     print(json.dumps(result))
 
 

@@ -84,6 +84,42 @@ class DrawLinesFunction(torch.autograd.Function):
         return g.reshape(ctx.lines_shape), None, None, None, None, None, None
 
 
+class PlanDrawFunction(torch.autograd.Function):
+    """image = plan.draw(zeros(plan.shape), lines, idxs, max_val, kernel);
+    the backward is ``plan.adjoint`` on the forward's segments."""
+
+    @staticmethod
+    def forward(ctx, lines: torch.Tensor, idxs: Optional[torch.Tensor], plan, max_val: float, kernel: str):
+        ln = lines.detach().to(torch.float32).contiguous().reshape(-1, 5)
+        ix = None if idxs is None else idxs.to(torch.int32).contiguous()
+        out = torch.zeros(plan.shape, dtype=torch.float32, device=ln.device)
+        plan.draw(out, ln, ix, max_val=max_val, kernel=kernel)
+        ctx.save_for_backward(ln, ix if ix is not None else torch.empty(0))
+        ctx.plan, ctx.draw_no, ctx.has_idxs = plan, plan.draws, ix is not None
+        ctx.max_val, ctx.kernel, ctx.lines_shape = max_val, kernel, lines.shape
+        return out
+
+    @staticmethod
+    def backward(ctx, grad_out: torch.Tensor):
+        ln, ix = ctx.saved_tensors
+        if ctx.plan.draws != ctx.draw_no:
+            raise RuntimeError("the plan drew again between this forward and its backward")
+        g = ctx.plan.adjoint(grad_out.to(torch.float32), ln, ix if ctx.has_idxs else None, ctx.max_val,
+                             ctx.kernel)
+        return g.reshape(ctx.lines_shape).to(grad_out.dtype), None, None, None, None
+
+
+def render_plan(plan, lines: torch.Tensor, idxs: Optional[torch.Tensor] = None, max_val: float = 1.0,
+                kernel: str = 'biweight') -> torch.Tensor:
+    """Differentiable draw through a :class:`batch.Plan` (float32, sum
+    overlap): no host synchronisation, and the backward reuses the forward's
+    segments and tile lists instead of walking the lines again.  The
+    backward must run before the plan draws again; a draw that exceeded the
+    plan's capacity is raised by ``plan.check()``."""
+    device.require_rocm('render_plan')
+    return PlanDrawFunction.apply(lines, idxs, plan, float(max_val), kernel)
+
+
 def render(lines: torch.Tensor, shape: Sequence[int], idxs: Optional[torch.Tensor] = None, max_val: float = 1.0,
            kernel: str = 'biweight', tile: Optional[int] = None, seg: Optional[int] = None) -> torch.Tensor:
     """Differentiable draw_lines into a zero image of ``shape`` (sum overlap)."""

@@ -80,6 +80,7 @@ class Plan:
         t_lines, t_idxs, t_kern = _lines_args(lines, idxs, kernels)
         self.n_lines = t_lines.shape[0]
         self.kernel = kernel
+        self.draws = 0   # draws so far (render_plan's backward checks that none came between)
         self.opts = _native.DrawOpts(int(tile or 0), int(seg or 0), 0, int(threads or 0), 1 if exact_values else 0,
                                      int(chunk or 0), None)
         self._ptr = ctypes.c_void_p()
@@ -105,10 +106,34 @@ class Plan:
             raise ValueError(f"out must be a contiguous tensor of shape {self.shape}")
         t_lines, t_idxs, t_kern = _lines_args(lines, idxs, kernels)
         kid = _native.kernel_id(kernel or self.kernel)
+        self.draws += 1
         _native.check(_native.lib().cbx_plan_draw_f32(self._ptr, out.data_ptr(), t_lines.data_ptr(),
                                                       t_lines.shape[0], ptr(t_idxs), ptr(t_kern), float(max_val),
                                                       kid, _native.overlap_id(overlap), stream_ptr()))
         return out
+
+    def adjoint(self, grad_out: torch.Tensor, lines: Any, idxs: Any = None, max_val: float = 1.0,
+                kernel: Optional[str] = None) -> torch.Tensor:
+        """Gradient of ``<grad_out, draw(lines)>`` with respect to the lines'
+        (x0, y0, x1, y1, width), shape (N, 5), as ``autograd.adjoint`` but on
+        the segments and tile lists of the plan's last draw, which must have
+        drawn the same ``lines``, ``idxs`` and profile (one profile for every
+        line).  On the current stream, without waiting; errors of that draw
+        leave the gradient zero and are raised by :meth:`check`."""
+        if not self._ptr:
+            raise RuntimeError("the plan is closed")
+        if not isinstance(grad_out, torch.Tensor) or grad_out.dtype != torch.float32 or not grad_out.is_cuda:
+            raise TypeError("grad_out must be a float32 CUDA tensor")
+        if tuple(grad_out.shape) != self.shape:
+            raise ValueError(f"grad_out must have the plan's shape {self.shape}")
+        g = grad_out.contiguous()
+        t_lines, t_idxs, _ = _lines_args(lines, idxs, None)
+        res = torch.empty((t_lines.shape[0], 5), dtype=torch.float32, device=t_lines.device)
+        kid = _native.kernel_id(kernel or self.kernel)
+        _native.check(_native.lib().cbx_plan_adjoint_f32(self._ptr, g.data_ptr(), t_lines.data_ptr(),
+                                                         t_lines.shape[0], ptr(t_idxs), float(max_val), kid,
+                                                         res.data_ptr(), stream_ptr()))
+        return res
 
     def check(self) -> None:
         """Wait for the stream and raise the error of the last draw, if any

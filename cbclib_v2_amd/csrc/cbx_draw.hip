@@ -1493,7 +1493,8 @@ k_adjoint_tiles(const T *__restrict__ grad, const T *__restrict__ lines, Geometr
 
     const int tile = blockIdx.x;
     const int beg = tile_off[tile], end = tile_off[tile + 1];
-    if (beg == end) return;
+    // a plan whose last draw failed (capacity, bad idxs) has no valid lists
+    if (*flags || beg == end) return;
     const int per_frame = g.ntx * g.nty;
     const int64_t f = tile / per_frame;
     const int r = tile - (int)f * per_frame;
@@ -1584,12 +1585,16 @@ k_adjoint_tiles(const T *__restrict__ grad, const T *__restrict__ lines, Geometr
     }
 }
 
-// tiles each segment slot touches (0 for unused slots)
+// tiles each segment slot touches (0 for unused slots, and for the slots of a
+// plan beyond the ones its last walk took, counters[1], when counters is given)
 template <typename T>
-__global__ void k_slot_count(const Segment<T> *__restrict__ segs, int64_t n_segs, int tile, int64_t *__restrict__ cnt)
+__global__ void k_slot_count(const Segment<T> *__restrict__ segs, int64_t n_segs, int tile,
+                             const unsigned long long *__restrict__ counters, int64_t *__restrict__ cnt)
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n_segs) cnt[i] = (segs[i].ncols > 0) ? segs[i].hi / tile - segs[i].lo / tile + 1 : 0;
+    if (i >= n_segs) return;
+    const bool taken = !counters || i < (int64_t)counters[1];
+    cnt[i] = (taken && segs[i].ncols > 0) ? segs[i].hi / tile - segs[i].lo / tile + 1 : 0;
 }
 
 // Per-line sums of the partials, in slot order (deterministic).  A line's
@@ -1597,10 +1602,12 @@ __global__ void k_slot_count(const Segment<T> *__restrict__ segs, int64_t n_segs
 template <typename T>
 __global__ void k_line_grad(const int64_t *__restrict__ line_base, const LineSetup<T> *__restrict__ setup, Geometry g,
                             const int64_t *__restrict__ slot_base, const double *__restrict__ partial,
-                            int64_t n_lines, T *__restrict__ grad_lines)
+                            int64_t n_lines, T *__restrict__ grad_lines, const int *__restrict__ flags)
 {
     int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= n_lines) return;
+    // after a failed draw of a plan the slot ranges are not valid (the
+    // gradient stays zero; the error is reported by the plan's check)
+    if (l >= n_lines || *flags) return;
     const int ub = segment_bound(setup[l], g);
     const int64_t s0 = slot_base[line_base[l]], s1 = slot_base[line_base[l] + ub];
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -2373,7 +2380,7 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     CBX_HIP(scnt.alloc(P.n_segs + 1, st));
     CBX_HIP(sbase.alloc(P.n_segs + 1, st));
     CBX_HIP(hipMemsetAsync(scnt.ptr + P.n_segs, 0, sizeof(int64_t), st));
-    k_slot_count<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.segs.ptr, P.n_segs, tile, scnt.ptr);
+    k_slot_count<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.segs.ptr, P.n_segs, tile, nullptr, scnt.ptr);
     CBX_HIP(exclusive_scan_i64(scnt.ptr, sbase.ptr, P.n_segs + 1, st));
     // every slot is written by exactly one tile (P.n_items of them)
     DevBuf<double> partial;
@@ -2387,7 +2394,7 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     }
     CBX_HIP(hipGetLastError());
     k_line_grad<T><<<grid_for(n_lines, blk), blk, 0, st>>>(line_base.ptr, P.setup.ptr, P.g, sbase.ptr, partial.ptr,
-                                                          n_lines, grad_lines);
+                                                          n_lines, grad_lines, P.flags.ptr);
     CBX_HIP(hipGetLastError());
     int fl = 0;
     {
@@ -2410,8 +2417,16 @@ struct cbx_plan {
     cbx::SegmentPlan<float> P; // capacities in P.n_segs (slots) and P.n_items
     cbx::DevBuf<unsigned long long> counters;
     cbx::DevBuf<unsigned long long> scan_state; // the tile-offset scan's look-back state
+    cbx::DevBuf<int64_t> line_base;             // each line's first slot in the last draw
+    // the adjoint's buffers (allocated by the plan's first adjoint)
+    cbx::DevBuf<int64_t> scnt, sbase;           // tiles per slot, and their offsets
+    cbx::DevBuf<double> partial;                // five partial sums per (slot, tile)
+    cbx::DevBuf<unsigned long long> adj_state;  // the slot-offset scan's look-back state
     int64_t n_lines_max = 0;
     int64_t bytes = 0;
+    // the last draw: line count, profile (-1: per-line profiles)
+    int64_t last_lines = -1;
+    int last_kernel = -1;
     cbx_draw_opts opts{};
 };
 
@@ -2480,6 +2495,7 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
     take(P.tile_cur.alloc(P.n_tiles, st));
     take(P.pairs.alloc(item_cap, st));
     take(pl->counters.alloc(2, st));
+    take(pl->line_base.alloc(n_lines, st));
     const size_t scan_words = scan_state_words(P.n_tiles + 1);
     take(pl->scan_state.alloc(scan_words, st));
     if (e == hipSuccess) e = clear_scan_state(pl->scan_state.ptr, scan_words, st);
@@ -2487,7 +2503,7 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
         delete pl;
         return fail(CBX_ERUNTIME, std::string("plan allocation: ") + hipGetErrorString(e));
     }
-    pl->bytes = (int64_t)(sizeof(LineSetup<float>) * n_lines + sizeof(Segment<float>) * slot_cap +
+    pl->bytes = (int64_t)((sizeof(LineSetup<float>) + sizeof(int64_t)) * n_lines + sizeof(Segment<float>) * slot_cap +
                           sizeof(int2) * item_cap + 3 * sizeof(int) * P.n_tiles + sizeof(int) +
                           sizeof(unsigned long long) * scan_words);
     *out_plan = pl;
@@ -2512,11 +2528,13 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
     k_setup_lines<float, int32_t><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                          P.setup.ptr, pl->counters.ptr, P.flags.ptr);
     if (n_lines <= WALK_RING_LINES)
-        k_walk_lines<float, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(
-            n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr, nullptr);
+        k_walk_lines<float, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                       pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
+                                                                       pl->line_base.ptr);
     else
-        k_walk_lines<float, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(
-            n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr, nullptr);
+        k_walk_lines<float, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                       pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
+                                                                       pl->line_base.ptr);
     // the plan's own scan state (cleared by k_plan_reset above)
     CBX_HIP(scan_i32_with_state(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, P.n_tiles, P.n_tiles + 1,
                                 pl->scan_state.ptr, st));
@@ -2531,6 +2549,64 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
     } else {
         run_tiles<float>(out, P, kernels ? K_MIXED : kernel, overlap, max_val, &pl->opts, st);
     }
+    CBX_HIP(hipGetLastError());
+    pl->last_lines = n_lines;
+    pl->last_kernel = kernels ? -1 : kernel;
+    return CBX_OK;
+}
+
+// The adjoint of the plan's last draw, on its segments and tile lists (no
+// second walk): the same kernels as cbx_draw_lines_adjoint_f32, with no host
+// synchronisation; errors of the draw leave grad_lines zero and are reported
+// by plan_check.  The first call allocates the adjoint's buffers.
+static int plan_adjoint(cbx_plan *pl, const float *grad, const float *lines, int64_t n_lines, const int32_t *idxs,
+                        float max_val, int kernel, float *grad_lines, hipStream_t st)
+{
+    if (!pl) return fail(CBX_EVALUE, "plan must not be NULL");
+    SegmentPlan<float> &P = pl->P;
+    int rc = check_common(P.g.n_frames, P.g.H, P.g.W, n_lines, idxs != nullptr, kernel, 0);
+    if (rc) return rc;
+    if (n_lines == 0) return CBX_OK;
+    if (pl->last_kernel < 0 && pl->last_lines >= 0)
+        return fail(CBX_EVALUE, "the plan's last draw used per-line profiles: no adjoint");
+    if (n_lines != pl->last_lines || kernel != pl->last_kernel)
+        return fail(CBX_EVALUE, "the adjoint needs the line count and profile of the plan's last draw");
+    if (!pl->partial.ptr) {
+        hipError_t e = pl->scnt.alloc(P.n_segs, st);
+        if (e == hipSuccess) e = pl->sbase.alloc(P.n_segs + 1, st);
+        if (e == hipSuccess) e = pl->partial.alloc(5 * P.n_items, st);
+        if (e == hipSuccess) e = pl->adj_state.alloc(scan_state_words(P.n_segs + 1), st);
+        if (e != hipSuccess) {
+            pl->partial.release();
+            return fail(CBX_ERUNTIME, std::string("plan adjoint allocation: ") + hipGetErrorString(e));
+        }
+        pl->bytes += (int64_t)(sizeof(int64_t) * (2 * P.n_segs + 1) + sizeof(double) * 5 * P.n_items +
+                               sizeof(unsigned long long) * pl->adj_state.n);
+    }
+    {
+        // the gradient and the scan's state, cleared by one kernel
+        void *const zp[2] = {grad_lines, pl->adj_state.ptr};
+        const int64_t zw[2] = {5 * n_lines, 2 * (int64_t)pl->adj_state.n};
+        CBX_HIP(zero_words(st, 2, zp, zw));
+    }
+    if (kernel == K_RECTANGULAR) return CBX_OK;   // a flat profile: zero gradient
+    const int blk = 256;
+    k_slot_count<float><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.segs.ptr, P.n_segs, P.g.tile, pl->counters.ptr,
+                                                                 pl->scnt.ptr);
+    CBX_HIP(scan_i64_with_state(pl->scnt.ptr, pl->sbase.ptr, P.n_segs, P.n_segs + 1, pl->adj_state.ptr, st));
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    (void)hipStreamIsCapturing(st, &cap);
+    {
+        KernelTimer timer(cap == hipStreamCaptureStatusNone ? "adjoint_tiles" : nullptr, st);
+        switch (kernel) {
+        case K_BIWEIGHT: launch_adjoint<float, K_BIWEIGHT>(grad, lines, P, pl->sbase.ptr, max_val, pl->partial.ptr, st); break;
+        case K_GAUSSIAN: launch_adjoint<float, K_GAUSSIAN>(grad, lines, P, pl->sbase.ptr, max_val, pl->partial.ptr, st); break;
+        case K_PARABOLIC: launch_adjoint<float, K_PARABOLIC>(grad, lines, P, pl->sbase.ptr, max_val, pl->partial.ptr, st); break;
+        default: launch_adjoint<float, K_TRIANGULAR>(grad, lines, P, pl->sbase.ptr, max_val, pl->partial.ptr, st); break;
+        }
+    }
+    k_line_grad<float><<<grid_for(n_lines, blk), blk, 0, st>>>(pl->line_base.ptr, P.setup.ptr, P.g, pl->sbase.ptr,
+                                                               pl->partial.ptr, n_lines, grad_lines, P.flags.ptr);
     CBX_HIP(hipGetLastError());
     return CBX_OK;
 }
@@ -2565,6 +2641,12 @@ int cbx_plan_draw_f32(cbx_plan *plan, float *out, const float *lines, int64_t n_
                       const uint8_t *kernels, float max_val, int kernel, int overlap, void *stream)
 {
     return cbx::plan_draw(plan, out, lines, n_lines, idxs, kernels, max_val, kernel, overlap, (hipStream_t)stream);
+}
+
+int cbx_plan_adjoint_f32(cbx_plan *plan, const float *grad_out, const float *lines, int64_t n_lines,
+                         const int32_t *idxs, float max_val, int kernel, float *grad_lines, void *stream)
+{
+    return cbx::plan_adjoint(plan, grad_out, lines, n_lines, idxs, max_val, kernel, grad_lines, (hipStream_t)stream);
 }
 
 int cbx_plan_check(cbx_plan *plan, void *stream) { return cbx::plan_check(plan, (hipStream_t)stream); }

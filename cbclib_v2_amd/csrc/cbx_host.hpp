@@ -121,9 +121,10 @@ struct KernelTimer {
     const char *name;
     hipStream_t stream;
     hipEvent_t a = nullptr, b = nullptr;
+    // (a null name times nothing: for calls that may be captured in a graph)
     KernelTimer(const char *n, hipStream_t s) : name(n), stream(s)
     {
-        if (!timing_on() || no_alloc_mode()) return;
+        if (!n || !timing_on() || no_alloc_mode()) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
             a = b = nullptr;
             return;
@@ -161,6 +162,8 @@ hipError_t exclusive_scan_total_i64(const int64_t *a, int64_t *out, int64_t n, h
 size_t scan_state_words(int64_t n_out);
 hipError_t clear_scan_state(unsigned long long *state, size_t words, hipStream_t s);
 hipError_t scan_i32_with_state(const int *a, const int *b, int *out, int64_t n_in, int64_t n_out,
+                               unsigned long long *state, hipStream_t s);
+hipError_t scan_i64_with_state(const int64_t *a, int64_t *out, int64_t n_in, int64_t n_out,
                                unsigned long long *state, hipStream_t s);
 
 // Clears n (at most ZERO_MAX) buffers of words[i] 4-byte words with one

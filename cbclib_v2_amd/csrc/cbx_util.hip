@@ -585,6 +585,12 @@ hipError_t scan_i32_with_state(const int *a, const int *b, int *out, int64_t n_i
     return launch_scan<int>(a, b, out, n_in, n_out, state, nullptr, 0, s);
 }
 
+hipError_t scan_i64_with_state(const int64_t *a, int64_t *out, int64_t n_in, int64_t n_out,
+                               unsigned long long *state, hipStream_t s)
+{
+    return launch_scan<int64_t>(a, nullptr, out, n_in, n_out, state, nullptr, 0, s);
+}
+
 // The look-back state of scans on this thread's open workspace: two states
 // per (device, stream) cache, used in turn, each scan clearing the one the
 // previous scan used; in a fixed workspace (nothing may be allocated) or with

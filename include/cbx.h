@@ -316,6 +316,20 @@ CBX_API int cbx_plan_create_f32(cbx_plan **plan, int64_t n_frames, int64_t heigh
                                 const cbx_draw_opts *opts, double headroom, void *stream);
 CBX_API int cbx_plan_draw_f32(cbx_plan *plan, float *out, const float *lines, int64_t n_lines, const int32_t *idxs,
                               const uint8_t *kernels, float max_val, int kernel, int overlap, void *stream);
+/*
+ * cbx_plan_adjoint_f32: the adjoint of the plan's last draw (no reference
+ * counterpart; the gradient of cbx_draw_lines_adjoint_f32, see there), on
+ * that draw's segments and tile lists instead of a second walk, with no host
+ * synchronisation.  `lines`, `idxs`, `n_lines` and `kernel` must be those of
+ * the last cbx_plan_draw_f32 on this plan (a different count or profile, or a
+ * draw with per-line profiles -> CBX_EVALUE; other differences are the
+ * caller's to avoid); the draw's overlap and max_val do not matter.  After a
+ * failed draw grad_lines stays zero and cbx_plan_check reports the error.
+ * The first call allocates the adjoint's buffers (about 40 bytes per item of
+ * the plan's capacity).
+ */
+CBX_API int cbx_plan_adjoint_f32(cbx_plan *plan, const float *grad_out, const float *lines, int64_t n_lines,
+                                 const int32_t *idxs, float max_val, int kernel, float *grad_lines, void *stream);
 CBX_API int cbx_plan_check(cbx_plan *plan, void *stream);
 CBX_API int64_t cbx_plan_bytes(const cbx_plan *plan);
 CBX_API void cbx_plan_destroy(cbx_plan *plan);

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 import torch
 
-from cbclib_v2_amd import _native, batch, ndimage
+from cbclib_v2_amd import _native, autograd, batch, ndimage
 from tests.lines_gen import random_segments
 
 pytestmark = pytest.mark.gpu
@@ -112,9 +112,12 @@ def test_plan_errors_leave_out_untouched(cuda):
     big[:, 2] = 79.0
     big[:, 4] = 20.0
     plan.draw(out, big, t_idxs)
+    # the adjoint of the failed draw: a zero gradient, the error at the check
+    g_big = plan.adjoint(torch.ones(shape, device='cuda'), big, t_idxs)
     with pytest.raises(_native.CapacityError):
         plan.check()
     assert float((out - 0.5).abs().max()) == 0.0
+    assert float(g_big.abs().max()) == 0.0
     # an index outside the frames
     bad = t_idxs.clone()
     bad[3] = shape[0]
@@ -126,6 +129,55 @@ def test_plan_errors_leave_out_untouched(cuda):
     plan.draw(out, t_small, t_idxs)
     plan.check()
     assert float((out - 0.5).abs().max()) > 0.0
+    plan.close()
+
+
+@pytest.mark.parametrize('tile,seg', [(64, 8), (32, 16)])
+def test_plan_adjoint_matches_eager(cuda, tile, seg):
+    """The plan's adjoint runs on its last draw's segments and tile lists; a
+    line's partial sums are added in the same slot order as in the eager
+    adjoint, so the gradients are identical."""
+    for seed, kernel in enumerate(['biweight', 'gaussian', 'parabolic', 'triangular', 'rectangular']):
+        shape, lines, idxs, _ = frames_case(50 + seed)
+        t_lines, t_idxs = torch.from_numpy(lines).cuda(), torch.from_numpy(idxs).cuda()
+        g_img = torch.from_numpy(np.random.default_rng(seed).normal(size=shape).astype(np.float32)).cuda()
+        plan = batch.Plan(shape, t_lines, t_idxs, kernel=kernel, tile=tile, seg=seg, headroom=1.5)
+        out = torch.zeros(shape, device='cuda')
+        for rep in range(2):
+            if rep:   # a perturbed set through the same plan
+                t_lines = t_lines.clone()
+                t_lines[:, :4] += 0.6 * torch.randn_like(t_lines[:, :4])
+            ref = autograd.adjoint(g_img, t_lines, t_idxs, 1.2, kernel, tile, seg)
+            out.zero_()
+            plan.draw(out, t_lines, t_idxs, max_val=1.2)
+            got = plan.adjoint(g_img, t_lines, t_idxs, 1.2)
+            plan.check()
+            assert torch.equal(got, ref), f"{kernel} rep {rep}: max |diff| {float((got - ref).abs().max())}"
+        plan.close()
+
+
+def test_render_plan_matches_render(cuda):
+    shape, lines, idxs, _ = frames_case(60, F=2, H=128, W=112, n=150)
+    t_idxs = torch.from_numpy(idxs).cuda()
+    target = torch.rand(shape, device='cuda')
+    plan = batch.Plan(shape, torch.from_numpy(lines).cuda(), t_idxs, kernel='biweight', headroom=1.5)
+    res = []
+    for use_plan in (False, True):
+        x = torch.from_numpy(lines).cuda().requires_grad_(True)
+        img = autograd.render_plan(plan, x, t_idxs) if use_plan else autograd.render(x, shape, t_idxs)
+        ((img - target) ** 2).sum().backward()
+        res.append((img.detach(), x.grad))
+    plan.check()
+    assert torch.equal(res[0][0], res[1][0])
+    assert torch.equal(res[0][1], res[1][1])
+    # a draw between the forward and its backward; a different line count
+    x = torch.from_numpy(lines).cuda().requires_grad_(True)
+    img = autograd.render_plan(plan, x, t_idxs)
+    plan.draw(torch.zeros(shape, device='cuda'), x.detach(), t_idxs)
+    with pytest.raises(RuntimeError):
+        img.sum().backward()
+    with pytest.raises(ValueError):
+        plan.adjoint(torch.zeros(shape, device='cuda'), x.detach()[:10], t_idxs[:10])
     plan.close()
 
 
