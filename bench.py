@@ -324,7 +324,7 @@ def workload_c5(args):
     render) / 50; Adam (lr 0.05) for `--adam-steps` steps."""
     import numpy as np
     import torch
-    from cbclib_v2_amd import autograd
+    from cbclib_v2_amd import autograd, batch
     from tests.lines_gen import random_segments
     dev = torch.device('cuda', 0)
     F, H, W, per = 16, 2048, 2048, 10000
@@ -341,10 +341,13 @@ def workload_c5(args):
     target = torch.poisson(50.0 * clean, generator=gen) / 50.0
     lines = torch.nn.Parameter(torch.tensor(init, dtype=torch.float32, device=dev))
     opt = torch.optim.Adam([lines], lr=0.05, betas=(0.9, 0.999), eps=1e-8)
+    # one plan for the whole refinement (headroom for the lines' drift; a
+    # draw beyond it would be raised by the check after the loop)
+    plan = batch.Plan((F, H, W), lines.detach(), idxs, kernel='biweight', headroom=1.5)
 
     def step():
         opt.zero_grad(set_to_none=True)
-        img = autograd.render(lines, (F, H, W), idxs, kernel='biweight')
+        img = autograd.render_plan(plan, lines, idxs, kernel='biweight')
         loss = ((img - target) ** 2).sum()
         loss.backward()
         opt.step()
@@ -363,8 +366,11 @@ def workload_c5(args):
         step()
     torch.cuda.synchronize()
     sec = time.perf_counter() - t0
+    plan.check()
+    plan.close()
     N = F * per
     return {"metric": "Adam refinement steps/s (C5)", "value": round(n / sec, 3), "unit": "steps/s",
+            "path": "batch.Plan + autograd.render_plan",
             "ms_per_step": round(sec * 1e3 / n, 4), "adam_steps": n,
             "endpoint_rmse_px": {"start": round(r0, 4), "end": round(rmse(lines), 4)},
             "config": {"workload": f"refinement: {F} frames of {H}x{W} f32, {N} segments ({per}/frame, seed 4), "
