@@ -309,6 +309,18 @@ k_setup_lines(const T *__restrict__ lines, const I *__restrict__ idxs, const uin
     if (threadIdx.x == 0 && bsum) atomicAdd(&counters[0], bsum);
 }
 
+// The unordered plan's item list of tile t is split into LIST_SUBS sub-lists
+// by the binning wave (slot / 64 mod LIST_SUBS), so that the cursor atomics
+// of a dense tile spread over LIST_SUBS counters; the sub-lists are
+// contiguous: tile t's items are [tile_off[t * LIST_SUBS],
+// tile_off[(t + 1) * LIST_SUBS]).
+constexpr int LIST_SUBS = 8;
+
+__device__ __forceinline__ int64_t list_index(int64_t tile, int64_t slot)
+{
+    return tile * LIST_SUBS + ((slot >> 6) & (LIST_SUBS - 1));
+}
+
 // Records per thread in the walk's LDS ring: 4 (40 KB per 256 threads) for
 // launches of up to WALK_RING_LINES lines, whose waves are all resident at
 // once anyway; 2 (20 KB) above that, where more resident workgroups per CU
@@ -397,7 +409,8 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
 #pragma unroll
         for (int k = 0; k < NW; k++) ring[slot][k][t] = cw[k];
         const int tb0 = cur.lo / g.tile, tb1 = cur.hi / g.tile;
-        for (int tb = tb0; tb <= tb1; tb++) atomicAdd(&tile_cnt[tile_index(g, L.frame, L.axis, tl, tb)], 1);
+        for (int tb = tb0; tb <= tb1; tb++)
+            atomicAdd(&tile_cnt[list_index(tile_index(g, L.frame, L.axis, tl, tb), base + nseg)], 1);
         nseg++;
         if (nseg % WALK_RING == 0) flush(nseg - WALK_RING, WALK_RING);
     });
@@ -411,7 +424,7 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
 
 // Clears a plan's counters before a draw (a kernel, not a memset: a draw may
 // be captured in a HIP graph).
-__global__ void k_plan_reset(int64_t n_tiles, unsigned long long *__restrict__ counters, int *__restrict__ flags,
+__global__ void k_plan_reset(int64_t n_lists, unsigned long long *__restrict__ counters, int *__restrict__ flags,
                              int *__restrict__ cnt, int *__restrict__ cur, unsigned long long *__restrict__ scan_state,
                              int64_t scan_words)
 {
@@ -422,7 +435,7 @@ __global__ void k_plan_reset(int64_t n_tiles, unsigned long long *__restrict__ c
         counters[1] = 0ull;
         *flags = 0;
     }
-    if (i < n_tiles) {
+    if (i < n_lists) {
         cnt[i] = 0;
         cur[i] = 0;
     }
@@ -445,7 +458,7 @@ k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup,
     // segments): bin nothing; otherwise the slots taken by the walk, and the
     // item total against the list capacity
     if (*flags & FLAG_OVERFLOW) return;
-    if ((int64_t)tile_off[n_tiles] > item_cap) {
+    if ((int64_t)tile_off[n_tiles * LIST_SUBS] > item_cap) {
         if (i == 0) atomicOr(flags, FLAG_OVERFLOW);
         return;
     }
@@ -465,7 +478,7 @@ k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup,
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int p = 0; __ballot(p < np) != 0ull; p++) {
         const bool act = p < np;
-        const int key = act ? (int)tile_index(g, frame, axis, tl, tb0 + p) : -1;
+        const int key = act ? (int)list_index(tile_index(g, frame, axis, tl, tb0 + p), i) : -1;
         // group the active lanes by tile: one leader (lowest lane) per group
         uint64_t rem = __ballot(act), grp = 0ull;
         int leader = 0;
@@ -761,7 +774,7 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     const int tile = blockIdx.x;
     // (one round trip for the item range and the error flags: a plan's draw
     // reports errors afterwards and leaves `out` untouched then)
-    const int beg = tile_off[tile], end = tile_off[tile + 1];
+    const int beg = tile_off[tile * LIST_SUBS], end = tile_off[(tile + 1) * LIST_SUBS];
     const int fl = *flags;
     if (fl || beg == end) return;
     const int per_frame = g.ntx * g.nty;
@@ -1492,7 +1505,7 @@ k_adjoint_tiles(const T *__restrict__ grad, const T *__restrict__ lines, Geometr
     __shared__ ItemHdr hdr[NW][CH];
 
     const int tile = blockIdx.x;
-    const int beg = tile_off[tile], end = tile_off[tile + 1];
+    const int beg = tile_off[tile * LIST_SUBS], end = tile_off[(tile + 1) * LIST_SUBS];
     // a plan whose last draw failed (capacity, bad idxs) has no valid lists
     if (*flags || beg == end) return;
     const int per_frame = g.ntx * g.nty;
@@ -1763,19 +1776,20 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     g.nty = (int32_t)((H + tile - 1) / tile);
     P.n_lines = n_lines;
     P.n_tiles = n_frames * g.ntx * g.nty;
-    if (P.n_tiles >= (int64_t(1) << 31)) return fail(CBX_EVALUE, "too many tiles");
+    if (P.n_tiles >= (int64_t(1) << 31) / LIST_SUBS) return fail(CBX_EVALUE, "too many tiles");
+    const int64_t n_lists = P.n_tiles * LIST_SUBS;
 
     DevBuf<unsigned long long> counters;
     CBX_HIP(P.setup.alloc(n_lines, st));
     CBX_HIP(P.flags.alloc(1, st));
     CBX_HIP(counters.alloc(2, st));
-    CBX_HIP(P.tile_cnt.alloc(P.n_tiles, st));
-    CBX_HIP(P.tile_off.alloc(P.n_tiles + 1, st));
-    CBX_HIP(P.tile_cur.alloc(P.n_tiles, st));
+    CBX_HIP(P.tile_cnt.alloc(n_lists, st));
+    CBX_HIP(P.tile_off.alloc(n_lists + 1, st));
+    CBX_HIP(P.tile_cur.alloc(n_lists, st));
     {
         // every counter of the pass, cleared by one kernel
         void *const zp[4] = {P.flags.ptr, counters.ptr, P.tile_cnt.ptr, P.tile_cur.ptr};
-        const int64_t zw[4] = {1, 4, P.n_tiles, P.n_tiles};
+        const int64_t zw[4] = {1, 4, n_lists, n_lists};
         CBX_HIP(zero_words(st, 4, zp, zw));
     }
     const int blk = 256;
@@ -1806,10 +1820,10 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
                 n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
     }
     CBX_HIP(hipGetLastError());
-    // tile list offsets (tile_off[n_tiles]: the item total)
-    CBX_HIP(exclusive_scan_total_i32(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, P.n_tiles, st));
+    // list offsets (tile_off[n_lists]: the item total)
+    CBX_HIP(exclusive_scan_total_i32(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, n_lists, st));
     {
-        const void *src[2] = {P.tile_off.ptr + P.n_tiles, P.flags.ptr};
+        const void *src[2] = {P.tile_off.ptr + n_lists, P.flags.ptr};
         const int nb[2] = {4, 4};
         CBX_HIP(read_device_values(st, 2, src, nb, hv));
     }
@@ -2475,6 +2489,7 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
     g.nty = (int32_t)((H + tile - 1) / tile);
     P.n_lines = n_lines;
     P.n_tiles = n_frames * g.ntx * g.nty;
+    const int64_t n_lists = P.n_tiles * LIST_SUBS;
     P.n_segs = slot_cap;
     P.n_items = item_cap;
     pl->n_lines_max = n_lines;
@@ -2490,13 +2505,13 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
     take(P.setup.alloc(n_lines, st));
     take(P.flags.alloc(1, st));
     take(P.segs.alloc(slot_cap, st));
-    take(P.tile_cnt.alloc(P.n_tiles, st));
-    take(P.tile_off.alloc(P.n_tiles + 1, st));
-    take(P.tile_cur.alloc(P.n_tiles, st));
+    take(P.tile_cnt.alloc(n_lists, st));
+    take(P.tile_off.alloc(n_lists + 1, st));
+    take(P.tile_cur.alloc(n_lists, st));
     take(P.pairs.alloc(item_cap, st));
     take(pl->counters.alloc(2, st));
     take(pl->line_base.alloc(n_lines, st));
-    const size_t scan_words = scan_state_words(P.n_tiles + 1);
+    const size_t scan_words = scan_state_words(n_lists + 1);
     take(pl->scan_state.alloc(scan_words, st));
     if (e == hipSuccess) e = clear_scan_state(pl->scan_state.ptr, scan_words, st);
     if (e != hipSuccess) {
@@ -2504,7 +2519,7 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
         return fail(CBX_ERUNTIME, std::string("plan allocation: ") + hipGetErrorString(e));
     }
     pl->bytes = (int64_t)((sizeof(LineSetup<float>) + sizeof(int64_t)) * n_lines + sizeof(Segment<float>) * slot_cap +
-                          sizeof(int2) * item_cap + 3 * sizeof(int) * P.n_tiles + sizeof(int) +
+                          sizeof(int2) * item_cap + 3 * sizeof(int) * n_lists + sizeof(int) +
                           sizeof(unsigned long long) * scan_words);
     *out_plan = pl;
     return CBX_OK;
@@ -2521,8 +2536,9 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
     if (n_lines == 0) return CBX_OK;
     const int blk = 256;
     const Geometry &g = P.g;
-    const int64_t reset_n = std::max<int64_t>(P.n_tiles + 1, (int64_t)pl->scan_state.n);
-    k_plan_reset<<<grid_for(reset_n, blk), blk, 0, st>>>(P.n_tiles, pl->counters.ptr, P.flags.ptr, P.tile_cnt.ptr,
+    const int64_t n_lists = P.n_tiles * LIST_SUBS;
+    const int64_t reset_n = std::max<int64_t>(n_lists, (int64_t)pl->scan_state.n);
+    k_plan_reset<<<grid_for(reset_n, blk), blk, 0, st>>>(n_lists, pl->counters.ptr, P.flags.ptr, P.tile_cnt.ptr,
                                                          P.tile_cur.ptr, pl->scan_state.ptr,
                                                          (int64_t)pl->scan_state.n);
     k_setup_lines<float, int32_t><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
@@ -2536,8 +2552,8 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
                                                                        pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
                                                                        pl->line_base.ptr);
     // the plan's own scan state (cleared by k_plan_reset above)
-    CBX_HIP(scan_i32_with_state(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, P.n_tiles, P.n_tiles + 1,
-                                pl->scan_state.ptr, st));
+    CBX_HIP(scan_i32_with_state(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, n_lists, n_lists + 1, pl->scan_state.ptr,
+                                st));
     k_bin_slots<float><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr,
                                                                 P.tile_off.ptr, P.tile_cur.ptr, P.pairs.ptr,
                                                                 pl->counters.ptr, P.n_tiles, P.n_items, P.flags.ptr);
