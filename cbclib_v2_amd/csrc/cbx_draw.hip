@@ -450,7 +450,7 @@ __global__ void __launch_bounds__(256)
 k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup, const Segment<T> *__restrict__ segs,
             const int *__restrict__ tile_off, int *__restrict__ cur, int2 *__restrict__ pairs,
             const unsigned long long *__restrict__ counters, int64_t n_tiles, int64_t item_cap,
-            int *__restrict__ flags)
+            int *__restrict__ flags, int *__restrict__ slot_tiles)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63;
@@ -475,6 +475,8 @@ k_bin_slots(int64_t n_slots, Geometry g, const LineSetup<T> *__restrict__ setup,
             item = make_int2((int)i, sg.line);
         }
     }
+    // tiles per slot, for the adjoint's partial sums (0 for unused slots)
+    if (slot_tiles && i < n_slots) slot_tiles[i] = np;
     const uint64_t below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     for (int p = 0; __ballot(p < np) != 0ull; p++) {
         const bool act = p < np;
@@ -1495,7 +1497,7 @@ template <typename T, int K, int TILE, int SEG, int NT, int CH>
 __global__ void __launch_bounds__(NT)
 k_adjoint_tiles(const T *__restrict__ grad, const T *__restrict__ lines, Geometry g, const int *__restrict__ tile_off,
                 const int2 *__restrict__ pairs, const Segment<T> *__restrict__ segs,
-                const LineSetup<T> *__restrict__ setup, const int64_t *__restrict__ slot_base, T max_val,
+                const LineSetup<T> *__restrict__ setup, const int *__restrict__ slot_base, T max_val,
                 double *__restrict__ partial, int *__restrict__ flags)
 {
     constexpr int NW = NT / 64;
@@ -1589,7 +1591,7 @@ k_adjoint_tiles(const T *__restrict__ grad, const T *__restrict__ lines, Geometr
                 const Segment<T> &sg = segs[sid];
                 const int a = setup[hdr[wave][i].line].axis;
                 const int tb = (a == 0) ? ty : tx;
-                double *p = partial + 5 * (slot_base[sid] + (tb - sg.lo / g.tile));
+                double *p = partial + 5 * (int64_t)(slot_base[sid] + (tb - sg.lo / g.tile));
 #pragma unroll
                 for (int j = 0; j < 5; j++) p[j] = acc[j];
             }
@@ -1598,35 +1600,37 @@ k_adjoint_tiles(const T *__restrict__ grad, const T *__restrict__ lines, Geometr
     }
 }
 
-// tiles each segment slot touches (0 for unused slots, and for the slots of a
-// plan beyond the ones its last walk took, counters[1], when counters is given)
-template <typename T>
-__global__ void k_slot_count(const Segment<T> *__restrict__ segs, int64_t n_segs, int tile,
-                             const unsigned long long *__restrict__ counters, int64_t *__restrict__ cnt)
-{
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_segs) return;
-    const bool taken = !counters || i < (int64_t)counters[1];
-    cnt[i] = (taken && segs[i].ncols > 0) ? segs[i].hi / tile - segs[i].lo / tile + 1 : 0;
-}
+// Per-line sums of the partials (deterministic): LG_LANES lanes per line,
+// lane j adding the line's partials j, j + LG_LANES, ... in slot order, then a
+// fixed butterfly over the lanes.  A line's slots are [line_base[l],
+// line_base[l] + segment_bound); slot s's partials start at slot_base[s].
+constexpr int LG_LANES = 8;
 
-// Per-line sums of the partials, in slot order (deterministic).  A line's
-// slots are [line_base[l], line_base[l] + segment_bound).
 template <typename T>
-__global__ void k_line_grad(const int64_t *__restrict__ line_base, const LineSetup<T> *__restrict__ setup, Geometry g,
-                            const int64_t *__restrict__ slot_base, const double *__restrict__ partial,
-                            int64_t n_lines, T *__restrict__ grad_lines, const int *__restrict__ flags)
+__global__ void __launch_bounds__(256)
+k_line_grad(const int64_t *__restrict__ line_base, const LineSetup<T> *__restrict__ setup, Geometry g,
+            const int *__restrict__ slot_base, const double *__restrict__ partial, int64_t n_lines,
+            T *__restrict__ grad_lines, const int *__restrict__ flags)
 {
-    int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t l = t / LG_LANES;
+    const int j = (int)(t % LG_LANES);
     // after a failed draw of a plan the slot ranges are not valid (the
     // gradient stays zero; the error is reported by the plan's check)
-    if (l >= n_lines || *flags) return;
-    const int ub = segment_bound(setup[l], g);
-    const int64_t s0 = slot_base[line_base[l]], s1 = slot_base[line_base[l] + ub];
+    const bool ok = l < n_lines && *flags == 0;
     double acc[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int64_t s = s0; s < s1; s++)
+    if (ok) {
+        const int ub = segment_bound(setup[l], g);
+        const int64_t s0 = slot_base[line_base[l]], s1 = slot_base[line_base[l] + ub];
+        for (int64_t s = s0 + j; s < s1; s += LG_LANES)
 #pragma unroll
-        for (int j = 0; j < 5; j++) acc[j] += partial[5 * s + j];
+            for (int k = 0; k < 5; k++) acc[k] += partial[5 * s + k];
+    }
+#pragma unroll
+    for (int m = LG_LANES / 2; m > 0; m >>= 1)
+#pragma unroll
+        for (int k = 0; k < 5; k++) acc[k] += __shfl_xor(acc[k], m, LG_LANES);
+    if (!ok || j != 0) return;
     // (sum c dx - sum c ts dx, ..., sum c ts dx, ...): see pixel_grad
     grad_lines[5 * l + 0] = (T)(acc[0] - acc[2]);
     grad_lines[5 * l + 1] = (T)(acc[1] - acc[3]);
@@ -1648,6 +1652,7 @@ struct SegmentPlan {
     DevBuf<LineSetup<T>> setup;
     DevBuf<int> seg_cnt, seg_off, tile_cnt, tile_off, tile_cur, items, flags;
     DevBuf<int2> pairs; // unordered plan: per-tile (segment, line) lists
+    DevBuf<int> slot_tiles; // unordered plan, when asked for: tiles per segment slot (the adjoint's)
     DevBuf<Segment<T>> segs;
 };
 
@@ -1764,7 +1769,7 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int kern
 template <typename T, typename I>
 static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs, const uint8_t *kernels,
                                 int kernel, int64_t n_lines, int64_t n_frames, int64_t H, int64_t W, int tile,
-                                int seg, hipStream_t st, int64_t *line_base = nullptr)
+                                int seg, hipStream_t st, int64_t *line_base = nullptr, bool slot_tiles = false)
 {
     Geometry &g = P.g;
     g.n_frames = n_frames;
@@ -1833,9 +1838,10 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     if (n_items < 0) return fail(CBX_ERUNTIME, "tile item count overflow");
     P.n_items = n_items;
     CBX_HIP(P.pairs.alloc(P.n_items, st));
+    if (slot_tiles) CBX_HIP(P.slot_tiles.alloc(P.n_segs, st));
     k_bin_slots<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr, P.tile_off.ptr,
                                                             P.tile_cur.ptr, P.pairs.ptr, counters.ptr, P.n_tiles,
-                                                            P.n_items, P.flags.ptr);
+                                                            P.n_items, P.flags.ptr, P.slot_tiles.ptr);
     CBX_HIP(hipGetLastError());
     return CBX_OK;
 }
@@ -2351,7 +2357,7 @@ static int accumulate_batch(T *out, const T *lines, int64_t n_lines, const I *bi
 }
 
 template <typename T, int K>
-static void launch_adjoint(const T *grad, const T *lines, const SegmentPlan<T> &P, const int64_t *slot_base,
+static void launch_adjoint(const T *grad, const T *lines, const SegmentPlan<T> &P, const int *slot_base,
                            T max_val, double *partial, hipStream_t st)
 {
     constexpr int NT = 256;
@@ -2386,17 +2392,16 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     DevBuf<int64_t> line_base;
     CBX_HIP(line_base.alloc(n_lines, st));
     if ((rc = build_plan_unordered<T, I>(P, lines, idxs, nullptr, kernel, n_lines, n_frames, H, W, tile, seg, st,
-                                         line_base.ptr)))
+                                         line_base.ptr, true)))
         return rc;
     if (P.n_segs == 0) return CBX_OK;
     const int blk = 256;
-    DevBuf<int64_t> scnt, sbase;
-    CBX_HIP(scnt.alloc(P.n_segs + 1, st));
+    // each slot's first partial (tiles per slot from the binning), P.n_items
+    // (< 2^31) in all
+    DevBuf<int> sbase;
     CBX_HIP(sbase.alloc(P.n_segs + 1, st));
-    CBX_HIP(hipMemsetAsync(scnt.ptr + P.n_segs, 0, sizeof(int64_t), st));
-    k_slot_count<T><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.segs.ptr, P.n_segs, tile, nullptr, scnt.ptr);
-    CBX_HIP(exclusive_scan_i64(scnt.ptr, sbase.ptr, P.n_segs + 1, st));
-    // every slot is written by exactly one tile (P.n_items of them)
+    CBX_HIP(exclusive_scan_total_i32(P.slot_tiles.ptr, nullptr, sbase.ptr, P.n_segs, st));
+    // every (slot, tile) partial is written by exactly one tile
     DevBuf<double> partial;
     CBX_HIP(partial.alloc(5 * P.n_items, st));
     KernelTimer timer("adjoint_tiles", st);
@@ -2407,8 +2412,8 @@ static int adjoint_impl(const T *grad, int64_t n_frames, int64_t H, int64_t W, c
     default: launch_adjoint<T, K_TRIANGULAR>(grad, lines, P, sbase.ptr, max_val, partial.ptr, st); break;
     }
     CBX_HIP(hipGetLastError());
-    k_line_grad<T><<<grid_for(n_lines, blk), blk, 0, st>>>(line_base.ptr, P.setup.ptr, P.g, sbase.ptr, partial.ptr,
-                                                          n_lines, grad_lines, P.flags.ptr);
+    k_line_grad<T><<<grid_for(n_lines * LG_LANES, blk), blk, 0, st>>>(line_base.ptr, P.setup.ptr, P.g, sbase.ptr,
+                                                                      partial.ptr, n_lines, grad_lines, P.flags.ptr);
     CBX_HIP(hipGetLastError());
     int fl = 0;
     {
@@ -2432,8 +2437,9 @@ struct cbx_plan {
     cbx::DevBuf<unsigned long long> counters;
     cbx::DevBuf<unsigned long long> scan_state; // the tile-offset scan's look-back state
     cbx::DevBuf<int64_t> line_base;             // each line's first slot in the last draw
-    // the adjoint's buffers (allocated by the plan's first adjoint)
-    cbx::DevBuf<int64_t> scnt, sbase;           // tiles per slot, and their offsets
+    // the adjoint's buffers (allocated by the plan's first adjoint; the tiles
+    // per slot, P.slot_tiles, are written by every draw)
+    cbx::DevBuf<int> sbase;                     // each slot's first partial
     cbx::DevBuf<double> partial;                // five partial sums per (slot, tile)
     cbx::DevBuf<unsigned long long> adj_state;  // the slot-offset scan's look-back state
     int64_t n_lines_max = 0;
@@ -2505,6 +2511,7 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
     take(P.setup.alloc(n_lines, st));
     take(P.flags.alloc(1, st));
     take(P.segs.alloc(slot_cap, st));
+    take(P.slot_tiles.alloc(slot_cap, st));
     take(P.tile_cnt.alloc(n_lists, st));
     take(P.tile_off.alloc(n_lists + 1, st));
     take(P.tile_cur.alloc(n_lists, st));
@@ -2518,7 +2525,8 @@ static int plan_create(cbx_plan **out_plan, int64_t n_frames, int64_t H, int64_t
         delete pl;
         return fail(CBX_ERUNTIME, std::string("plan allocation: ") + hipGetErrorString(e));
     }
-    pl->bytes = (int64_t)((sizeof(LineSetup<float>) + sizeof(int64_t)) * n_lines + sizeof(Segment<float>) * slot_cap +
+    pl->bytes = (int64_t)((sizeof(LineSetup<float>) + sizeof(int64_t)) * n_lines +
+                          (sizeof(Segment<float>) + sizeof(int)) * slot_cap +
                           sizeof(int2) * item_cap + 3 * sizeof(int) * n_lists + sizeof(int) +
                           sizeof(unsigned long long) * scan_words);
     *out_plan = pl;
@@ -2556,7 +2564,8 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
                                 st));
     k_bin_slots<float><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.n_segs, g, P.setup.ptr, P.segs.ptr,
                                                                 P.tile_off.ptr, P.tile_cur.ptr, P.pairs.ptr,
-                                                                pl->counters.ptr, P.n_tiles, P.n_items, P.flags.ptr);
+                                                                pl->counters.ptr, P.n_tiles, P.n_items, P.flags.ptr,
+                                                                P.slot_tiles.ptr);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(st, &cap);
     if (cap == hipStreamCaptureStatusNone) {
@@ -2588,15 +2597,14 @@ static int plan_adjoint(cbx_plan *pl, const float *grad, const float *lines, int
     if (n_lines != pl->last_lines || kernel != pl->last_kernel)
         return fail(CBX_EVALUE, "the adjoint needs the line count and profile of the plan's last draw");
     if (!pl->partial.ptr) {
-        hipError_t e = pl->scnt.alloc(P.n_segs, st);
-        if (e == hipSuccess) e = pl->sbase.alloc(P.n_segs + 1, st);
+        hipError_t e = pl->sbase.alloc(P.n_segs + 1, st);
         if (e == hipSuccess) e = pl->partial.alloc(5 * P.n_items, st);
         if (e == hipSuccess) e = pl->adj_state.alloc(scan_state_words(P.n_segs + 1), st);
         if (e != hipSuccess) {
             pl->partial.release();
             return fail(CBX_ERUNTIME, std::string("plan adjoint allocation: ") + hipGetErrorString(e));
         }
-        pl->bytes += (int64_t)(sizeof(int64_t) * (2 * P.n_segs + 1) + sizeof(double) * 5 * P.n_items +
+        pl->bytes += (int64_t)(sizeof(int) * (P.n_segs + 1) + sizeof(double) * 5 * P.n_items +
                                sizeof(unsigned long long) * pl->adj_state.n);
     }
     {
@@ -2607,9 +2615,8 @@ static int plan_adjoint(cbx_plan *pl, const float *grad, const float *lines, int
     }
     if (kernel == K_RECTANGULAR) return CBX_OK;   // a flat profile: zero gradient
     const int blk = 256;
-    k_slot_count<float><<<grid_for(P.n_segs, blk), blk, 0, st>>>(P.segs.ptr, P.n_segs, P.g.tile, pl->counters.ptr,
-                                                                 pl->scnt.ptr);
-    CBX_HIP(scan_i64_with_state(pl->scnt.ptr, pl->sbase.ptr, P.n_segs, P.n_segs + 1, pl->adj_state.ptr, st));
+    CBX_HIP(scan_i32_with_state(P.slot_tiles.ptr, nullptr, pl->sbase.ptr, P.n_segs, P.n_segs + 1, pl->adj_state.ptr,
+                                st));
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     (void)hipStreamIsCapturing(st, &cap);
     {
@@ -2621,8 +2628,8 @@ static int plan_adjoint(cbx_plan *pl, const float *grad, const float *lines, int
         default: launch_adjoint<float, K_TRIANGULAR>(grad, lines, P, pl->sbase.ptr, max_val, pl->partial.ptr, st); break;
         }
     }
-    k_line_grad<float><<<grid_for(n_lines, blk), blk, 0, st>>>(pl->line_base.ptr, P.setup.ptr, P.g, pl->sbase.ptr,
-                                                               pl->partial.ptr, n_lines, grad_lines, P.flags.ptr);
+    k_line_grad<float><<<grid_for(n_lines * LG_LANES, blk), blk, 0, st>>>(
+        pl->line_base.ptr, P.setup.ptr, P.g, pl->sbase.ptr, pl->partial.ptr, n_lines, grad_lines, P.flags.ptr);
     CBX_HIP(hipGetLastError());
     return CBX_OK;
 }
