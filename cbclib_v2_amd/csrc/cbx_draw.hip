@@ -321,9 +321,9 @@ __device__ __forceinline__ int64_t list_index(int64_t tile, int64_t slot)
     return tile * LIST_SUBS + ((slot >> 6) & (LIST_SUBS - 1));
 }
 
-// Records per thread in the walk's LDS ring: 4 (40 KB per 256 threads) for
+// Records per thread in the walk's LDS ring: 4 (32 KB per 256 threads) for
 // launches of up to WALK_RING_LINES lines, whose waves are all resident at
-// once anyway; 2 (20 KB) above that, where more resident workgroups per CU
+// once anyway; 2 (16 KB) above that, where more resident workgroups per CU
 // matter more than the stores saved (measured on 128k and 972k lines).
 constexpr int64_t WALK_RING_LINES = 262144;
 
