@@ -280,13 +280,13 @@ def workload_c4(args):
     def step():
         lines.grad = None
         img = autograd.render_plan(plan, lines, idxs, kernel='biweight')
-        loss = ((img - target) ** 2).sum()
+        loss = torch.nn.functional.mse_loss(img, target, reduction="sum")   # sum((img - target)^2)
         loss.backward()
 
     def step_eager():
         lines.grad = None
         img = autograd.render(lines, (F, H, W), idxs, kernel='biweight')
-        loss = ((img - target) ** 2).sum()
+        loss = torch.nn.functional.mse_loss(img, target, reduction="sum")   # sum((img - target)^2)
         loss.backward()
 
     eager_ms = timed(step_eager, args.steps, args.warmup)
@@ -348,7 +348,7 @@ def workload_c5(args):
     def step():
         opt.zero_grad(set_to_none=True)
         img = autograd.render_plan(plan, lines, idxs, kernel='biweight')
-        loss = ((img - target) ** 2).sum()
+        loss = torch.nn.functional.mse_loss(img, target, reduction="sum")   # sum((img - target)^2)
         loss.backward()
         opt.step()
 
