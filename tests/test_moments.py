@@ -210,6 +210,42 @@ def test_p_values_match_oracle(cuda, oracle, dtype, shape, radii):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize('dtype', [np.float32, np.float64])
+def test_moments_many_regions_per_patch(cuda, oracle, dtype):
+    """The GPU pass pre-reduces each patch of 8 rows x 512 pixels in a table of
+    128 regions; a patch with more regions (here every other pixel of every
+    row: 2048 single-pixel regions per patch, beside long row-spanning ones)
+    sends the rest to the global sums directly.  Same outputs, and p-value
+    counts bit-exact."""
+    import torch
+    from cbclib_v2_amd import label as cl
+    H, W = 24, 1100
+    lab = np.zeros((H, W), np.int32)
+    n = 0
+    for y in range(H):
+        if y % 3 == 2:                      # a long run: one region spanning the row
+            n += 1
+            lab[y, :] = n
+            continue
+        xs = np.arange(y % 2, W, 2)         # isolated pixels: one region each
+        lab[y, xs] = np.arange(n + 1, n + 1 + len(xs))
+        n += len(xs)
+    rng = np.random.default_rng(9)
+    data = (rng.random((H, W)) * 4.0 - 1.0).astype(dtype)
+    print(f"many regions: {n} regions in {H}x{W}", flush=True)
+    index = np.arange(1, n + 1)
+    check_moments(oracle, lab, index, data, 'many regions per patch')
+    lines = oracle.region_moments('line_fit', lab, index, data)
+    res = cl.CPLabelResult(labels=torch.from_numpy(lab).cuda(), index=torch.from_numpy(index).cuda())
+    dt = np.dtype(dtype).type
+    ref, cnt = oracle.p_values(lab, index, lines, data, dt(0.3), dt(0.5), dt(1.5), counts=True)
+    got, gcnt = cl.p_values(res, torch.from_numpy(lines).cuda(), torch.from_numpy(data).cuda(), 0.3, 0.5, 1.5,
+                            counts=True)
+    assert np.array_equal(gcnt.cpu().numpy(), cnt)
+    np.testing.assert_allclose(got.cpu().numpy(), ref, rtol=1e-5 if dtype == np.float32 else 1e-12, atol=0)
+
+
+@pytest.mark.gpu
 def test_moments_numpy_label_result(cuda, oracle):
     """A NumPy LabelResult (the CPU label path's result type) gives NumPy outputs."""
     from cbclib_v2_amd import label as cl
