@@ -28,7 +28,7 @@ EXPORTS = ('cbx_last_error', 'cbx_version', 'cbx_kernel_id', 'cbx_draw_lines_f32
            'cbx_plan_check', 'cbx_plan_bytes', 'cbx_plan_destroy', 'cbx_label_noalloc', 'cbx_label_noalloc_capacity', 'cbx_region_moments_f32',
            'cbx_region_moments_f64', 'cbx_p_values_f32', 'cbx_p_values_f64', 'cbx_draw_lines3_f32',
            'cbx_draw_lines3_f64', 'cbx_accumulate_lines3_f32', 'cbx_accumulate_lines3_f64',
-           'cbx_binary_dilation')
+           'cbx_binary_dilation', 'cbx_exclusive_scan_i32', 'cbx_exclusive_scan_i64')
 MOMENTS = {'total_mass': 0, 'mean': 1, 'center_of_mass': 2, 'moment_of_inertia': 3, 'covariance_matrix': 4,
            'line_fit': 5}
 
@@ -108,6 +108,10 @@ def _declare(L: ctypes.CDLL) -> None:
     L.cbx_plan_create_f32.argtypes = [ctypes.POINTER(ctypes.c_void_p), I64, I64, I64, P, I64, P, P, I32, OP, D, P]
     L.cbx_plan_draw_f32.restype = I32
     L.cbx_plan_draw_f32.argtypes = [P, P, P, I64, P, P, F, I32, I32, P]
+    L.cbx_exclusive_scan_i32.restype = I32
+    L.cbx_exclusive_scan_i32.argtypes = [P, P, I64, I32, P]
+    L.cbx_exclusive_scan_i64.restype = I32
+    L.cbx_exclusive_scan_i64.argtypes = [P, P, I64, I32, P]
     L.cbx_plan_adjoint_f32.restype = I32
     L.cbx_plan_adjoint_f32.argtypes = [P, P, P, I64, P, F, I32, P, P]
     L.cbx_plan_check.restype = I32

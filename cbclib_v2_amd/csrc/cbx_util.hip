@@ -674,6 +674,29 @@ int64_t cbx_workspace_release(void) { return (int64_t)cbx::workspace_release(); 
 
 int64_t cbx_workspace_bytes(void) { return (int64_t)cbx::workspace_bytes(); }
 
+int cbx_exclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, int total, void *stream)
+{
+    if (n < 0 || (!in && n > 0) || (!out && (n > 0 || total))) return cbx::fail(CBX_EVALUE, "invalid scan arguments");
+    if (n + (total ? 1 : 0) >= (int64_t(1) << 31)) return cbx::fail(CBX_EVALUE, "scan longer than 2^31 - 1 entries");
+    hipStream_t st = (hipStream_t)stream;
+    cbx::Workspace ws(st);
+    const hipError_t e = total ? cbx::exclusive_scan_total_i32(in, nullptr, out, n, st)
+                               : cbx::exclusive_scan_i32(in, out, n, st);
+    if (e != hipSuccess) return cbx::fail(CBX_ERUNTIME, std::string("exclusive scan: ") + hipGetErrorString(e));
+    return CBX_OK;
+}
+
+int cbx_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int total, void *stream)
+{
+    if (n < 0 || (!in && n > 0) || (!out && (n > 0 || total))) return cbx::fail(CBX_EVALUE, "invalid scan arguments");
+    if (n + (total ? 1 : 0) >= (int64_t(1) << 31)) return cbx::fail(CBX_EVALUE, "scan longer than 2^31 - 1 entries");
+    hipStream_t st = (hipStream_t)stream;
+    cbx::Workspace ws(st);
+    const hipError_t e = total ? cbx::exclusive_scan_total_i64(in, out, n, st) : cbx::exclusive_scan_i64(in, out, n, st);
+    if (e != hipSuccess) return cbx::fail(CBX_ERUNTIME, std::string("exclusive scan: ") + hipGetErrorString(e));
+    return CBX_OK;
+}
+
 int cbx_kernel_id(const char *name)
 {
     static const char *names[5] = {"biweight", "gaussian", "parabolic", "rectangular", "triangular"};

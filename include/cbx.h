@@ -368,6 +368,17 @@ CBX_API int cbx_timing_read(const char *name, double *total_ms, int64_t *count);
 CBX_API int64_t cbx_workspace_release(void);
 CBX_API int64_t cbx_workspace_bytes(void);
 
+/*
+ * The device-wide exclusive prefix sum the entry points use (no reference
+ * counterpart; exported for tests and callers): out[i] = in[0] + ... +
+ * in[i-1] for i < n, one kernel (single pass, decoupled look-back) on
+ * `stream`; total != 0 also writes out[n] = the sum of all n (out holds
+ * n + 1 entries).  int32 sums wrap; int64 sums must stay within +-2^61.
+ * At most 2^31 - 1 outputs.
+ */
+CBX_API int cbx_exclusive_scan_i32(const int32_t *in, int32_t *out, int64_t n, int total, void *stream);
+CBX_API int cbx_exclusive_scan_i64(const int64_t *in, int64_t *out, int64_t n, int total, void *stream);
+
 #ifdef __cplusplus
 }
 #endif
