@@ -1,32 +1,35 @@
 // draw_lines on gfx950: segment checkpoints, tile-binned LDS accumulation,
-// pixel lists, the deterministic (reference-order) mode and the adjoint.
+// pixel lists, the deterministic (reference-order) mode, accumulate_lines,
+// plans and the adjoint.
 //
-// Data flow of one call (F frames of H x W, N lines):
+// Tile path (draw_lines, plans; F frames of H x W, N lines):
 //
-//  1. k_segments<pass 1>   one lane per line: builds the line's constants
-//                          (LineSetup) and replays the reference's walk
-//                          once, cutting it into segments of <= SEG
-//                          long-axis columns that lie inside the frame.  A
-//                          segment stores the exact walk state at its first
-//                          column (the checkpoint) and the short-axis pixel
-//                          range its fills can reach.  Counts segments per
-//                          line and (segment, tile) items per tile.
-//  2. exclusive scans      segment offsets per line, item offsets per tile.
-//  3. k_segments<pass 2>   same walk, writes segments and the per-tile item
-//                          lists.
-//  4. k_tiles              one workgroup per TILE x TILE pixel tile: waves
-//                          resume each item's walk from its checkpoint
-//                          (phase 1, one lane per item, column records into
-//                          LDS), then fill the columns (phase 2, one lane per
-//                          column), accumulating into an LDS tile with LDS
-//                          atomics; the tile is flushed to HBM once with
-//                          coalesced read-modify-write.  HBM traffic of the
-//                          image: one read and one write per touched tile.
+//  1. k_setup_lines   one lane per line: the line's constants (LineSetup) and
+//                     its segment bound; the bound total (one host read).
+//  2. k_walk_lines    one lane per line: replays the reference's walk once,
+//                     cutting it into segments of <= SEG long-axis columns
+//                     inside the frame.  A segment stores the exact walk
+//                     state at its first column (the checkpoint) and the
+//                     short-axis pixel range its fills can reach; the walk
+//                     counts (segment, tile) items per tile sub-list.
+//  3. k_scan          item offsets per tile sub-list (one host read: the
+//                     item total).
+//  4. k_bin_slots     one lane per segment slot: the per-tile item lists.
+//  5. k_tiles         one workgroup per TILE x TILE pixel tile: one wave
+//                     resumes each item's walk from its checkpoint (phase 1,
+//                     column records into LDS), then all waves fill the
+//                     columns (phase 2), accumulating into an LDS tile with
+//                     LDS atomics (fixed point for float sums); the tile is
+//                     flushed to HBM once with a coalesced read-modify-write.
+//                     HBM traffic of the image: one read and one write per
+//                     touched tile.
 //
-// The pixel-list and deterministic modes replace step 4 by an emission pass
-// over segments (k_emit) that writes one entry per emitted pixel in the
-// reference's emission order, a stable radix sort by pixel and an ordered
-// segmented reduction.
+// Ordered path (pixel lists, the deterministic mode, accumulate_lines with
+// different overlaps): k_segments walks twice (count, then write) so that a
+// line's segments are consecutive and lines keep their input order, and
+// k_tiles_ordered applies each pixel's terms in ascending line order, the
+// single-threaded reference's order.  The adjoint (k_adjoint_tiles,
+// k_line_grad) runs on the tile path's segments and lists.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
