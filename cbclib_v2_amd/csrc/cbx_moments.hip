@@ -6,15 +6,18 @@
 // Region j is the set of pixels whose label is index[j] (the first entry of
 // index holding a label owns its accumulators; later duplicates read them).
 //
-// One pass over the label map (k_region_accumulate): each lane takes a chunk
-// of MOM_CHUNK pixels of one row, folds consecutive pixels of equal label
-// into run sums (sum v, sum v x, sum v x^2 along the row) and, per run,
-// adds the region's raw moments in double (the other coordinates are
-// constant along a row) and lowers the region's origin (its smallest flat
-// index, integer atomicMin) -- into an LDS table of its workgroup's patch
-// of rows, which then adds each region's sums to the global accumulators
-// once.  Background chunks end after the label load.  k_region_finish then turns the raw sums into the
-// reference's quantities, in double, written in the data's type.
+// One pass over the label map (k_region_accumulate): one workgroup per patch
+// of MOM_PR rows x MOM_PC chunks; each lane takes a chunk of MOM_CHUNK
+// pixels of one row, folds consecutive pixels of equal label into run sums
+// (sum v, sum v x, sum v x^2 along the row) and, per run, adds the region's
+// raw moments in double (the other coordinates are constant along a row) and
+// lowers the region's origin (its smallest flat index, integer atomicMin) in
+// an LDS table of the patch's regions (a region that finds no entry goes to
+// the global accumulators directly); the table then adds each region's sums
+// to the global accumulators once.  Background chunks end after the label
+// load.  k_region_finish turns the raw sums into the reference's
+// quantities, in double, written in the data's type.  p_values counts
+// (k_region_count) take the same patch table with 32-bit counts.
 //
 // The reference accumulates in the data's type relative to the region's
 // first pixel (MomentsND::insert); here the sums are double and absolute, so
