@@ -719,13 +719,15 @@ __device__ __forceinline__ double fix_value(uint32_t acc, const uint32_t *zb, in
 
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-// fix_value for a pixel already in a register: o + acc * scale (or + zero,
-// the value of exact zeros, when only zeros touched it: zbit set), or o
-// untouched.
-__device__ __forceinline__ float fold_fix(float o, uint32_t acc, uint32_t zbit, double scale, double zero)
+// fix_value for a pixel already in a register, in float: o + (acc 2^-F)
+// max_val (or + zero, the value of exact zeros, when only zeros touched it:
+// zbit set), or o untouched.  acc 2^-F is exact (p2 = 2^-F); the rounding of
+// acc to 24 bits and of the two operations stays below 2e-7 of the pixel's
+// value, far inside the fixed-point terms' own error (<= 2^-F max_val each).
+__device__ __forceinline__ float fold_fix(float o, uint32_t acc, uint32_t zbit, float p2, float max_val, float zero)
 {
-    const double add = acc ? (double)acc * scale : zero;
-    return (acc | zbit) ? (float)((double)o + add) : o;
+    const float add = acc ? ((float)acc * p2) * max_val : zero;
+    return (acc | zbit) ? o + add : o;
 }
 
 // LDS row pitch of the adjoint's gradient tile, in words: 4 words of padding
@@ -958,7 +960,8 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
             if (PREF && full) {
                 // all LDS reads first (4 pixels and their zero bits per read),
                 // then branch-free updates
-                const double zero = (double)((float)max_val * 0.0f);
+                const float zero = (float)max_val * 0.0f;
+                const float p2 = __int_as_float((127 - F) << 23);   // 2^-F, F <= 31
                 uint4 acc[PV];
                 uint32_t zw[PV];
 #pragma unroll
@@ -970,10 +973,10 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
                 }
 #pragma unroll
                 for (int j = 0; j < PV; j++) {
-                    pre[j].x = fold_fix(pre[j].x, acc[j].x, zw[j] & 1u, scale, zero);
-                    pre[j].y = fold_fix(pre[j].y, acc[j].y, zw[j] & 2u, scale, zero);
-                    pre[j].z = fold_fix(pre[j].z, acc[j].z, zw[j] & 4u, scale, zero);
-                    pre[j].w = fold_fix(pre[j].w, acc[j].w, zw[j] & 8u, scale, zero);
+                    pre[j].x = fold_fix(pre[j].x, acc[j].x, zw[j] & 1u, p2, (float)max_val, zero);
+                    pre[j].y = fold_fix(pre[j].y, acc[j].y, zw[j] & 2u, p2, (float)max_val, zero);
+                    pre[j].z = fold_fix(pre[j].z, acc[j].z, zw[j] & 4u, p2, (float)max_val, zero);
+                    pre[j].w = fold_fix(pre[j].w, acc[j].w, zw[j] & 8u, p2, (float)max_val, zero);
                 }
             } else {
                 for (int i = tid; i < TILE * TILE; i += NT) {
