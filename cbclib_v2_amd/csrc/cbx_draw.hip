@@ -823,9 +823,16 @@ k_tiles(T *__restrict__ out, Geometry g, const int *__restrict__ tile_off, const
     }
 
     auto clear = [&]() {
-        for (int i = tid; i < LDS_PX; i += NT) {
-            if constexpr (OV == 0) img[i] = Acc(-0.0); // additive identity, keeps -0.0 pixels intact
-            else img[i] = Acc(0);                      // below every max code / empty fixed-point sum
+        if constexpr (OV != 0 && sizeof(Acc) == 4) {
+            // zero words (below every max code / empty fixed-point sum), 16 bytes per store
+            static_assert(LDS_PX % 4 == 0, "whole 16-byte stores");
+            uint4 *p = reinterpret_cast<uint4 *>(img);
+            for (int i = tid; i < LDS_PX / 4; i += NT) p[i] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            for (int i = tid; i < LDS_PX; i += NT) {
+                if constexpr (OV == 0) img[i] = Acc(-0.0); // additive identity, keeps -0.0 pixels intact
+                else img[i] = Acc(0);                      // below every max code
+            }
         }
         if constexpr (OV == 2)
             for (int i = tid; i < LDS_PX / 32; i += NT) zb[i] = 0u;
