@@ -117,6 +117,7 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
                                               int *__restrict__ flags, OnSeg on_seg)
 {
     const int a = L.axis, b = 1 - a;
+    const bool ax0 = a == 0;
     const int da = (a == 0) ? g.W : g.H, db = (a == 0) ? g.H : g.W;
     const int sa = pick(L.st, a);
     const double ta = (double)m_abs(pick(L.t, a));
@@ -207,12 +208,15 @@ __device__ __forceinline__ void walk_segments(const LineSetup<T> &L, int64_t l, 
             if ((k == kmax) | (pick(s.c, a) == c_end) | walk_done(L, s)) break;
             bool nx, ny;
             walk_next(L, s.Ne, nx, ny);
-            const bool em = (a == 0) ? nx : ny;
+            // (lane masks combined with the loop-invariant axis mask; a NaN
+            // |N| leaves maxN, as fmax would)
+            const bool em = (nx && ax0) || (ny && !ax0);
             const int sb = pick(s.c, b);
+            const T an = m_abs(s.Ne);
             nem += em ? 1 : 0;
             smin = em ? min(smin, sb) : smin;
             smax = em ? max(smax, sb) : smax;
-            maxN = em ? fmax(maxN, m_abs(s.Ne)) : maxN;
+            maxN = (em && an > maxN) ? an : maxN;
             walk_step(L, s, nx, ny);
         }
         it += k;
