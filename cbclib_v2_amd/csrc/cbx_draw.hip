@@ -334,19 +334,53 @@ __device__ __forceinline__ int64_t list_index(int64_t tile, int64_t slot)
 // matter more than the stores saved (measured on 128k and 972k lines).
 constexpr int64_t WALK_RING_LINES = 262144;
 
-template <typename T, int WALK_RING>
+template <typename T, int WALK_RING, bool SORT>
 __global__ void __launch_bounds__(256)
 k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
              int64_t slot_cap, unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt,
              int *__restrict__ flags, int64_t *__restrict__ line_base)
 {
-    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t l0 = (int64_t)blockIdx.x * blockDim.x;
     const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6, n_waves = (blockDim.x + 63) >> 6;
     LineSetup<T> L;
     int ub = 0;
+    int64_t l = l0 + threadIdx.x;
     if (l < n_lines) {
         L = setup[l];
         ub = segment_bound(L, g);
+    }
+    if constexpr (SORT) {
+        // A wave steps until its longest walk ends.  The workgroup's lines
+        // are ordered by segment bound (min(bound, 63); a stable LSD radix
+        // sort of one bit per pass, ballot splits), so that each wave walks
+        // lines of similar length; thread t then walks the t-th line.
+        __shared__ int s_ub[256], s_id[256], s_zero[4];
+        int id = threadIdx.x;
+        const uint64_t below = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int bit = 0; bit < 6; bit++) {
+            const bool one = (min(ub, 63) >> bit) & 1;
+            const uint64_t zm = __ballot(!one);
+            const int zb = __popcll(zm & below);
+            if (lane == 0) s_zero[wave] = __popcll(zm);
+            __syncthreads();
+            int zt = 0, zw = 0;
+            for (int w = 0; w < n_waves; w++) {
+                const int z = s_zero[w];
+                zw += (w < wave) ? z : 0;
+                zt += z;
+            }
+            // zeros keep their order in front, ones after them
+            const int pos = one ? zt + (wave * 64 - zw) + (lane - zb) : zw + zb;
+            s_ub[pos] = ub;
+            s_id[pos] = id;
+            __syncthreads();
+            ub = s_ub[threadIdx.x];
+            id = s_id[threadIdx.x];
+        }
+        l = l0 + id;
+        if (ub > 0) L = setup[l];
     }
     // wave prefix of the bounds, one slot reservation per wave
     int pre = ub;
@@ -360,7 +394,6 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
     // slots for the workgroup's waves with one global atomic (same-address
     // atomics serialise), handed out in wave order
     __shared__ unsigned long long wsum[4];
-    const int wave = threadIdx.x >> 6, n_waves = (blockDim.x + 63) >> 6;
     if (lane == 0) wsum[wave] = (unsigned long long)total;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -386,7 +419,10 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
     // stores: a store's data registers cannot be reused before the store
     // completes, so writing each segment from registers at its close would
     // stall every close on the previous store.  The tile counts are atomics
-    // whose results the walk does not use (k_bin_slots places the items).
+    // whose results the walk does not use (k_bin_slots places the items);
+    // their rate bounds the walk, so each lane adds up runs of items into
+    // the same list (consecutive segments of a line mostly share their
+    // tiles) in two pending counters and issues one atomic per run.
     // (ring words [slot][word][thread]: a record moves as 32-bit words with
     // constant indices, so it stays in registers)
     constexpr int NW = (int)(sizeof(Segment<T>) / 4);
@@ -409,6 +445,7 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
             }
         }
     };
+    int pend_a = -1, pend_b = -1, n_a = 0, n_b = 0;   // pending list counts (list index, items)
     walk_segments(L, l, g, flags, [&](const Segment<T> &cur, int tl) {
         uint32_t cw[NW];
         __builtin_memcpy(cw, &cur, sizeof(cw));
@@ -416,11 +453,25 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
 #pragma unroll
         for (int k = 0; k < NW; k++) ring[slot][k][t] = cw[k];
         const int tb0 = cur.lo / g.tile, tb1 = cur.hi / g.tile;
-        for (int tb = tb0; tb <= tb1; tb++)
-            atomicAdd(&tile_cnt[list_index(tile_index(g, L.frame, L.axis, tl, tb), base + nseg)], 1);
+        for (int tb = tb0; tb <= tb1; tb++) {
+            const int li = (int)list_index(tile_index(g, L.frame, L.axis, tl, tb), base + nseg);
+            if (li == pend_a) {
+                n_a++;
+            } else if (li == pend_b) {
+                n_b++;
+            } else {
+                if (n_b) atomicAdd(&tile_cnt[pend_b], n_b);
+                pend_b = pend_a;
+                n_b = n_a;
+                pend_a = li;
+                n_a = 1;
+            }
+        }
         nseg++;
         if (nseg % WALK_RING == 0) flush(nseg - WALK_RING, WALK_RING);
     });
+    if (n_a) atomicAdd(&tile_cnt[pend_a], n_a);
+    if (n_b) atomicAdd(&tile_cnt[pend_b], n_b);
     if (nseg % WALK_RING) {
         const int done = (nseg / WALK_RING) * WALK_RING;   // records flushed at closes
         flush(done, nseg - done);
@@ -1835,10 +1886,10 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     {
         KernelTimer timer("segments_walk", st);
         if (n_lines <= WALK_RING_LINES)
-            k_walk_lines<T, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(
+            k_walk_lines<T, 4, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
                 n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
         else
-            k_walk_lines<T, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(
+            k_walk_lines<T, 2, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
                 n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
     }
     CBX_HIP(hipGetLastError());
@@ -2569,11 +2620,11 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
     k_setup_lines<float, int32_t><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                          P.setup.ptr, pl->counters.ptr, P.flags.ptr);
     if (n_lines <= WALK_RING_LINES)
-        k_walk_lines<float, 4><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+        k_walk_lines<float, 4, true><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
                                                                        pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
                                                                        pl->line_base.ptr);
     else
-        k_walk_lines<float, 2><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+        k_walk_lines<float, 2, true><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
                                                                        pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
                                                                        pl->line_base.ptr);
     // the plan's own scan state (cleared by k_plan_reset above)
