@@ -11,7 +11,8 @@
 //                     inside the frame.  A segment stores the exact walk
 //                     state at its first column (the checkpoint) and the
 //                     short-axis pixel range its fills can reach; the walk
-//                     counts (segment, tile) items per tile sub-list.
+//                     counts (segment, tile) items per tile sub-list, one
+//                     atomic per run of items into one sub-list.
 //  3. k_scan          item offsets per tile sub-list (one host read: the
 //                     item total).
 //  4. k_bin_slots     one lane per segment slot: the per-tile item lists.
@@ -285,10 +286,11 @@ k_segments(const T *__restrict__ lines, const I *__restrict__ idxs, int kernel, 
 // order): one walk per line.
 //   k_setup_lines   line constants, flags, and the total of the per-line
 //                   segment bounds (counters[0]);
-//   k_walk_lines    each wave takes slots for its lines' bounds with one
-//                   atomic (counters[1]), walks, writes the segments into
-//                   them (unused slots get ncols = 0) and counts tile items
-//                   (atomics the walk does not wait for);
+//   k_walk_lines    orders the workgroup's lines by segment bound, takes
+//                   slots for their bounds with one atomic (counters[1]),
+//                   walks, writes the segments into them (unused slots get
+//                   ncols = 0) and counts tile items (atomics the walk does
+//                   not wait for);
 //   k_bin_slots     one lane per slot: writes the per-tile item lists, one
 //                   cursor atomic per (wave, tile).
 template <typename T, typename I>
