@@ -330,13 +330,12 @@ __device__ __forceinline__ int64_t list_index(int64_t tile, int64_t slot)
     return tile * LIST_SUBS + ((slot >> 6) & (LIST_SUBS - 1));
 }
 
-// Records per thread in the walk's LDS ring: 4 (32 KB per 256 threads) for
-// launches of up to WALK_RING_LINES lines, whose waves are all resident at
-// once anyway; 2 (16 KB) above that, where more resident workgroups per CU
-// matter more than the stores saved (measured on 128k and 972k lines).
-constexpr int64_t WALK_RING_LINES = 262144;
+// Records per thread in the walk's LDS ring: 2 (16 KB per 256 threads; a
+// ring of 4 gave no faster walk on 128k lines and fewer resident workgroups
+// on 972k lines).
+constexpr int WALK_RING = 2;
 
-template <typename T, int WALK_RING, bool SORT>
+template <typename T>
 __global__ void __launch_bounds__(256)
 k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup, Segment<T> *__restrict__ segs,
              int64_t slot_cap, unsigned long long *__restrict__ counters, int *__restrict__ tile_cnt,
@@ -352,7 +351,7 @@ k_walk_lines(int64_t n_lines, Geometry g, const LineSetup<T> *__restrict__ setup
         L = setup[l];
         ub = segment_bound(L, g);
     }
-    if constexpr (SORT) {
+    {
         // A wave steps until its longest walk ends.  The workgroup's lines
         // are ordered by segment bound (min(bound, 63); a stable LSD radix
         // sort of one bit per pass, ballot splits), so that each wave walks
@@ -1887,12 +1886,8 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     CBX_HIP(P.segs.alloc(P.n_segs, st));
     {
         KernelTimer timer("segments_walk", st);
-        if (n_lines <= WALK_RING_LINES)
-            k_walk_lines<T, 4, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
-                n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
-        else
-            k_walk_lines<T, 2, true><<<grid_for(n_lines, blk), blk, 0, st>>>(
-                n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs, counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
+        k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
     }
     CBX_HIP(hipGetLastError());
     // list offsets (tile_off[n_lists]: the item total)
@@ -2621,14 +2616,9 @@ static int plan_draw(cbx_plan *pl, float *out, const float *lines, int64_t n_lin
                                                          (int64_t)pl->scan_state.n);
     k_setup_lines<float, int32_t><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                          P.setup.ptr, pl->counters.ptr, P.flags.ptr);
-    if (n_lines <= WALK_RING_LINES)
-        k_walk_lines<float, 4, true><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                       pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
-                                                                       pl->line_base.ptr);
-    else
-        k_walk_lines<float, 2, true><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                       pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
-                                                                       pl->line_base.ptr);
+    k_walk_lines<float><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
+                                                                pl->counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
+                                                                pl->line_base.ptr);
     // the plan's own scan state (cleared by k_plan_reset above)
     CBX_HIP(scan_i32_with_state(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, n_lists, n_lists + 1, pl->scan_state.ptr,
                                 st));
