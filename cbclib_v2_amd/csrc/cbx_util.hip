@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <chrono>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -274,20 +276,33 @@ struct MailSrc {
     int n;
 };
 
-__global__ void k_mail(MailSrc s, uint64_t *dst)
+// Writes the values into dst[0 .. n), then the call's sequence number into
+// dst[4] with a system-scope release, after every value is visible to the
+// host.
+__global__ void k_mail(MailSrc s, uint64_t *dst, uint64_t seq)
 {
     const int i = threadIdx.x;
-    if (i < s.n)
+    if (i < s.n) {
         dst[i] = (s.bytes[i] == 8) ? *static_cast<const uint64_t *>(s.p[i])
                                    : (uint64_t) * static_cast<const uint32_t *>(s.p[i]);
+        __threadfence_system();
+    }
+    __syncthreads();
+    if (i == 0) __hip_atomic_store(dst + 4, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
-// One pinned, mapped, coherent mailbox of 4 values per host thread and device
-// (a host thread runs one call at a time; kept until the process ends).
+// One pinned, mapped, coherent mailbox of 4 values and a sequence number per
+// host thread and device (a host thread runs one call at a time; kept until
+// the process ends).
 struct Mailbox {
     uint64_t *host = nullptr;
     uint64_t *dev = nullptr;
+    uint64_t seq = 0;
 };
+
+// How long read_device_values polls the mailbox before it waits for the
+// stream instead (a long queue of work ahead of the read, or a failed launch).
+constexpr auto MAIL_SPIN = std::chrono::milliseconds(20);
 
 hipError_t read_device_values(hipStream_t s, int n, const void *const *src, const int *bytes, uint64_t *out)
 {
@@ -313,7 +328,7 @@ hipError_t read_device_values(hipStream_t s, int n, const void *const *src, cons
     Mailbox &mb = boxes[dev];
     if (!mb.host) {
         void *h = nullptr;
-        e = hipHostMalloc(&h, 4 * sizeof(uint64_t), hipHostMallocMapped | hipHostMallocCoherent);
+        e = hipHostMalloc(&h, 8 * sizeof(uint64_t), hipHostMallocMapped | hipHostMallocCoherent);
         if (e != hipSuccess) return e;
         void *d = nullptr;
         e = hipHostGetDevicePointer(&d, h, 0);
@@ -321,6 +336,7 @@ hipError_t read_device_values(hipStream_t s, int n, const void *const *src, cons
             (void)hipHostFree(h);
             return e;
         }
+        std::memset(h, 0, 8 * sizeof(uint64_t));   // no sequence number yet (they start at 1)
         mb.host = static_cast<uint64_t *>(h);
         mb.dev = static_cast<uint64_t *>(d);
     }
@@ -331,12 +347,31 @@ hipError_t read_device_values(hipStream_t s, int n, const void *const *src, cons
         ms.bytes[i] = bytes[i];
     }
     ms.n = n;
-    k_mail<<<1, 64, 0, s>>>(ms, mb.dev);
+    const uint64_t seq = ++mb.seq;
+    k_mail<<<1, 64, 0, s>>>(ms, mb.dev, seq);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
+    // The values are final once the sequence number arrives (the stream runs
+    // in order, so everything before the read has completed): polling the
+    // coherent mailbox wakes the host sooner than a stream synchronisation.
+    // Past MAIL_SPIN the stream is waited for, which also reports errors.
     const volatile uint64_t *h = mb.host;
+    bool arrived = false;
+    const auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t it = 0;; it++) {
+        if (h[4] == seq) {
+            arrived = true;
+            break;
+        }
+        if ((it & 255u) == 255u && std::chrono::steady_clock::now() - t0 > MAIL_SPIN) break;
+        __builtin_ia32_pause();
+    }
+    if (arrived) {
+        std::atomic_thread_fence(std::memory_order_acquire);
+    } else {
+        e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+    }
     for (int i = 0; i < n; i++) out[i] = h[i];
     return hipSuccess;
 }
