@@ -34,6 +34,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <climits>
 #include <cstddef>
 #include <cmath>
@@ -1831,6 +1832,9 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int kern
     return CBX_OK;
 }
 
+// Slot total of the last eager unordered plan (a capacity hint for the next).
+static std::atomic<int64_t> g_slot_hint{0};
+
 // Unordered plan for the tile path: one walk per line (k_setup_lines,
 // k_walk_lines, k_bin_slots).  Segments live in slots (n_segs = slots; unused
 // slots have ncols = 0 and are in no item list).  Two host syncs: the slot
@@ -1870,36 +1874,60 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     k_setup_lines<T, I><<<grid_for(n_lines, blk), blk, 0, st>>>(lines, idxs, kernels, kernel, n_lines, g,
                                                                P.setup.ptr, counters.ptr, P.flags.ptr);
     CBX_HIP(hipGetLastError());
-    uint64_t hv[2];
-    {
+    uint64_t hv[3];
+    int fl = 0, rc = 0;
+    // Slot capacity: the previous call's slot total when it is not far above
+    // what these lines could need, so the walk starts without reading this
+    // call's total first; a walk that runs out of slots (FLAG_OVERFLOW with a
+    // larger total) is cleared and run again at the exact total.  Without a
+    // usable hint the total is read first.
+    int64_t cap = g_slot_hint.load(std::memory_order_relaxed);
+    if (cap <= 0 || cap > 64 * n_lines + 65536) {
         const void *src[2] = {counters.ptr, P.flags.ptr};
         const int nb[2] = {8, 4};
         CBX_HIP(read_device_values(st, 2, src, nb, hv));
+        fl = (int)(uint32_t)hv[1];
+        if ((rc = flags_to_status(fl))) return rc;
+        if (hv[0] >= (1ull << 31)) return fail(CBX_EVALUE, "too many segments (2^31)");
+        cap = (int64_t)hv[0];
+        if (cap == 0) {
+            P.n_segs = 0;
+            return CBX_OK;
+        }
     }
-    const unsigned long long slots = hv[0];
-    int fl = (int)(uint32_t)hv[1];
-    int rc = flags_to_status(fl);
-    if (rc) return rc;
-    if (slots >= (1ull << 31)) return fail(CBX_EVALUE, "too many segments (2^31)");
-    P.n_segs = (int64_t)slots;
+    int64_t slots = 0;
+    for (int attempt = 0;; attempt++) {
+        CBX_HIP(P.segs.alloc(cap, st));
+        {
+            KernelTimer timer("segments_walk", st);
+            k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, cap,
+                                                                    counters.ptr, P.tile_cnt.ptr, P.flags.ptr,
+                                                                    line_base);
+        }
+        CBX_HIP(hipGetLastError());
+        // list offsets (tile_off[n_lists]: the item total)
+        CBX_HIP(exclusive_scan_total_i32(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, n_lists, st));
+        {
+            const void *src[3] = {P.tile_off.ptr + n_lists, P.flags.ptr, counters.ptr};
+            const int nb[3] = {4, 4, 8};
+            CBX_HIP(read_device_values(st, 3, src, nb, hv));
+        }
+        fl = (int)(uint32_t)hv[1];
+        if ((rc = flags_to_status(fl & ~FLAG_OVERFLOW))) return rc;
+        if (hv[2] >= (1ull << 31)) return fail(CBX_EVALUE, "too many segments (2^31)");
+        slots = (int64_t)hv[2];
+        if (!(fl & FLAG_OVERFLOW)) break;
+        if (attempt > 0 || slots <= cap) return flags_to_status(fl);   // cannot happen: the total fits
+        // too few slots: clear what the walk counted and walk again
+        cap = slots;
+        void *const zp[4] = {P.flags.ptr, counters.ptr + 1, P.tile_cnt.ptr, P.tile_cur.ptr};
+        const int64_t zw[4] = {1, 2, n_lists, n_lists};
+        CBX_HIP(zero_words(st, 4, zp, zw));
+    }
+    g_slot_hint.store(slots, std::memory_order_relaxed);
+    P.n_segs = slots;
     if (P.n_segs == 0) return CBX_OK;
-    CBX_HIP(P.segs.alloc(P.n_segs, st));
-    {
-        KernelTimer timer("segments_walk", st);
-        k_walk_lines<T><<<grid_for(n_lines, blk), blk, 0, st>>>(n_lines, g, P.setup.ptr, P.segs.ptr, P.n_segs,
-                                                                counters.ptr, P.tile_cnt.ptr, P.flags.ptr, line_base);
-    }
-    CBX_HIP(hipGetLastError());
-    // list offsets (tile_off[n_lists]: the item total)
-    CBX_HIP(exclusive_scan_total_i32(P.tile_cnt.ptr, nullptr, P.tile_off.ptr, n_lists, st));
-    {
-        const void *src[2] = {P.tile_off.ptr + n_lists, P.flags.ptr};
-        const int nb[2] = {4, 4};
-        CBX_HIP(read_device_values(st, 2, src, nb, hv));
-    }
     const int n_items = (int)(uint32_t)hv[0];
-    fl = (int)(uint32_t)hv[1];
-    if ((rc = flags_to_status(fl))) return rc;
     if (n_items < 0) return fail(CBX_ERUNTIME, "tile item count overflow");
     P.n_items = n_items;
     CBX_HIP(P.pairs.alloc(P.n_items, st));

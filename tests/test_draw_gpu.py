@@ -123,9 +123,8 @@ def test_image_tiles(cuda, oracle, dtype, overlap, kernel, tile, seg, threads, c
 
 
 def test_many_lines_parity(cuda, oracle):
-    """More than 262144 lines: the segment walk then uses its smaller LDS ring
-    (two records per thread).  300000 short lines over 4 frames of 256^2,
-    gaussian sum and rectangular max, against the oracle."""
+    """300000 short lines over 4 frames of 256^2 (1172 workgroups of the
+    segment walk), gaussian sum and rectangular max, against the oracle."""
     rng = np.random.default_rng(7)
     n, F, H, W = 300_000, 4, 256, 256
     p0 = rng.uniform(-4, 260, (n, 2))
@@ -144,6 +143,22 @@ def test_many_lines_parity(cuda, oracle):
             assert np.array_equal(got, ref)
         else:
             np.testing.assert_allclose(got, ref, rtol=RTOL, atol=ATOL)
+        assert np.array_equal(got != 0, ref != 0)
+
+
+def test_slot_capacity_from_previous_call(cuda, oracle):
+    """The walk's slot capacity is the previous call's slot total: a call
+    that needs more runs out of slots, is cleared and walks again.  Calls
+    of 30, 3000 and again 30 lines (long lines into 2 frames of 512^2, and
+    short ones), each against the oracle."""
+    F, H, W = 2, 512, 512
+    for n, lmax, seed in ((30, 20.0, 11), (3000, 400.0, 12), (30, 20.0, 13), (3000, 400.0, 14)):
+        rng = np.random.default_rng(seed)
+        lines = random_segments(rng, n, H, W, 1.0, lmax, 0.5, 5.0, pad=10.0).astype(np.float32)
+        idxs = rng.integers(0, F, n).astype(np.int32)
+        ref = oracle.draw_lines(np.zeros((F, H, W), np.float32), lines, idxs, 1.0, 'gaussian', 'sum')
+        got = gpu_image((F, H, W), lines, idxs, 'gaussian', 'sum', np.float32, max_val=1.0)
+        np.testing.assert_allclose(got, ref, rtol=RTOL, atol=ATOL, err_msg=f"{n} lines")
         assert np.array_equal(got != 0, ref != 0)
 
 
