@@ -5,7 +5,9 @@
 // Tile path (draw_lines, plans; F frames of H x W, N lines):
 //
 //  1. k_setup_lines   one lane per line: the line's constants (LineSetup) and
-//                     its segment bound; the bound total (one host read).
+//                     its segment bound; the bound total (the slot count;
+//                     an eager draw reads it first only when the previous
+//                     call's total is no usable capacity).
 //  2. k_walk_lines    one lane per line: replays the reference's walk once,
 //                     cutting it into segments of <= SEG long-axis columns
 //                     inside the frame.  A segment stores the exact walk
@@ -14,7 +16,7 @@
 //                     counts (segment, tile) items per tile sub-list, one
 //                     atomic per run of items into one sub-list.
 //  3. k_scan          item offsets per tile sub-list (one host read: the
-//                     item total).
+//                     item total, flags and slot count; plans read nothing).
 //  4. k_bin_slots     one lane per segment slot: the per-tile item lists.
 //  5. k_tiles         one workgroup per TILE x TILE pixel tile: one wave
 //                     resumes each item's walk from its checkpoint (phase 1,
@@ -1837,8 +1839,9 @@ static std::atomic<int64_t> g_slot_hint{0};
 
 // Unordered plan for the tile path: one walk per line (k_setup_lines,
 // k_walk_lines, k_bin_slots).  Segments live in slots (n_segs = slots; unused
-// slots have ncols = 0 and are in no item list).  Two host syncs: the slot
-// total, then the item total.
+// slots have ncols = 0 and are in no item list).  One host read after the
+// walk (the item total, flags, the slot total), one more before it when the
+// previous call's slot total is no usable capacity.
 template <typename T, typename I>
 static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs, const uint8_t *kernels,
                                 int kernel, int64_t n_lines, int64_t n_frames, int64_t H, int64_t W, int tile,
