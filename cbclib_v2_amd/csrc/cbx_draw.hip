@@ -1834,7 +1834,8 @@ static int build_plan(SegmentPlan<T> &P, const T *lines, const I *idxs, int kern
     return CBX_OK;
 }
 
-// Slot total of the last eager unordered plan (a capacity hint for the next).
+// Largest slot total of the eager unordered plans so far (a capacity hint
+// for the next).
 static std::atomic<int64_t> g_slot_hint{0};
 
 // Unordered plan for the tile path: one walk per line (k_setup_lines,
@@ -1879,11 +1880,11 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
     CBX_HIP(hipGetLastError());
     uint64_t hv[3];
     int fl = 0, rc = 0;
-    // Slot capacity: the previous call's slot total when it is not far above
-    // what these lines could need, so the walk starts without reading this
-    // call's total first; a walk that runs out of slots (FLAG_OVERFLOW with a
-    // larger total) is cleared and run again at the exact total.  Without a
-    // usable hint the total is read first.
+    // Slot capacity: the largest slot total of the previous calls when it is
+    // not far above what these lines could need, so the walk starts without
+    // reading this call's total first; a walk that runs out of slots
+    // (FLAG_OVERFLOW with a larger total) is cleared and run again at the
+    // exact total.  Without a usable hint the total is read first.
     int64_t cap = g_slot_hint.load(std::memory_order_relaxed);
     if (cap <= 0 || cap > 64 * n_lines + 65536) {
         const void *src[2] = {counters.ptr, P.flags.ptr};
@@ -1927,7 +1928,10 @@ static int build_plan_unordered(SegmentPlan<T> &P, const T *lines, const I *idxs
         const int64_t zw[4] = {1, 2, n_lists, n_lists};
         CBX_HIP(zero_words(st, 4, zp, zw));
     }
-    g_slot_hint.store(slots, std::memory_order_relaxed);
+    // the largest total so far: calls of varying sizes rarely walk twice
+    for (int64_t h = g_slot_hint.load(std::memory_order_relaxed);
+         h < slots && !g_slot_hint.compare_exchange_weak(h, slots, std::memory_order_relaxed);) {
+    }
     P.n_segs = slots;
     if (P.n_segs == 0) return CBX_OK;
     const int n_items = (int)(uint32_t)hv[0];

@@ -146,20 +146,42 @@ def test_many_lines_parity(cuda, oracle):
         assert np.array_equal(got != 0, ref != 0)
 
 
-def test_slot_capacity_from_previous_call(cuda, oracle):
-    """The walk's slot capacity is the previous call's slot total: a call
-    that needs more runs out of slots, is cleared and walks again.  Calls
-    of 30, 3000 and again 30 lines (long lines into 2 frames of 512^2, and
-    short ones), each against the oracle."""
-    F, H, W = 2, 512, 512
-    for n, lmax, seed in ((30, 20.0, 11), (3000, 400.0, 12), (30, 20.0, 13), (3000, 400.0, 14)):
-        rng = np.random.default_rng(seed)
-        lines = random_segments(rng, n, H, W, 1.0, lmax, 0.5, 5.0, pad=10.0).astype(np.float32)
-        idxs = rng.integers(0, F, n).astype(np.int32)
-        ref = oracle.draw_lines(np.zeros((F, H, W), np.float32), lines, idxs, 1.0, 'gaussian', 'sum')
-        got = gpu_image((F, H, W), lines, idxs, 'gaussian', 'sum', np.float32, max_val=1.0)
-        np.testing.assert_allclose(got, ref, rtol=RTOL, atol=ATOL, err_msg=f"{n} lines")
-        assert np.array_equal(got != 0, ref != 0)
+SLOT_SEQUENCE = r"""
+import sys
+import numpy as np
+import torch
+sys.path.insert(0, sys.argv[1])
+from cbclib_v2_amd import ndimage
+from oracle import oracle as O
+from tests.lines_gen import random_segments
+F, H, W = 2, 512, 512
+for n, lmax, seed in ((30, 20.0, 11), (3000, 400.0, 12), (30, 20.0, 13), (3000, 400.0, 14)):
+    rng = np.random.default_rng(seed)
+    lines = random_segments(rng, n, H, W, 1.0, lmax, 0.5, 5.0, pad=10.0).astype(np.float32)
+    idxs = rng.integers(0, F, n).astype(np.int32)
+    ref = O.draw_lines(np.zeros((F, H, W), np.float32), lines, idxs, 1.0, 'gaussian', 'sum')
+    out = torch.zeros((F, H, W), dtype=torch.float32, device='cuda')
+    ndimage.draw_lines(out, torch.from_numpy(lines).cuda(), torch.from_numpy(idxs).cuda(), 1.0, 'gaussian', 'sum')
+    got = out.cpu().numpy()
+    np.testing.assert_allclose(got, ref, rtol=1e-5, atol=1e-6, err_msg=f"{n} lines")
+    assert np.array_equal(got != 0, ref != 0), f"{n} lines: support"
+    print(n, "lines ok", flush=True)
+"""
+
+
+def test_slot_capacity_from_previous_call(cuda):
+    """An eager draw walks at the largest slot total of the previous calls; a
+    call that needs more runs out of slots, is cleared and walks again.  In a
+    fresh process (no previous calls) draws of 30 short lines, 3000 long ones
+    (more slots than the first call's total: a second walk), 30 and 3000
+    again, into 2 frames of 512^2, each against the oracle."""
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run([sys.executable, '-c', SLOT_SEQUENCE, root], capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert r.stdout.count("lines ok") == 4
 
 
 def test_config1_parity(cuda, oracle):
