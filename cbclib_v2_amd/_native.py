@@ -185,9 +185,10 @@ def overlap_id(overlap: str) -> int:
     return OVERLAPS[overlap]
 
 
-def timing_enable(on: bool = True) -> None:
-    """Record HIP events around the library's main kernels (see cbx_timing_read)."""
-    lib().cbx_timing_enable(1 if on else 0)
+def timing_enable(on: bool = True, tiles_only: bool = False) -> None:
+    """Record HIP events around the library's main kernels (see cbx_timing_read);
+    ``tiles_only``: around the tile kernels only (``draw_tiles``, ``adjoint_tiles``, ...)."""
+    lib().cbx_timing_enable((2 if tiles_only else 1) if on else 0)
 
 
 def timing_reset() -> None:

@@ -114,7 +114,7 @@ struct DevBuf {
 
 // Optional per-kernel timing with HIP events recorded on the launch stream
 // (enabled by cbx_timing_enable); read back by cbx_timing_read.
-bool timing_on();
+bool timing_on(const char *name);
 void timing_record(const char *name, hipEvent_t begin, hipEvent_t end);
 
 struct KernelTimer {
@@ -124,7 +124,7 @@ struct KernelTimer {
     // (a null name times nothing: for calls that may be captured in a graph)
     KernelTimer(const char *n, hipStream_t s) : name(n), stream(s)
     {
-        if (!n || !timing_on() || no_alloc_mode()) return;
+        if (!n || !timing_on(n) || no_alloc_mode()) return;
         if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
             a = b = nullptr;
             return;

@@ -29,7 +29,7 @@ int fail(int code, const std::string &msg)
 
 struct TimingState {
     std::mutex mu;
-    bool on = false;
+    int on = 0;   // 0: off, 1: every timer, 2: the tile kernels' timers only
     struct Pending {
         std::string name;
         hipEvent_t a, b;
@@ -44,7 +44,11 @@ static TimingState &timing()
     return t;
 }
 
-bool timing_on() { return timing().on; }
+bool timing_on(const char *name)
+{
+    const int on = timing().on;
+    return on == 1 || (on == 2 && std::strstr(name, "_tiles") != nullptr);
+}
 
 void timing_record(const char *name, hipEvent_t a, hipEvent_t b)
 {
@@ -683,7 +687,7 @@ void cbx_timing_enable(int on)
 {
     cbx::TimingState &t = cbx::timing();
     std::lock_guard<std::mutex> g(t.mu);
-    t.on = on != 0;
+    t.on = (on == 0 || on == 2) ? on : 1;
 }
 
 void cbx_timing_reset(void)

@@ -350,7 +350,9 @@ CBX_API void cbx_label_force_path(int path);
  * stream around the library's main kernels ("draw_tiles",
  * "adjoint_tiles", "segments_pass1", "label_union", ...).  cbx_timing_read
  * waits for the recorded events and returns the accumulated milliseconds and
- * the number of launches since the last reset.
+ * the number of launches since the last reset.  cbx_timing_enable: 0 off,
+ * 2 only the tile kernels' timers (names ending in "_tiles": the fewest
+ * events in a timed loop), any other value every timer.
  */
 CBX_API void cbx_timing_enable(int on);
 CBX_API void cbx_timing_reset(void);
